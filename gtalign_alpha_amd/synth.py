@@ -1,0 +1,126 @@
+"""Seeded synthetic workloads (the benchmark configurations of the stage).
+
+Chains come from gta_synth_chain (csrc/synth.cpp): 5-20 residue helix,
+strand and coil segments inside a sphere of radius 2.2*L^0.38 A.  Homologs
+come from gta_synth_homolog: a source chain with deletions, short coil
+insertions, Gaussian noise, a random rotation and a translation.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from .chunk import Chunk
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+SYNTH_LIB = os.path.join(_HERE, "libgta_synth.so")
+AA = np.frombuffer(b"ACDEFGHIKLMNPQRSTVWY", dtype=np.uint8)
+BASE_SEED = 20261202
+
+_lib = None
+
+
+def _synth():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(SYNTH_LIB):
+            raise RuntimeError("%s is not built; run __graft_entry__.build()" % SYNTH_LIB)
+        _lib = C.CDLL(SYNTH_LIB)
+        _lib.gta_synth_chain.argtypes = [C.c_uint64, C.c_int, C.c_void_p]
+        _lib.gta_synth_chain.restype = None
+        _lib.gta_synth_homolog.argtypes = [C.c_uint64, C.c_void_p, C.c_int, C.c_float, C.c_float,
+                                           C.c_float, C.c_float, C.c_void_p, C.c_int, C.c_void_p]
+        _lib.gta_synth_homolog.restype = C.c_int
+    return _lib
+
+
+def chain(seed, L):
+    out = np.empty((L, 3), dtype=np.float32)
+    _synth().gta_synth_chain(C.c_uint64(seed), int(L), out.ctypes.data)
+    return out
+
+
+def homolog(seed, src, sigma, maxtrans, delfrac, insfrac):
+    src = np.ascontiguousarray(src, dtype=np.float32)
+    cap = 2 * len(src) + 8
+    out = np.empty((cap, 3), dtype=np.float32)
+    idx = np.empty(cap, dtype=np.int32)
+    n = _synth().gta_synth_homolog(C.c_uint64(seed), src.ctypes.data, len(src), float(sigma),
+                                   float(maxtrans), float(delfrac), float(insfrac), out.ctypes.data,
+                                   cap, idx.ctypes.data)
+    return out[:n].copy(), idx[:n].copy()
+
+
+def lognormal_lengths(rng, n, median, sigma, lo, hi):
+    return np.clip(np.round(np.exp(rng.normal(np.log(median), sigma, n))), lo, hi).astype(np.int64)
+
+
+def residues(rng, L):
+    return AA[rng.integers(0, len(AA), L)].tobytes().decode()
+
+
+def _hom_residues(rng, src_res, idx):
+    out = np.frombuffer(residues(rng, len(idx)).encode(), dtype=np.uint8).copy()
+    src = np.frombuffer(src_res.encode(), dtype=np.uint8)
+    keep = idx >= 0
+    out[keep] = src[idx[keep]]
+    return out.tobytes().decode()
+
+
+def make_set(seed, n, lens, queries, qres, hom_frac, sigma_lo, sigma_hi, del_hi, ins_hi,
+             maxtrans=50.0):
+    """n reference chains; a fraction hom_frac are homologs of a random query."""
+    rng = np.random.default_rng(seed)
+    structs, res, src = [], [], np.full(n, -1, dtype=np.int64)
+    for i in range(n):
+        s = int(rng.integers(0, 2**62))
+        if len(queries) and rng.random() < hom_frac:
+            qi = int(rng.integers(0, len(queries)))
+            sig = rng.uniform(sigma_lo, sigma_hi)
+            xyz, idx = homolog(s, queries[qi], sig, rng.uniform(0, maxtrans),
+                               rng.uniform(0, del_hi), rng.uniform(0, ins_hi))
+            structs.append(xyz)
+            res.append(_hom_residues(rng, qres[qi], idx))
+            src[i] = qi
+        else:
+            L = int(lens[i])
+            structs.append(chain(s, L))
+            res.append(residues(rng, L))
+    return structs, res, src
+
+
+def config(index, scale=1.0, seed=None):
+    """Query and reference chunks of one benchmark configuration.
+
+    index 0: 1 query of 150 vs 256 refs U(50,300), 30% homologs (sigma 1, 0-10% deletions)
+    index 1: 16 queries x 50,000 refs, lognormal(250, 0.5) in [30,1000], 20% homologs
+    index 2: 1 query of 400 x 500,000 refs, lognormal(220, 0.7) in [20,4000], 10% homologs
+    index 4: all-vs-all family set, lognormal(200, 0.5) in [40,800] (chunked by the caller)
+    `scale` shrinks the number of references (tests and bounded CPU samples).
+    Returns (queries, references, homolog-source index per reference).
+    """
+    seed = BASE_SEED + index if seed is None else seed
+    rng = np.random.default_rng(seed)
+    if index == 0:
+        nq, nr = 1, max(1, int(round(256 * scale)))
+        qlens = np.array([150])
+        rlens = rng.integers(50, 301, nr)
+        hom, slo, shi, dhi, ihi = 0.3, 1.0, 1.0, 0.10, 0.0
+    elif index == 1:
+        nq, nr = 16, max(1, int(round(50000 * scale)))
+        qlens = lognormal_lengths(rng, nq, 250, 0.5, 30, 1000)
+        rlens = lognormal_lengths(rng, nr, 250, 0.5, 30, 1000)
+        hom, slo, shi, dhi, ihi = 0.2, 0.5, 3.0, 0.075, 0.075
+    elif index == 2:
+        nq, nr = 1, max(1, int(round(500000 * scale)))
+        qlens = np.array([400])
+        rlens = lognormal_lengths(rng, nr, 220, 0.7, 20, 4000)
+        hom, slo, shi, dhi, ihi = 0.1, 0.5, 3.0, 0.075, 0.075
+    else:
+        raise ValueError("unsupported configuration %d" % index)
+    qs = [chain(int(rng.integers(0, 2**62)), int(L)) for L in qlens]
+    qr = [residues(rng, int(L)) for L in qlens]
+    rs, rr, src = make_set(int(rng.integers(0, 2**62)), nr, rlens, qs, qr, hom, slo, shi, dhi, ihi)
+    qc = Chunk.from_structures(qs, qr)
+    rc = Chunk.from_structures(rs, rr)
+    return qc, rc, src[rc.order]

@@ -1,0 +1,1396 @@
+// gta_oracle.cpp -- CPU restatement of GTalign's structure-alignment stage.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the oracle the HIP implementation
+// is checked against; nothing in the product links or calls it.  It restates
+// the reference CPU path (libmymp) in plain single-threaded FP32 C++, pair by
+// pair, following:
+//   MpBatch::ProcessBlock             src/libmymp/mpbatch/MpBatch.cpp:208-555
+//   MpStage1 (Run, kernels)           src/libmymp/mpstage1/MpStage1.h:78-85,
+//                                     MpStage1.cpp:28-1255
+//   MpStage2::RunSpecialized / Align  src/libmymp/mpstage1/MpStage2.h:60-131
+//   MpStageFin (Run, Align, Refine,   src/libmymp/mpstage1/MpStageFin.h:66-470,
+//     ProduceOutputScores, kernels)   MpStageFin.cpp:44-1500
+//   MpStageBase helpers               src/libmymp/mpstage1/MpStageBase.h:256-1101
+//   MpDPHub DP / backtrace / output   src/libmymp/mpdp/MpDPHub.cpp:73-388,687-984,1500-2142
+//   MpSecStr (protein)                src/libmymp/mpss/MpSecStr.cpp:20-110, MpSecStr.h:150-180
+//   Kabsch primitives                 src/libmymp/mpstages/transformbase.h:167-583
+//   d0/d8 constants                   src/libmycu/custages/stagecnsts.cuh:48-175
+//   fragment geometry                 src/libmycu/custages/fragment.cuh:19-109
+//
+// Pairs are independent: every per-pair quantity of the reference lives in
+// per-(query,reference) slots of its working memory, and the only chunk-wide
+// inputs are the grid sizes (seed range, number of refinement slots), which
+// are computed here from the chunk exactly as the reference does.  Slot ties
+// are resolved towards the lowest slot index, which is what the reference CPU
+// path does with one thread (and what the GPU path does, scoring.cu:858-875).
+//
+// Compile with -ffp-contract=off: the DP recurrence must round like the
+// reference's scalar code for alignments to be bit-exact.
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <cstdint>
+#include <vector>
+#include <algorithm>
+#include <string>
+
+#include "gta/gtalign_amd.h"
+
+namespace {
+
+// ---- constants (reference file:line in comments) --------------------------
+constexpr float CP_LARGEDST = 999999.9f;          // stagecnsts.cuh:17
+constexpr float CP_LARGEDST_cmp = 399999.9f;      // stagecnsts.cuh:19
+constexpr float D02s_PROC_INC = 1.0f;             // stagecnsts.cuh:32
+constexpr float RM_CONVEPSILON = 1.e-5f;          // stagecnsts.cuh:42
+constexpr float SCORE_CONVEPSILON = 1.e-6f;       // scoringbase.h:36
+constexpr float TFMEPSILON = 1.e-8f;              // transformbase.h:16
+constexpr float TFMTOL = 0.01f;                   // transformbase.h:17
+constexpr int FRAGREF_NMAXSUBFRAGS = 6;           // mpprocconfbase.h:28
+constexpr int FRAGREF_SFRAGSTEP = 40;             // mpprocconfbase.h:30
+constexpr int FRAGREF_SFRAGSTEP_mini = 1;         // mpprocconfbase.h:31
+constexpr int FRAGREF_NMAXCONVIT = 20;            // mpprocconfbase.h:33
+constexpr int FIN_MAX_NPOSITIONS = 128;           // mpprocconfbase.h:81
+constexpr int XDIM = 128;                         // mpprocconf.h:60 (CPU lane count)
+constexpr float DEFCOORD_QRY_cmp = 99990.0f;      // mpdpbase.h:26
+constexpr float DEFCOORD_RFN_cmp = -99990.0f;     // mpdpbase.h:29
+constexpr float EQUIV_D2 = 25.0f;                 // gproc.h:10 squared
+
+enum { CONV_SCOREDP = 2, CONV_LOWTMSC = 8 };      // gproc.h:39-49
+enum { BT_STOP = 0, BT_LEFT = 1, BT_UP = 2, BT_DIAG = 3 };  // gproc.h:180-185
+enum { READCNST_CALC = 0, READCNST_CALC2 = 1 };   // stagecnsts.cuh:23-26
+
+// working vector of one slot: 9 CC entries, query centre, reference centre,
+// #positions, then the squares used for RMSD (gproc.h:56-112)
+enum { CCM00 = 0, CVq0 = 9, CVr0 = 12, NALN = 15, CV2q0 = 16, CV2r0 = 19, NCCX = 22 };
+constexpr int NEFF = 16;
+
+const float oneTHIRDf = 1.0f / 3.0f;
+const float oneNINTHf = 1.0f / 9.0f;
+const float SQRTf3 = 1.7320508075688772f;
+
+inline float sqrd(float x) { return x * x; }
+
+// ---- d0 / d8 (stagecnsts.cuh:48-175) --------------------------------------
+inline float GetLnorm(int q, int r) { return (float)std::min(q, r); }
+inline float GetD8(int q, int r) { return 1.5f * powf(GetLnorm(q, r), 0.3f) + 3.5f; }
+inline float GetD82(int q, int r) { float d8 = GetD8(q, r); return d8 * d8; }
+inline float GetD0(int q, int r) {
+    float l = GetLnorm(q, r), d0 = 0.168f;
+    if (l > 19.f) d0 = 1.24f * cbrtf(l - 15.f) - 1.8f;
+    return d0 + 0.8f;
+}
+inline float GetD02(int q, int r) { float d0 = GetD0(q, r); return d0 * d0; }
+inline float GetD02_dpscan(int q, int r) { float d0 = GetD0(q, r) + 1.5f; return d0 * d0; }
+inline float GetD0finProtein(int q, int r) {
+    float l = GetLnorm(q, r), d0 = 0.5f;
+    if (l > 21.f) d0 = 1.24f * cbrtf(l - 15.f) - 1.8f;
+    if (d0 < 0.5f) d0 = 0.5f;
+    return d0;
+}
+inline float GetD0finNA(int q, int r) {
+    float l = GetLnorm(q, r), d0 = 0.3f;
+    if (30.f <= l) d0 = 0.6f * sqrtf(l - 0.5f) - 2.5f;
+    else if (23.f < l) d0 = 0.7f;
+    else if (19.f < l) d0 = 0.6f;
+    else if (15.f < l) d0 = 0.5f;
+    else if (11.f < l) d0 = 0.4f;
+    return std::max(0.3f, d0);
+}
+inline float GetD0fin(int q, int r, int na) { return na ? GetD0finNA(q, r) : GetD0finProtein(q, r); }
+inline float GetD0s(float d0) { return std::max(std::min(d0, 8.f), 4.5f); }
+inline float GetD02s(float d0) { float s = GetD0s(d0); return s * s; }
+
+// ---- fragment geometry (fragment.cuh:19-109) ------------------------------
+inline int GetMaxNFragSteps(int maxalnlen, int step, int fraglen) {
+    return (int)ceilf((float)(maxalnlen + step - fraglen) / step);
+}
+inline int GetFragLength(int qlen, int rlen, int qpos, int rpos, int ndx) {
+    int aln = std::min(qlen - qpos, rlen - rpos);
+    if (aln < 4 && ndx == 0) return aln;
+    aln >>= ndx;
+    if (aln == 2 && 0 < ndx && ((aln >> (ndx - 1)) & 1)) aln = 4;
+    if (aln < 3) return 0;
+    if (aln < 4) aln = 4;
+    if (ndx + 1 == FRAGREF_NMAXSUBFRAGS) aln = 4;
+    return aln;
+}
+inline int GetMinFragLengthForAln(int maxalnlen) {
+    if (maxalnlen < 4) return maxalnlen;
+    int f = 0;
+    for (int k = FRAGREF_NMAXSUBFRAGS - 1; k >= 0; k--)
+        if ((f = GetFragLength(maxalnlen, maxalnlen, 0, 0, k)) > 0) break;
+    return f;
+}
+inline void UpdateLengths(int& qlen, int& rlen, int& qpos, int& rpos, int fraglen) {
+    int qe = qpos + fraglen, re = rpos + fraglen;
+    int delta = std::max(qe - qlen, re - rlen);
+    if (0 < delta) { qpos -= delta; rpos -= delta; }
+    else { qlen = qe; rlen = re; }
+}
+// covariancebase.h:21-76
+inline int FragStepMedium(int len) {
+    if (len > 250) return 45;
+    if (len > 200) return 35;
+    if (len > 150) return 25;
+    int l3 = std::max(1, (int)((float)len * oneTHIRDf));
+    return l3 < 15 ? l3 : 15;
+}
+
+// ---- transforms (transformbase.h:40-109) -----------------------------------
+inline void transform_point(const float* t, float& x0, float& x1, float& x2) {
+    float x = x0, y = x1, z = x2;
+    x0 = t[0] * x + t[1] * y + t[2] * z + t[9];
+    x1 = t[3] * x + t[4] * y + t[5] * z + t[10];
+    x2 = t[6] * x + t[7] * y + t[8] * z + t[11];
+}
+inline float distance2(float a0, float a1, float a2, float b0, float b1, float b2) {
+    return sqrd(a0 - b0) + sqrd(a1 - b1) + sqrd(a2 - b2);
+}
+inline float tfm_dist2(const float* t, float qx, float qy, float qz, float rx, float ry, float rz) {
+    transform_point(t, qx, qy, qz);
+    return distance2(qx, qy, qz, rx, ry, rz);
+}
+inline float PairScore(float d02, float d2) { return d02 / (d02 + d2); }
+inline void set_identity(float* t) {
+    for (int i = 0; i < 12; i++) t[i] = 0.f;
+    t[0] = t[4] = t[8] = 1.f;
+}
+
+// ---- Kabsch (transformbase.h:167-583; MpStageBase.h CalcTfmMatricesHelper_Complete)
+void CalcRmatrix(float* c) {
+    float c01 = c[1], c02 = c[2], c12 = c[5];
+    c[0] = c[0] - c[CVq0 + 0] * c[CVr0 + 0];
+    c[1] = c[3] - c[CVq0 + 1] * c[CVr0 + 0];
+    c[2] = c[6] - c[CVq0 + 2] * c[CVr0 + 0];
+    c[3] = c01 - c[CVq0 + 0] * c[CVr0 + 1];
+    c[4] = c[4] - c[CVq0 + 1] * c[CVr0 + 1];
+    c[5] = c[7] - c[CVq0 + 2] * c[CVr0 + 1];
+    c[6] = c02 - c[CVq0 + 0] * c[CVr0 + 2];
+    c[7] = c12 - c[CVq0 + 1] * c[CVr0 + 2];
+    c[8] = c[8] - c[CVq0 + 2] * c[CVr0 + 2];
+}
+float GetRScale(const float* r) {
+    return (fabsf(r[0]) + fabsf(r[1]) + fabsf(r[2]) + fabsf(r[3]) + fabsf(r[4]) +
+            fabsf(r[5]) + fabsf(r[6]) + fabsf(r[7]) + fabsf(r[8])) * oneNINTHf;
+}
+float CalcDet(const float* c) {
+    return c[0] * (c[4] * c[8] - c[5] * c[7]) - c[1] * (c[3] * c[8] - c[5] * c[6]) +
+           c[2] * (c[3] * c[7] - c[4] * c[6]);
+}
+void CalcRTR(const float* c, float* rr) {
+    rr[0] = c[0] * c[0] + c[3] * c[3] + c[6] * c[6];
+    rr[1] = c[0] * c[1] + c[3] * c[4] + c[6] * c[7];
+    rr[2] = c[1] * c[1] + c[4] * c[4] + c[7] * c[7];
+    rr[3] = c[0] * c[2] + c[3] * c[5] + c[6] * c[8];
+    rr[4] = c[1] * c[2] + c[4] * c[5] + c[7] * c[8];
+    rr[5] = c[2] * c[2] + c[5] * c[5] + c[8] * c[8];
+}
+bool SolveCubic(float det, float spur, float cof, float& e0, float& e1, float& e2) {
+    float h = spur * spur - cof;
+    if (h <= 0.0f) return false;
+    float g = (spur * cof - det * det) * 0.5f - spur * h;
+    float sqrth = sqrtf(h);
+    float d = atan2f(sqrtf(std::max(h * h * h - g * g, 0.0f)), -g) * oneTHIRDf;
+    float sth = sinf(d), cth = cosf(d);
+    cth = sqrth * cth;
+    sth = sqrth * SQRTf3 * sth;
+    e0 = (spur + cth) + cth;
+    e1 = (spur - cth) + sth;
+    e2 = (spur - cth) - sth;
+    return true;
+}
+void CalcPartialA(int col, float d, const float* rr, float* a) {
+    float ss0 = (d - rr[2]) * (d - rr[5]) - rr[4] * rr[4];
+    float ss1 = (d - rr[5]) * rr[1] + rr[3] * rr[4];
+    float ss2 = (d - rr[0]) * (d - rr[5]) - rr[3] * rr[3];
+    float ss3 = (d - rr[2]) * rr[3] + rr[1] * rr[4];
+    float ss4 = (d - rr[0]) * rr[4] + rr[1] * rr[3];
+    float ss5 = (d - rr[0]) * (d - rr[2]) - rr[1] * rr[1];
+    if (fabsf(ss0) <= TFMEPSILON) ss0 = 0.0f;
+    if (fabsf(ss1) <= TFMEPSILON) ss1 = 0.0f;
+    if (fabsf(ss2) <= TFMEPSILON) ss2 = 0.0f;
+    if (fabsf(ss3) <= TFMEPSILON) ss3 = 0.0f;
+    if (fabsf(ss4) <= TFMEPSILON) ss4 = 0.0f;
+    if (fabsf(ss5) <= TFMEPSILON) ss5 = 0.0f;
+    if (fabsf(ss0) >= fabsf(ss2)) {
+        if (fabsf(ss0) < fabsf(ss5)) { a[col] = ss3; a[col + 3] = ss4; a[col + 6] = ss5; }
+        else { a[col] = ss0; a[col + 3] = ss1; a[col + 6] = ss3; }
+    } else if (fabsf(ss2) >= fabsf(ss5)) {
+        a[col] = ss1; a[col + 3] = ss2; a[col + 6] = ss4;
+    } else {
+        a[col] = ss3; a[col + 3] = ss4; a[col + 6] = ss5;
+    }
+    d = sqrd(a[col]) + sqrd(a[col + 3]) + sqrd(a[col + 6]);
+    d = (d > TFMEPSILON) ? 1.0f / sqrtf(d) : 0.0f;
+    a[col] *= d; a[col + 3] *= d; a[col + 6] *= d;
+}
+bool CalcCompleteA(float e0, float e1, float e2, float* a) {
+    float d = a[0] * a[2] + a[3] * a[5] + a[6] * a[8];
+    int m1 = 0, m = 2;
+    if (e0 - e1 > e1 - e2) { m1 = 2; m = 0; }
+    float p = 0.0f;
+    a[m1] -= d * a[m]; p += a[m1] * a[m1];
+    a[m1 + 3] -= d * a[m + 3]; p += a[m1 + 3] * a[m1 + 3];
+    a[m1 + 6] -= d * a[m + 6]; p += a[m1 + 6] * a[m1 + 6];
+    if (p > TFMTOL) {
+        p = 1.0f / sqrtf(p);
+        a[m1] *= p; a[m1 + 3] *= p; a[m1 + 6] *= p;
+    } else {
+        p = 1.0f;
+        int j = 6, k = 0, l = 3;
+        if (fabsf(a[m]) <= p) { p = fabsf(a[m]); j = 0; k = 3; l = 6; }
+        if (fabsf(a[m + 3]) <= p) { p = fabsf(a[m + 3]); j = 3; k = 6; l = 0; }
+        if (fabsf(a[m + 6]) <= p) { p = fabsf(a[m + 6]); j = 6; k = 0; l = 3; }
+        p = sqrtf(a[k + m] * a[k + m] + a[l + m] * a[l + m]);
+        if (p <= TFMTOL) return false;
+        a[j + m1] = 0.0f;
+        a[k + m1] = -a[l + m] / p;
+        a[l + m1] = a[k + m] / p;
+    }
+    a[1] = a[5] * a[6] - a[3] * a[8];
+    a[4] = a[8] * a[0] - a[6] * a[2];
+    a[7] = a[2] * a[3] - a[0] * a[5];
+    return true;
+}
+bool CalcRotMtx(const float* a, float* b) {
+    float d = 0.0f;
+    float r0 = b[0], r1 = b[3], r2 = b[6];
+    b[0] = r0 * a[0] + b[1] * a[3] + b[2] * a[6]; d += sqrd(b[0]);
+    b[3] = r1 * a[0] + b[4] * a[3] + b[5] * a[6]; d += sqrd(b[3]);
+    b[6] = r2 * a[0] + b[7] * a[3] + b[8] * a[6]; d += sqrd(b[6]);
+    d = (d > TFMEPSILON) ? 1.0f / sqrtf(d) : 0.0f;
+    b[0] *= d; b[3] *= d; b[6] *= d;
+    d = 0.0f;
+    b[1] = r0 * a[1] + b[1] * a[4] + b[2] * a[7]; d += sqrd(b[1]);
+    b[4] = r1 * a[1] + b[4] * a[4] + b[5] * a[7]; d += sqrd(b[4]);
+    b[7] = r2 * a[1] + b[7] * a[4] + b[8] * a[7]; d += sqrd(b[7]);
+    d = (d > TFMEPSILON) ? 1.0f / sqrtf(d) : 0.0f;
+    b[1] *= d; b[4] *= d; b[7] *= d;
+    d = b[0] * b[1] + b[3] * b[4] + b[6] * b[7];
+    float p = 0.0f;
+    b[1] -= d * b[0]; p += sqrd(b[1]);
+    b[4] -= d * b[3]; p += sqrd(b[4]);
+    b[7] -= d * b[6]; p += sqrd(b[7]);
+    if (p > TFMTOL) {
+        p = 1.0f / sqrtf(p);
+        b[1] *= p; b[4] *= p; b[7] *= p;
+    } else {
+        p = 1.0f;
+        int j = 6, k = 0, l = 3;
+        if (fabsf(b[0]) <= p) { p = fabsf(b[0]); j = 0; k = 3; l = 6; }
+        if (fabsf(b[3]) <= p) { p = fabsf(b[3]); j = 3; k = 6; l = 0; }
+        if (fabsf(b[6]) <= p) { p = fabsf(b[6]); j = 6; k = 0; l = 3; }
+        p = sqrtf(sqrd(b[k]) + sqrd(b[l]));
+        if (p <= TFMTOL) return false;
+        b[j + 1] = 0.0f;
+        b[k + 1] = -b[l] / p;
+        b[l + 1] = b[k] / p;
+    }
+    b[2] = b[3] * b[7] - b[4] * b[6];
+    b[5] = b[6] * b[1] - b[7] * b[0];
+    b[8] = b[0] * b[4] - b[1] * b[3];
+    r0 = b[0]; r1 = b[1]; r2 = b[2];
+    b[0] = r0 * a[0] + r1 * a[1] + r2 * a[2];
+    b[1] = r0 * a[3] + r1 * a[4] + r2 * a[5];
+    b[2] = r0 * a[6] + r1 * a[7] + r2 * a[8];
+    r0 = b[3]; r1 = b[4]; r2 = b[5];
+    b[3] = r0 * a[0] + r1 * a[1] + r2 * a[2];
+    b[4] = r0 * a[3] + r1 * a[4] + r2 * a[5];
+    b[5] = r0 * a[6] + r1 * a[7] + r2 * a[8];
+    r0 = b[6]; r1 = b[7]; r2 = b[8];
+    b[6] = r0 * a[0] + r1 * a[1] + r2 * a[2];
+    b[7] = r0 * a[3] + r1 * a[4] + r2 * a[5];
+    b[8] = r0 * a[6] + r1 * a[7] + r2 * a[8];
+    return true;
+}
+void CalcTrlVec(float* u) {
+    float x0 = u[CVq0], x1 = u[CVq0 + 1], x2 = u[CVq0 + 2];
+    u[CVq0 + 0] = u[CVr0 + 0] - u[0] * x0 - u[1] * x1 - u[2] * x2;
+    u[CVq0 + 1] = u[CVr0 + 1] - u[3] * x0 - u[4] * x1 - u[5] * x2;
+    u[CVq0 + 2] = u[CVr0 + 2] - u[6] * x0 - u[7] * x1 - u[8] * x2;
+}
+// MpStageBase.h:680-745: ccm -> rotation in ccm[0..8], translation in ccm[9..11]
+void KabschHelper(float* ccm, float n) {
+    float a[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, rr[6];
+    for (int i = CVq0; i < CVq0 + 3; i++) ccm[i] /= n;
+    CalcRmatrix(ccm);
+    for (int i = CVr0; i < CVr0 + 3; i++) ccm[i] /= n;
+    float scale = GetRScale(ccm);
+    for (int i = 0; i < 9; i++) ccm[i] /= scale;
+    float det = CalcDet(ccm);
+    CalcRTR(ccm, rr);
+    float spur = (rr[0] + rr[2] + rr[5]) * oneTHIRDf;
+    float cof = (((((rr[2] * rr[5] - sqrd(rr[4])) + rr[0] * rr[5]) - sqrd(rr[3])) + rr[0] * rr[2]) -
+                 sqrd(rr[1])) * oneTHIRDf;
+    bool ok = (spur > 0.0f);
+    if (ok) {
+        float e0, e1, e2;
+        if (SolveCubic(det, spur, cof, e0, e1, e2)) {
+            CalcPartialA(0, e0, rr, a);
+            CalcPartialA(2, e2, rr, a);
+            ok = CalcCompleteA(e0, e1, e2, a);
+        }
+        if (ok) ok = CalcRotMtx(a, ccm);
+    }
+    if (!ok) {
+        for (int i = 0; i < 9; i++) ccm[i] = 0.0f;
+        ccm[0] = ccm[4] = ccm[8] = 1.0f;
+    }
+    CalcTrlVec(ccm);
+}
+// MpStageBase.h:751-777 (CalcTfmMatrices_Complete<DOUBLY_INVERTED>)
+void CalcTfm(int qlen, int rlen, float* ccm, bool dinv) {
+    float n = ccm[NALN];
+    if (n <= 0.0f) return;
+    bool flip = dinv && (qlen < rlen);
+    if (flip) {
+        std::swap(ccm[1], ccm[3]); std::swap(ccm[2], ccm[6]); std::swap(ccm[5], ccm[7]);
+        for (int i = 0; i < 3; i++) std::swap(ccm[CVq0 + i], ccm[CVr0 + i]);
+    }
+    KabschHelper(ccm, n);
+    if (flip) {
+        std::swap(ccm[1], ccm[3]); std::swap(ccm[2], ccm[6]); std::swap(ccm[5], ccm[7]);
+        float t0 = ccm[9], t1 = ccm[10], t2 = ccm[11];
+        float x0 = ccm[0] * t0 + ccm[1] * t1 + ccm[2] * t2;
+        float x1 = ccm[3] * t0 + ccm[4] * t1 + ccm[5] * t2;
+        float x2 = ccm[6] * t0 + ccm[7] * t1 + ccm[8] * t2;
+        ccm[9] = -x0; ccm[10] = -x1; ccm[11] = -x2;
+    }
+}
+// MpStageFin.h:477-540 (CalcRMSD_Complete)
+float CalcRMSD(float* c) {
+    float n = c[NALN];
+    if (n <= 0.0f) return 0.0f;
+    float rr[6];
+    for (int i = CVq0; i < CVq0 + 3; i++) c[i] /= n;
+    for (int i = CV2q0; i < CV2q0 + 3; i++) c[i] /= n;
+    CalcRmatrix(c);
+    for (int i = CVr0; i < CVr0 + 3; i++) c[i] /= n;
+    for (int i = CV2r0; i < CV2r0 + 3; i++) c[i] /= n;
+    float scale = GetRScale(c);
+    for (int i = 0; i < 9; i++) c[i] /= scale;
+    float det = CalcDet(c);
+    CalcRTR(c, rr);
+    float E0 = c[CV2q0] - sqrd(c[CVq0]) + c[CV2q0 + 1] - sqrd(c[CVq0 + 1]) + c[CV2q0 + 2] -
+               sqrd(c[CVq0 + 2]) + c[CV2r0] - sqrd(c[CVr0]) + c[CV2r0 + 1] - sqrd(c[CVr0 + 1]) +
+               c[CV2r0 + 2] - sqrd(c[CVr0 + 2]);
+    float spur = (rr[0] + rr[2] + rr[5]) * oneTHIRDf;
+    float cof = (((((rr[2] * rr[5] - sqrd(rr[4])) + rr[0] * rr[5]) - sqrd(rr[3])) + rr[0] * rr[2]) -
+                 sqrd(rr[1])) * oneTHIRDf;
+    float e0 = spur, e1 = spur, e2 = spur;
+    if (spur > 0.0f) SolveCubic(det, spur, cof, e0, e1, e2);
+    e0 = (e0 <= 0.0f) ? 0.0f : sqrtf(e0);
+    e1 = (e1 <= 0.0f) ? 0.0f : sqrtf(e1);
+    e2 = (e2 <= 0.0f) ? 0.0f : (sqrtf(e2) * ((det < 0.0f) ? -1.0f : 1.0f));
+    E0 -= (2.0f * scale * (e0 + e1 + e2)) / n;
+    return (E0 <= 0.0f) ? 0.0f : sqrtf(E0);
+}
+
+// ---- chunk access ---------------------------------------------------------
+struct Str {
+    int len = 0, dst = 0, na = 0;
+    const float *x, *y, *z;
+    const char* rsd;
+    const char* ss;
+};
+Str get_str(const gta_chunk_view* c, const std::vector<char>& ss, int i) {
+    Str s;
+    s.len = c->len[i];
+    s.dst = (int)c->dst[i];
+    s.na = c->ins_ch_ord ? (c->ins_ch_ord[s.dst] > 0) : 0;
+    s.x = c->x + s.dst; s.y = c->y + s.dst; s.z = c->z + s.dst;
+    s.rsd = c->rsd + s.dst;
+    s.ss = ss.data() + s.dst;
+    return s;
+}
+
+// ---- secondary structure (MpSecStr.cpp:20-110, MpSecStr.h:150-180) -------
+char AssignSS(float d13, float d14, float d15, float d24, float d25, float d35) {
+    float e = 2.1f;
+    if (fabsf(d15 - 6.37f) < e && fabsf(d14 - 5.18f) < e && fabsf(d25 - 5.18f) < e &&
+        fabsf(d13 - 5.45f) < e && fabsf(d24 - 5.45f) < e && fabsf(d35 - 5.45f) < e)
+        return 'h';
+    e = 1.42f;
+    if (fabsf(d15 - 13.0f) < e && fabsf(d14 - 10.4f) < e && fabsf(d25 - 10.4f) < e &&
+        fabsf(d13 - 6.1f) < e && fabsf(d24 - 6.1f) < e && fabsf(d35 - 6.1f) < e)
+        return 'e';
+    if (d15 < 8.0f) return 't';
+    return ' ';
+}
+void calc_secstr(const gta_chunk_view* c, std::vector<char>& ss) {
+    ss.assign(c->nposs, ' ');
+    if (c->ss) std::memcpy(ss.data(), c->ss, c->nposs);
+    for (int s = 0; s < c->nstrs; s++) {
+        int L = c->len[s], D = (int)c->dst[s];
+        bool na = c->ins_ch_ord && c->ins_ch_ord[D] > 0;
+        if (na) continue;  // nucleic-acid SS assignment is outside this stage
+        auto dist = [&](int i, int j) {
+            return sqrtf(distance2(c->x[D + i], c->y[D + i], c->z[D + i], c->x[D + j], c->y[D + j],
+                                   c->z[D + j]));
+        };
+        for (int j = 0; j < L; j++) {
+            if (j < 2 || L - 2 <= j) { ss[D + j] = ' '; continue; }
+            ss[D + j] = AssignSS(dist(j - 2, j), dist(j - 2, j + 1), dist(j - 2, j + 2),
+                                 dist(j - 1, j + 1), dist(j - 1, j + 2), dist(j, j + 2));
+        }
+    }
+}
+
+// ---- lane-structured reductions as in the reference CPU kernels -------------
+// A pass over n items splits them into blocks of XDIM; item (block ai, lane pi)
+// accumulates into lane pi; lanes are reduced in order.
+struct LaneCC {
+    float v[NCCX][XDIM];
+    void clear() { std::memset(v, 0, sizeof(v)); }
+    void add(int pi, float qx, float qy, float qz, float rx, float ry, float rz, bool sq) {
+        v[0][pi] += qx * rx; v[1][pi] += qx * ry; v[2][pi] += qx * rz;
+        v[3][pi] += qy * rx; v[4][pi] += qy * ry; v[5][pi] += qy * rz;
+        v[6][pi] += qz * rx; v[7][pi] += qz * ry; v[8][pi] += qz * rz;
+        v[9][pi] += qx; v[10][pi] += qy; v[11][pi] += qz;
+        v[12][pi] += rx; v[13][pi] += ry; v[14][pi] += rz;
+        if (sq) {
+            v[16][pi] += qx * qx; v[17][pi] += qy * qy; v[18][pi] += qz * qz;
+            v[19][pi] += rx * rx; v[20][pi] += ry * ry; v[21][pi] += rz * rz;
+        }
+    }
+    float sum(int f) const { float s = 0.f; for (int i = 0; i < XDIM; i++) s += v[f][i]; return s; }
+};
+
+struct Min3 {
+    float d[3][XDIM];
+    void clear() { for (int f = 0; f < 3; f++) for (int i = 0; i < XDIM; i++) d[f][i] = CP_LARGEDST; }
+    // MpStageBase.h:553-566 (StoreMinDst)
+    void store(int pi, float dst) {
+        if (dst < d[0][pi]) { d[2][pi] = d[1][pi]; d[1][pi] = d[0][pi]; d[0][pi] = dst; }
+        else if (dst < d[1][pi]) { d[2][pi] = d[1][pi]; d[1][pi] = dst; }
+        else if (dst < d[2][pi]) d[2][pi] = dst;
+    }
+    // index-exclusion selection (MpStageBase.h:489-509): 3rd smallest of the multiset
+    float third_multiset() const {
+        float m1 = CP_LARGEDST, m2 = CP_LARGEDST, m3 = CP_LARGEDST;
+        int f1 = 9, f2 = 9, p1 = XDIM, p2 = XDIM;
+        for (int f = 0; f < 3; f++) for (int p = 0; p < XDIM; p++)
+            if (d[f][p] < m1) { m1 = d[f][p]; f1 = f; p1 = p; }
+        for (int f = 0; f < 3; f++) for (int p = 0; p < XDIM; p++)
+            if ((f != f1 || p != p1) && d[f][p] < m2) { m2 = d[f][p]; f2 = f; p2 = p; }
+        for (int f = 0; f < 3; f++) for (int p = 0; p < XDIM; p++)
+            if ((f != f1 || p != p1) && (f != f2 || p != p2) && d[f][p] < m3) m3 = d[f][p];
+        return m3;
+    }
+    // strict-inequality selection (MpStage1.h:760-776): 3rd smallest distinct value
+    float third_distinct() const {
+        float m1 = CP_LARGEDST, m2 = CP_LARGEDST, m3 = CP_LARGEDST;
+        for (int f = 0; f < 3; f++) for (int p = 0; p < XDIM; p++) if (d[f][p] < m1) m1 = d[f][p];
+        for (int f = 0; f < 3; f++) for (int p = 0; p < XDIM; p++)
+            if (m1 < d[f][p] && d[f][p] < m2) m2 = d[f][p];
+        for (int f = 0; f < 3; f++) for (int p = 0; p < XDIM; p++)
+            if (m2 < d[f][p] && d[f][p] < m3) m3 = d[f][p];
+        return m3;
+    }
+};
+
+// ---- per-pair state (the per-pair slice of the reference working memory) ---
+struct PairState {
+    int flags = 0;
+    float grand = 0.f;      // tawmvGrandBest
+    float best0 = 0.f;      // tawmvBestScore at slot 0
+    float prevbest0 = 0.f;  // tawmvBest0
+    int nalnposs = 0;       // tawmvNAlnPoss
+    int qrypos = 0, rfnpos = 0;
+    int sfragndx = 0, sfragpos = 0;     // tawmvSubFragNdx/Pos
+    float tfm_ibest[12];
+    float tfm_grand[12];
+    // aligned coordinates in reverse alignment order (MpDPHub.cpp:1585-1600)
+    std::vector<float> aln[6];
+    // constrained alignment extras: d2, query pos, reference pos (MpDPHub.cpp:1790-1800)
+    std::vector<float> cd2, cqp, crp;
+    float out[GTA_ALN_NFIELDS];
+    std::vector<char> rows[GTA_ALN_NROWS];
+};
+
+struct ChunkGeom {
+    int qystr1len, dbstr1len, qystrnlen, dbstrnlen;
+    int minfraglen;       // GetMinFragLengthForAln(GetMaxAlnLength()) = 4
+    int stepinit;         // 5 at shallow depth, else 1 (MpBatch.cpp:230)
+    int n1, n2;           // seed diagonal range (MpStage1.cpp:288-291)
+    int nlocsteps;        // refinement slots (MpStage1.cpp:512-516)
+    int maxnfragfcts;     // final-stage positions (MpStageFin.cpp:308-309)
+};
+
+struct Ctx {
+    const gta_options* o;
+    ChunkGeom g;
+    Str q, r;
+    PairState* st;
+    std::vector<float> dst;  // per-position distances of the current slot (tmpdpdiagbuffers)
+    double cells = 0.0;
+};
+
+// ---- DP with backtrace (MpDPHub.cpp:73-388 and 687-984) ------------------
+// Fills the Lq x Lr matrix under transform tfm and returns the backtrace codes.
+void dp_fill(const Str& q, const Str& r, const float* tfm, float d02, float gapopen, bool gap0,
+             bool usess, float ssw, std::vector<uint8_t>& bt, double& cells) {
+    const int Lq = q.len, Lr = r.len;
+    bt.assign((size_t)Lq * Lr, 0);
+    std::vector<float> prev(Lr + 1, 0.f), cur(Lr + 1, 0.f);  // index j+1; [0] is the zero boundary
+    std::vector<float> tq(3 * Lq);
+    for (int i = 0; i < Lq; i++) {
+        float x = q.x[i], y = q.y[i], z = q.z[i];
+        transform_point(tfm, x, y, z);
+        tq[3 * i] = x; tq[3 * i + 1] = y; tq[3 * i + 2] = z;
+    }
+    for (int i = 0; i < Lq; i++) {
+        cur[0] = 0.f;
+        const float qx = tq[3 * i], qy = tq[3 * i + 1], qz = tq[3 * i + 2];
+        for (int j = 0; j < Lr; j++) {
+            float v = 0.0f, w;
+            if (qx < DEFCOORD_QRY_cmp && DEFCOORD_RFN_cmp < r.x[j]) {
+                v = distance2(qx, qy, qz, r.x[j], r.y[j], r.z[j]);
+                v = PairScore(d02, v);
+            }
+            if (usess) v += (float)(q.ss[i] == r.ss[j]) * ssw;
+            float D = prev[j];
+            v += gap0 ? D : fabsf(D);
+            int bk = BT_DIAG;
+            w = cur[j];                       // left
+            if (!gap0 && w < 0.0f) w = gapopen - w;
+            if (v < w) { v = w; bk = BT_LEFT; }
+            w = prev[j + 1];                  // up
+            if (!gap0 && w < 0.0f) w = gapopen - w;
+            if (v < w) { v = w; bk = BT_UP; }
+            cur[j + 1] = (gap0 || bk != BT_DIAG) ? v : -v;
+            bt[(size_t)i * Lr + j] = (uint8_t)bk;
+        }
+        std::swap(prev, cur);
+    }
+    cells += (double)Lq * Lr;
+}
+
+// backtrace from the bottom-right cell; returns matched (qpos, rpos) in walk order
+void backtrace(const std::vector<uint8_t>& bt, int Lq, int Lr, std::vector<int>& qp,
+               std::vector<int>& rp) {
+    qp.clear(); rp.clear();
+    int y = Lq - 1, x = Lr - 1;
+    while (x >= 0 && y >= 0) {
+        int b = bt[(size_t)y * Lr + x];
+        if (b == BT_STOP) break;
+        if (b == BT_UP) { y--; continue; }
+        if (b == BT_LEFT) { x--; continue; }
+        qp.push_back(y); rp.push_back(x);
+        x--; y--;
+    }
+}
+
+// BtckToMatched128xKernel (MpDPHub.cpp:1500-1610): aligned coordinates
+void store_matched(Ctx& c, const std::vector<int>& qp, const std::vector<int>& rp) {
+    PairState& s = *c.st;
+    int n = (int)qp.size();
+    for (int f = 0; f < 6; f++) s.aln[f].resize(n);
+    for (int k = 0; k < n; k++) {
+        s.aln[0][k] = c.q.x[qp[k]]; s.aln[1][k] = c.q.y[qp[k]]; s.aln[2][k] = c.q.z[qp[k]];
+        s.aln[3][k] = c.r.x[rp[k]]; s.aln[4][k] = c.r.y[rp[k]]; s.aln[5][k] = c.r.z[rp[k]];
+    }
+    s.nalnposs = n;
+}
+
+// ConstrainedBtckToMatched128xKernel (MpDPHub.cpp:1620-1810)
+void store_matched_constrained(Ctx& c, const std::vector<int>& qp, const std::vector<int>& rp) {
+    PairState& s = *c.st;
+    const float d82 = GetD82(c.q.len, c.r.len);
+    for (int f = 0; f < 6; f++) s.aln[f].clear();
+    s.cd2.clear(); s.cqp.clear(); s.crp.clear();
+    for (size_t k = 0; k < qp.size(); k++) {
+        float qx = c.q.x[qp[k]], qy = c.q.y[qp[k]], qz = c.q.z[qp[k]];
+        float rx = c.r.x[rp[k]], ry = c.r.y[rp[k]], rz = c.r.z[rp[k]];
+        float d2 = tfm_dist2(s.tfm_grand, qx, qy, qz, rx, ry, rz);
+        if (!(d2 <= d82)) continue;
+        s.aln[0].push_back(qx); s.aln[1].push_back(qy); s.aln[2].push_back(qz);
+        s.aln[3].push_back(rx); s.aln[4].push_back(ry); s.aln[5].push_back(rz);
+        s.cd2.push_back(d2); s.cqp.push_back((float)qp[k]); s.crp.push_back((float)rp[k]);
+    }
+    s.nalnposs = (int)s.aln[0].size();
+}
+
+// ---- passes over the DP alignment (MpStageBase.h:280-470) -----------------
+// index of forward item (ai+pi) of an alignment of length L in reverse storage
+inline int rev_index(int L, int rfnpos, int ai, int piend, int pi) {
+    return L - 1 - (rfnpos + ai + piend - 1) + pi;
+}
+
+// CalcCCMatrices_DPRefined_Complete / CalcExtCCMatrices_DPRefined_Complete
+void cc_dp_fragment(Ctx& c, int fraglen, int fragpos, bool sq, float* out) {
+    PairState& s = *c.st;
+    int qlen = s.nalnposs, rlen = s.nalnposs, qpos = fragpos, rpos = fragpos;
+    int L = s.nalnposs;
+    UpdateLengths(rlen, rlen, qpos, rpos, fraglen);
+    (void)qlen;
+    static thread_local LaneCC cc;
+    cc.clear();
+    for (int ai = 0; ai < fraglen; ai += XDIM) {
+        int piend = std::min(XDIM, fraglen - ai);
+        for (int pi = 0; pi < piend; pi++) {
+            int k = rev_index(L, rpos, ai, piend, pi);
+            cc.add(pi, s.aln[0][k], s.aln[1][k], s.aln[2][k], s.aln[3][k], s.aln[4][k], s.aln[5][k], sq);
+        }
+    }
+    for (int f = 0; f < NCCX; f++) out[f] = 0.f;
+    int nf = sq ? NCCX : NALN;
+    for (int f = 0; f < nf; f++) if (f != NALN) out[f] = cc.sum(f);
+    out[NALN] = (float)fraglen;
+}
+
+// CalcCCMatrices_DPRefinedExtended_Complete: pairs with stored d2 <= threshold
+void cc_dp_extended(Ctx& c, int readcnst, float d0, float dst32, float* out) {
+    PairState& s = *c.st;
+    float d02s = dst32;
+    if (readcnst != READCNST_CALC) { float d0s = GetD0s(d0) + 1.0f; d02s = d0s * d0s; }
+    const int L = s.nalnposs;
+    static thread_local LaneCC cc;
+    cc.clear();
+    float cnt[XDIM] = {0};
+    for (int ai = 0; ai < L; ai += XDIM) {
+        int piend = std::min(XDIM, L - ai);
+        for (int pi = 0; pi < piend; pi++) {
+            float d = c.dst[ai + pi];
+            if (d02s < d) continue;
+            int k = rev_index(L, 0, ai, piend, pi);
+            cc.add(pi, s.aln[0][k], s.aln[1][k], s.aln[2][k], s.aln[3][k], s.aln[4][k], s.aln[5][k], false);
+            cnt[pi] += 1.0f;
+        }
+    }
+    for (int f = 0; f < NALN; f++) out[f] = cc.sum(f);
+    float n = 0.f;
+    for (int i = 0; i < XDIM; i++) n += cnt[i];
+    out[NALN] = n;
+}
+
+// CalcScoresUnrl_DPRefined_Complete: TM sum, stores d2, returns next threshold in *thr
+float scores_dp(Ctx& c, int readcnst, float d0, float d02, float d82, bool chkdst, const float* tfm,
+                float* thr) {
+    PairState& s = *c.st;
+    const int L = s.nalnposs;
+    float scv[XDIM] = {0};
+    static thread_local Min3 m3;
+    m3.clear();
+    c.dst.resize(std::max<size_t>(c.dst.size(), (size_t)L));
+    for (int ai = 0; ai < L; ai += XDIM) {
+        int piend = std::min(XDIM, L - ai);
+        for (int pi = 0; pi < piend; pi++) {
+            int k = rev_index(L, 0, ai, piend, pi);
+            float d = tfm_dist2(tfm, s.aln[0][k], s.aln[1][k], s.aln[2][k], s.aln[3][k], s.aln[4][k],
+                                s.aln[5][k]);
+            if (!chkdst || d <= d82) scv[pi] += PairScore(d02, d);
+            c.dst[ai + pi] = d;
+            if (readcnst == READCNST_CALC) m3.store(pi, d);
+        }
+    }
+    float sum = 0.f;
+    for (int i = 0; i < XDIM; i++) sum += scv[i];
+    if (readcnst == READCNST_CALC && thr) {
+        float min3 = m3.third_multiset();
+        float d0s = GetD0s(d0) + ((readcnst == READCNST_CALC2) ? 1.0f : -1.0f);
+        float d02s = d0s * d0s;
+        if (CP_LARGEDST_cmp < min3 || min3 < d02s || L <= 3) min3 = d02s;
+        else { min3 = d0s + ceilf((sqrtf(min3) - d0s) * 2.0f) * 0.5f; min3 = min3 * min3; }
+        *thr = min3;
+    }
+    return sum;
+}
+
+// CheckConvergenceRefined_Complete (MpStageBase.h:262-277)
+bool converged(const float* cc, const float* last) {
+    int n = 0;
+    for (int f = 0; f < NEFF; f++) {
+        float a = cc[f], b = last[f];
+        if (fabsf(a - b) < std::min(fabsf(a), fabsf(b)) * RM_CONVEPSILON) n++;
+    }
+    return NEFF <= n;
+}
+
+// the iteration of one slot over the DP alignment: returns best and tfmBest
+// (MpStage1.cpp:880-920, MpStageFin.cpp:170-210, ProductionRefinementPhase2InnerLoop)
+float refine_slot_dp(Ctx& c, int fraglen, int fragpos, float d0, float d02, float d82, bool chkdst,
+                     float bestinit, float* tfmBest, const float* ccinit) {
+    const PairState& s = *c.st;
+    float ccm[NCCX], tfm[NCCX], last[NCCX];
+    if (ccinit) std::memcpy(ccm, ccinit, sizeof(ccm));
+    else cc_dp_fragment(c, fraglen, fragpos, false, ccm);
+    float dst32 = CP_LARGEDST, best = bestinit;
+    (void)s;
+    for (int cit = 0; cit < FRAGREF_NMAXCONVIT + 2; cit++) {
+        if (0 < cit) {
+            cc_dp_extended(c, (cit < 2) ? READCNST_CALC : READCNST_CALC2, d0, dst32, ccm);
+            if (converged(ccm, last)) break;
+        }
+        if (ccm[NALN] < 1.0f) break;
+        for (int f = 0; f < NEFF; f++) tfm[f] = last[f] = ccm[f];
+        CalcTfm(c.q.len, c.r.len, tfm, c.o->symmetric);
+        float thr = dst32;
+        float sc = scores_dp(c, (cit < 1) ? READCNST_CALC : READCNST_CALC2, d0, d02, d82, chkdst, tfm,
+                             &thr);
+        dst32 = thr;
+        if (best < sc) { best = sc; std::memcpy(tfmBest, tfm, 12 * sizeof(float)); }
+    }
+    return best;
+}
+
+// ---- gapless passes for stage 1 (MpStage1.h:470-1000) ---------------------
+void cc_gapless(Ctx& c, int qpos, int rpos, int n, float* out) {
+    static thread_local LaneCC cc;
+    cc.clear();
+    for (int ai = 0; ai < n; ai += XDIM) {
+        int piend = std::min(XDIM, n - ai);
+        for (int pi = 0; pi < piend; pi++) {
+            int qi = qpos + ai + pi, ri = rpos + ai + pi;
+            cc.add(pi, c.q.x[qi], c.q.y[qi], c.q.z[qi], c.r.x[ri], c.r.y[ri], c.r.z[ri], false);
+        }
+    }
+    for (int f = 0; f < NALN; f++) out[f] = cc.sum(f);
+    out[NALN] = (float)n;
+}
+void cc_gapless_extended(Ctx& c, int qpos, int rpos, int n, float d02s, float* out) {
+    static thread_local LaneCC cc;
+    cc.clear();
+    float cnt[XDIM] = {0};
+    for (int ai = 0; ai < n; ai += XDIM) {
+        int piend = std::min(XDIM, n - ai);
+        for (int pi = 0; pi < piend; pi++) {
+            if (d02s < c.dst[ai + pi]) continue;
+            int qi = qpos + ai + pi, ri = rpos + ai + pi;
+            cc.add(pi, c.q.x[qi], c.q.y[qi], c.q.z[qi], c.r.x[ri], c.r.y[ri], c.r.z[ri], false);
+            cnt[pi] += 1.0f;
+        }
+    }
+    for (int f = 0; f < NALN; f++) out[f] = cc.sum(f);
+    float s = 0.f;
+    for (int i = 0; i < XDIM; i++) s += cnt[i];
+    out[NALN] = s;
+}
+// CalcScoresUnrl_Complete (MpStage1.h:700-790): all pairs scored, distinct-value threshold
+float scores_gapless_seed(Ctx& c, int readcnst, int qpos, int rpos, int n, float d0, float d02,
+                          const float* tfm, bool savepos, float* thr) {
+    float scv[XDIM] = {0};
+    static thread_local Min3 m3;
+    m3.clear();
+    if (c.dst.size() < (size_t)n) c.dst.resize(n);
+    for (int ai = 0; ai < n; ai += XDIM) {
+        int piend = std::min(XDIM, n - ai);
+        for (int pi = 0; pi < piend; pi++) {
+            int qi = qpos + ai + pi, ri = rpos + ai + pi;
+            float d = tfm_dist2(tfm, c.q.x[qi], c.q.y[qi], c.q.z[qi], c.r.x[ri], c.r.y[ri], c.r.z[ri]);
+            scv[pi] += PairScore(d02, d);
+            if (savepos) { c.dst[ai + pi] = d; m3.store(pi, d); }
+        }
+    }
+    float sum = 0.f;
+    for (int i = 0; i < XDIM; i++) sum += scv[i];
+    if (savepos && thr) {
+        float min3 = m3.third_distinct();
+        float d02s = GetD02s(d0);
+        if (readcnst == READCNST_CALC2) d02s += D02s_PROC_INC;
+        if (CP_LARGEDST_cmp < min3 || min3 < d02s || n <= 3) min3 = d02s;
+        else min3 = d02s + ceilf((min3 - d02s) * 2.0f) * 0.5f;
+        *thr = min3;
+    }
+    return sum;
+}
+// CalcScoresUnrlRefined_Complete (MpStage1.h:800-880)
+float scores_gapless_refined(Ctx& c, int readcnst, int qpos, int rpos, float d0, float d02,
+                             float d82, const float* tfm, float* thr) {
+    const int n = std::min(c.q.len - qpos, c.r.len - rpos);
+    float scv[XDIM] = {0};
+    static thread_local Min3 m3;
+    m3.clear();
+    if (c.dst.size() < (size_t)n) c.dst.resize(n);
+    for (int ai = 0; ai < n; ai += XDIM) {
+        int piend = std::min(XDIM, n - ai);
+        for (int pi = 0; pi < piend; pi++) {
+            int qi = qpos + ai + pi, ri = rpos + ai + pi;
+            float d = tfm_dist2(tfm, c.q.x[qi], c.q.y[qi], c.q.z[qi], c.r.x[ri], c.r.y[ri], c.r.z[ri]);
+            if (d <= d82) scv[pi] += PairScore(d02, d);
+            c.dst[ai + pi] = d;
+            if (readcnst == READCNST_CALC) m3.store(pi, d);
+        }
+    }
+    float sum = 0.f;
+    for (int i = 0; i < XDIM; i++) sum += scv[i];
+    if (readcnst == READCNST_CALC && thr) {
+        float min3 = m3.third_multiset();
+        float d0s = GetD0s(d0) + ((readcnst == READCNST_CALC2) ? 1.0f : -1.0f);
+        float d02s = d0s * d0s;
+        if (CP_LARGEDST_cmp < min3 || min3 < d02s || n <= 3) min3 = d02s;
+        else { min3 = d0s + ceilf((sqrtf(min3) - d0s) * 2.0f) * 0.5f; min3 = min3 * min3; }
+        *thr = min3;
+    }
+    return sum;
+}
+
+// ---- stage 1 --------------------------------------------------------------
+// FindFragKernel (MpStage1.cpp:275-440)
+void find_frag(Ctx& c) {
+    PairState& s = *c.st;
+    if (s.flags & CONV_LOWTMSC) return;
+    const int qlen = c.q.len, rlen = c.r.len;
+    const int nseeds = (c.g.n2 - c.g.n1) / c.g.stepinit + 1;
+    float bestbest = 0.f;
+    int bq = 0, br = 0;
+    for (int si = 0; si < nseeds; si++) {
+        const int abspos = c.g.n1 + si * c.g.stepinit;
+        const int qpos = std::max(0, abspos), rpos = std::max(-abspos, 0);
+        const int prv = std::max(std::min(qlen, rlen) >> 1, 5);
+        if (qlen <= qpos || rlen <= rpos) continue;
+        if (qlen - qpos < prv || rlen - rpos < prv) continue;
+        const int n = std::min(qlen - qpos, rlen - rpos);
+        const float d0 = GetD0(qlen, rlen), d02 = d0 * d0;
+        float best = 0.f, dst32, ccm[NCCX], tfm[NCCX];
+        for (;;) {
+            cc_gapless(c, qpos, rpos, n, ccm);
+            if (ccm[NALN] < 1.0f) break;
+            std::memcpy(tfm, ccm, sizeof(tfm));
+            CalcTfm(qlen, rlen, tfm, c.o->symmetric);
+            float sc = scores_gapless_seed(c, READCNST_CALC, qpos, rpos, n, d0, d02, tfm, true, &dst32);
+            if (best < sc) best = sc;
+            cc_gapless_extended(c, qpos, rpos, n, dst32, ccm);
+            if (ccm[NALN] < 1.0f) break;
+            if (ccm[NALN] == (float)n) break;
+            std::memcpy(tfm, ccm, sizeof(tfm));
+            CalcTfm(qlen, rlen, tfm, c.o->symmetric);
+            sc = scores_gapless_seed(c, READCNST_CALC2, qpos, rpos, n, d0, d02, tfm, true, &dst32);
+            if (best < sc) best = sc;
+            cc_gapless_extended(c, qpos, rpos, n, dst32, ccm);
+            if (ccm[NALN] < 1.0f) break;
+            if (ccm[NALN] == (float)n) break;
+            std::memcpy(tfm, ccm, sizeof(tfm));
+            CalcTfm(qlen, rlen, tfm, c.o->symmetric);
+            sc = scores_gapless_seed(c, READCNST_CALC2, qpos, rpos, n, d0, d02, tfm, false, nullptr);
+            if (best < sc) best = sc;
+            break;
+        }
+        // SaveBestScoreAndPositions_Complete + SaveBestScoreAmongBests (lowest seed on ties)
+        if (best > 0.f && bestbest < best) { bestbest = best; bq = qpos; br = rpos; }
+    }
+    s.best0 = bestbest;
+    s.qrypos = bq;
+    s.rfnpos = br;
+}
+
+// slot selection among all slots of one call (SaveBestScoreAndTMAmongBests,
+// MpStageBase.h:790-860) with GRANDUPDATE
+void among_bests(PairState& s, float bscore, const float* tfmbest, bool have) {
+    s.best0 = have ? bscore : 0.f;
+    if (have) std::memcpy(s.tfm_ibest, tfmbest, 12 * sizeof(float));
+    if (have && s.grand < bscore) {
+        s.grand = bscore;
+        std::memcpy(s.tfm_grand, tfmbest, 12 * sizeof(float));
+    }
+}
+
+// RefineFragInitKernel (MpStage1.cpp:470-640)
+void refine_frag_init(Ctx& c) {
+    PairState& s = *c.st;
+    if (s.flags & CONV_LOWTMSC) { s.best0 = 0.f; return; }
+    const int qlen = c.q.len, rlen = c.r.len, qpos = s.qrypos, rpos = s.rfnpos;
+    float bscore = 0.f, btfm[12];
+    bool have = false;
+    if (!(qlen <= qpos || rlen <= rpos)) {
+        const float d0 = GetD0(qlen, rlen), d02 = d0 * d0, d82 = GetD82(qlen, rlen);
+        for (int si = 0; si < c.g.nlocsteps; si++) {
+            const int sfragfct = si / FRAGREF_NMAXSUBFRAGS;
+            const int sfragndx = si - sfragfct * FRAGREF_NMAXSUBFRAGS;
+            const int sfragpos = sfragfct * FRAGREF_SFRAGSTEP;
+            const int fraglen = GetFragLength(qlen, rlen, qpos, rpos, sfragndx);
+            if (fraglen < 1) continue;
+            if (qlen + FRAGREF_SFRAGSTEP <= qpos + sfragpos + fraglen ||
+                rlen + FRAGREF_SFRAGSTEP <= rpos + sfragpos + fraglen)
+                continue;
+            // CalcCCMatricesRefined_Complete
+            float ccm[NCCX], tfm[NCCX], last[NCCX], tfmBest[12];
+            {
+                int ql = qlen, rl = rlen, qp = qpos + sfragpos, rp = rpos + sfragpos;
+                UpdateLengths(ql, rl, qp, rp, fraglen);
+                cc_gapless(c, qp, rp, fraglen, ccm);
+            }
+            float dst32 = CP_LARGEDST, best = 0.f;
+            for (int cit = 0; cit < FRAGREF_NMAXCONVIT + 2; cit++) {
+                if (0 < cit) {
+                    float d02s = dst32;
+                    if (cit >= 2) { float d0s = GetD0s(d0) + 1.0f; d02s = d0s * d0s; }
+                    cc_gapless_extended(c, qpos, rpos, std::min(qlen - qpos, rlen - rpos), d02s, ccm);
+                    if (converged(ccm, last)) break;
+                }
+                if (ccm[NALN] < 1.0f) break;
+                for (int f = 0; f < NEFF; f++) tfm[f] = last[f] = ccm[f];
+                CalcTfm(qlen, rlen, tfm, c.o->symmetric);
+                float thr = dst32;
+                float sc = scores_gapless_refined(c, (cit < 1) ? READCNST_CALC : READCNST_CALC2, qpos,
+                                                  rpos, d0, d02, d82, tfm, &thr);
+                dst32 = thr;
+                if (best < sc) { best = sc; std::memcpy(tfmBest, tfm, sizeof(tfmBest)); }
+            }
+            if (best > 0.f && bscore < best) { bscore = best; std::memcpy(btfm, tfmBest, sizeof(btfm)); have = true; }
+        }
+    }
+    among_bests(s, bscore, btfm, have);
+}
+
+// RefineFragDPKernel (MpStage1.cpp:780-960); readlocalconv only changes which
+// flags are read, and the flags are per pair here
+void refine_frag_dp(Ctx& c) {
+    PairState& s = *c.st;
+    if (s.flags) { s.best0 = 0.f; return; }
+    const int L = s.nalnposs;
+    const float d0 = GetD0(c.q.len, c.r.len), d02 = d0 * d0, d82 = GetD82(c.q.len, c.r.len);
+    float bscore = 0.f, btfm[12];
+    bool have = false;
+    for (int si = 0; si < c.g.nlocsteps; si++) {
+        const int sfragfct = si / FRAGREF_NMAXSUBFRAGS;
+        const int sfragndx = si - sfragfct * FRAGREF_NMAXSUBFRAGS;
+        const int sfragpos = sfragfct * FRAGREF_SFRAGSTEP;
+        const int fraglen = GetFragLength(L, L, 0, 0, sfragndx);
+        if (fraglen < 1) continue;
+        if (L + FRAGREF_SFRAGSTEP <= sfragpos + fraglen) continue;
+        float tfmBest[12];
+        float best = refine_slot_dp(c, fraglen, sfragpos, d0, d02, d82, true, 0.f, tfmBest, nullptr);
+        if (best > 0.f && bscore < best) { bscore = best; std::memcpy(btfm, tfmBest, sizeof(btfm)); have = true; }
+    }
+    among_bests(s, bscore, btfm, have);
+}
+
+void dp_align_ibest(Ctx& c, float gapopen, bool gap0) {
+    PairState& s = *c.st;
+    // ExecDPwBtck128xKernel skip (MpDPHub.cpp:159-165) + BtckToMatched (1555-1565)
+    if (s.flags & (CONV_LOWTMSC | CONV_SCOREDP)) { s.nalnposs = 0; return; }
+    std::vector<uint8_t> bt;
+    dp_fill(c.q, c.r, s.tfm_ibest, GetD02(c.q.len, c.r.len), gapopen, gap0, false, 0.f, bt, c.cells);
+    std::vector<int> qp, rp;
+    backtrace(bt, c.q.len, c.r.len, qp, rp);
+    store_matched(c, qp, rp);
+}
+
+// DPRefine<GAP0,PRESCREEN,false> (MpStage1.cpp:455-510)
+void dp_refine(Ctx& c, bool gap0, bool prescreen, int maxndpiters, float prescorethr) {
+    PairState& s = *c.st;
+    const float gcost = gap0 ? 0.0f : -0.6f;
+    for (int dpi = 0; dpi < maxndpiters; dpi++) {
+        dp_align_ibest(c, gcost, gap0);
+        refine_frag_dp(c);
+        // CheckScoreConvergenceKernel (MpStage1.cpp:1075-1120)
+        if (0 < dpi) {
+            if (!(s.flags & (CONV_SCOREDP | CONV_LOWTMSC)) && fabsf(s.best0 - s.prevbest0) < SCORE_CONVEPSILON)
+                s.flags |= CONV_SCOREDP;
+        }
+        // SaveLastScore0Kernel (MpStage1.cpp:1130-1165)
+        if (dpi + 1 < maxndpiters) {
+            if (!(s.flags & (CONV_SCOREDP | CONV_LOWTMSC))) s.prevbest0 = s.best0;
+        }
+        // SetLowScoreConvergenceFlagKernel (MpStage1.cpp:1175-1215)
+        if (prescreen && maxndpiters <= dpi + 1 && 0.0f < prescorethr) {
+            if (s.grand < prescorethr * (float)std::min(c.q.len, c.r.len)) s.flags |= CONV_LOWTMSC;
+        }
+    }
+    s.flags &= ~CONV_SCOREDP;   // InitScoresKernel<INITOPT_CONVFLAG_SCOREDP>
+}
+
+// ---- stage 2 (MpStage2.h:60-131) -------------------------------------------
+void dp_align_grand(Ctx& c, float gapopen, bool gap0, bool usess, bool dpscan, bool constrained) {
+    PairState& s = *c.st;
+    if (s.flags & CONV_LOWTMSC) {
+        if (!constrained) s.nalnposs = 0;
+        return;
+    }
+    std::vector<uint8_t> bt;
+    float d02 = dpscan ? GetD02_dpscan(c.q.len, c.r.len) : GetD02(c.q.len, c.r.len);
+    dp_fill(c.q, c.r, s.tfm_grand, d02, gapopen, gap0, usess, usess ? 0.5f : 0.f, bt, c.cells);
+    std::vector<int> qp, rp;
+    backtrace(bt, c.q.len, c.r.len, qp, rp);
+    if (constrained) store_matched_constrained(c, qp, rp);
+    else {
+        if (s.flags & CONV_SCOREDP) { s.nalnposs = 0; return; }
+        store_matched(c, qp, rp);
+    }
+}
+
+void stage2(Ctx& c, bool gap0, bool usess, bool dpscan, int maxndpiters, bool checklow, float scorethld) {
+    PairState& s = *c.st;
+    dp_align_grand(c, gap0 ? 0.0f : -1.0f, gap0, usess, dpscan, false);
+    refine_frag_dp(c);
+    dp_refine(c, false, false, 2, c.o->prescore);
+    dp_refine(c, true, false, maxndpiters, c.o->prescore);
+    if (checklow && 0.0f < scorethld) {
+        if (s.grand < scorethld * (float)std::min(c.q.len, c.r.len)) s.flags |= CONV_LOWTMSC;
+    }
+}
+
+// ---- final stage (MpStageFin.h:66-470, MpStageFin.cpp) -----------------------
+// FinalFragmentBasedDPAlignmentRefinementPhase1Kernel<false,CHECK,...>
+void fin_phase1(Ctx& c) {
+    PairState& s = *c.st;
+    if (s.flags & CONV_LOWTMSC) return;
+    const int L = s.nalnposs;
+    const float d0 = GetD0(c.q.len, c.r.len), d02 = d0 * d0, d82 = GetD82(c.q.len, c.r.len);
+    float bscore = 0.f, btfm[12];
+    int bndx = 0, bpos = 0;
+    bool have = false;
+    for (int si = 0; si < c.g.nlocsteps; si++) {
+        const int sfragfct = si / FRAGREF_NMAXSUBFRAGS;
+        const int sfragndx = si - sfragfct * FRAGREF_NMAXSUBFRAGS;
+        const int sfragpos = sfragfct * FRAGREF_SFRAGSTEP;
+        const int fraglen = GetFragLength(L, L, 0, 0, sfragndx);
+        if (fraglen < 1) continue;
+        if (L + FRAGREF_SFRAGSTEP <= sfragpos + fraglen) continue;
+        float tfmBest[12];
+        float best = refine_slot_dp(c, fraglen, sfragpos, d0, d02, d82, true, -1.0f, tfmBest, nullptr);
+        if (best > 0.f && bscore < best) {
+            bscore = best; std::memcpy(btfm, tfmBest, sizeof(btfm)); bndx = sfragndx; bpos = sfragpos; have = true;
+        }
+    }
+    // AmongBests with WRITEFRAGINFO and FORCEWRITEFRAGINFO
+    s.best0 = have ? bscore : 0.f;
+    if (have) std::memcpy(s.tfm_ibest, btfm, sizeof(btfm));
+    s.sfragndx = have ? bndx : 0;
+    s.sfragpos = have ? bpos : 0;
+    if (have && s.grand < bscore) { s.grand = bscore; std::memcpy(s.tfm_grand, btfm, sizeof(btfm)); }
+}
+
+// FinalFragmentBasedDPAlignmentRefinementPhase2Kernel (MpStageFin.cpp:270-470)
+void fin_phase2(Ctx& c) {
+    PairState& s = *c.st;
+    if (s.flags & CONV_LOWTMSC) return;
+    const int L = s.nalnposs;
+    const float d0 = GetD0(c.q.len, c.r.len), d02 = d0 * d0, d82 = GetD82(c.q.len, c.r.len);
+    float bscore = 0.f, btfm[12];
+    bool have = false;
+    for (int sfragfct = 0; sfragfct < c.g.maxnfragfcts; sfragfct++) {
+        int sfragndx = s.sfragndx, sfragpos = s.sfragpos;
+        if (sfragndx == 0) sfragndx++;
+        sfragpos += (sfragfct - (c.g.maxnfragfcts >> 1)) * FRAGREF_SFRAGSTEP_mini;
+        const int fraglen = GetFragLength(L, L, 0, 0, sfragndx);
+        if (fraglen < 1) continue;
+        if (sfragpos < 0 || L + FRAGREF_SFRAGSTEP_mini <= sfragpos + fraglen) continue;
+        float tfmBest[12];
+        float best = refine_slot_dp(c, fraglen, sfragpos, d0, d02, d82, true, -1.0f, tfmBest, nullptr);
+        if (best > 0.f && bscore < best) { bscore = best; std::memcpy(btfm, tfmBest, sizeof(btfm)); have = true; }
+    }
+    s.best0 = have ? bscore : 0.f;
+    if (have) std::memcpy(s.tfm_ibest, btfm, sizeof(btfm));
+    if (have && s.grand < bscore) { s.grand = bscore; std::memcpy(s.tfm_grand, btfm, sizeof(btfm)); }
+}
+
+// production slot: refine with final d0 over all pairs, then score for the
+// longer structure (MpStageFin.cpp:590-680)
+float production_slot(Ctx& c, int fraglen, int fragpos, float d0, float d02, float d82, float g02,
+                      float* tfmBest, float* gbest, const float* ccinit) {
+    float best = refine_slot_dp(c, fraglen, fragpos, d0, d02, d82, false, -1.0f, tfmBest, ccinit);
+    *gbest = best;
+    if (c.q.len != c.r.len) {
+        float g0 = sqrtf(g02);
+        *gbest = scores_dp(c, READCNST_CALC2, g0, g02, d82, false, tfmBest, nullptr);
+    }
+    return best;
+}
+
+// ProductionSaveBestScoresAndTMAmongBests (MpStageFin.h:880-975)
+void production_among_bests(Ctx& c, float bscore, float gbest, const float* btfm, bool have,
+                            bool conditional) {
+    PairState& s = *c.st;
+    float grand = conditional ? s.grand : 0.0f;
+    if (!(have && grand < bscore)) return;
+    s.grand = bscore;
+    const int qlen = c.q.len, rlen = c.r.len;
+    float sq = bscore, sr = gbest;
+    if (rlen < qlen) std::swap(sq, sr);
+    s.out[GTA_ALN_SCOREQ] = sq / (float)qlen;
+    s.out[GTA_ALN_SCORER] = sr / (float)rlen;
+    s.out[GTA_ALN_D0Q] = GetD0fin(qlen, qlen, c.q.na);
+    s.out[GTA_ALN_D0R] = GetD0fin(rlen, rlen, c.r.na);
+    std::memcpy(s.tfm_grand, btfm, 12 * sizeof(float));
+}
+
+// ProductionFragmentBasedDPAlignmentRefinementPhase1Kernel (MpStageFin.cpp:747-975)
+void production_phase1(Ctx& c) {
+    PairState& s = *c.st;
+    if (s.flags & CONV_LOWTMSC) return;
+    const int L = s.nalnposs;
+    const float d0 = GetD0fin(c.q.len, c.r.len, c.r.na), d02 = d0 * d0, d82 = GetD82(c.q.len, c.r.len);
+    const int glen = std::max(c.q.len, c.r.len);
+    const float g0 = GetD0fin(glen, glen, c.r.na), g02 = g0 * g0;
+    float bscore = 0.f, bg = 0.f, btfm[12];
+    int bndx = 0, bpos = 0;
+    bool have = false;
+    for (int si = 0; si < c.g.nlocsteps; si++) {
+        const int sfragfct = si / FRAGREF_NMAXSUBFRAGS;
+        const int sfragndx = si - sfragfct * FRAGREF_NMAXSUBFRAGS;
+        const int sfragpos = sfragfct * FRAGREF_SFRAGSTEP;
+        const int fraglen = GetFragLength(L, L, 0, 0, sfragndx);
+        if (fraglen < 1) continue;
+        if (L + FRAGREF_SFRAGSTEP <= sfragpos + fraglen) continue;
+        float ccinit[NCCX];
+        const float* ccp = nullptr;
+        if (si == 0) {
+            // CalcExtCCMatrices_DPRefined_Complete + CalcRMSD_Complete
+            cc_dp_fragment(c, fraglen, sfragpos, true, ccinit);
+            float tmp[NCCX];
+            std::memcpy(tmp, ccinit, sizeof(tmp));
+            s.out[GTA_ALN_RMSD] = CalcRMSD(tmp);
+            ccp = ccinit;
+        }
+        float tfmBest[12], gb;
+        float best = production_slot(c, fraglen, sfragpos, d0, d02, d82, g02, tfmBest, &gb, ccp);
+        // SaveBestQRScoresAndTM_Complete (currentbest starts at 0)
+        if (0.0f < best && bscore < best) {
+            bscore = best; bg = gb; std::memcpy(btfm, tfmBest, sizeof(btfm));
+            bndx = sfragndx; bpos = sfragpos; have = true;
+        }
+    }
+    if (have) { s.sfragndx = bndx; s.sfragpos = bpos; }
+    production_among_bests(c, bscore, bg, btfm, have, false);
+}
+
+// ProductionFragmentBasedDPAlignmentRefinementPhase2Kernel (MpStageFin.cpp:1310-1470)
+void production_phase2(Ctx& c) {
+    PairState& s = *c.st;
+    if (s.flags & CONV_LOWTMSC) return;
+    const int L = s.nalnposs;
+    const float d0 = GetD0fin(c.q.len, c.r.len, c.r.na), d02 = d0 * d0, d82 = GetD82(c.q.len, c.r.len);
+    const int glen = std::max(c.q.len, c.r.len);
+    const float g0 = GetD0fin(glen, glen, c.r.na), g02 = g0 * g0;
+    float bscore = 0.f, bg = 0.f, btfm[12];
+    bool have = false;
+    for (int sfragfct = 0; sfragfct < c.g.maxnfragfcts; sfragfct++) {
+        int sfragndx = s.sfragndx, sfragpos = s.sfragpos;
+        sfragpos += (sfragfct - (c.g.maxnfragfcts >> 1)) * FRAGREF_SFRAGSTEP_mini;
+        if (sfragndx == 0) sfragndx++;
+        const int fraglen = GetFragLength(L, L, 0, 0, sfragndx);
+        if (fraglen < 1) continue;
+        if (sfragpos < 0 || L + FRAGREF_SFRAGSTEP_mini <= sfragpos + fraglen) continue;
+        float tfmBest[12], gb;
+        float best = production_slot(c, fraglen, sfragpos, d0, d02, d82, g02, tfmBest, &gb, nullptr);
+        if (0.0f < best && bscore < best) { bscore = best; bg = gb; std::memcpy(btfm, tfmBest, sizeof(btfm)); have = true; }
+    }
+    production_among_bests(c, bscore, bg, btfm, have, true);
+}
+
+// ProductionMatchToAlignment128xKernel (MpDPHub.cpp:1919-2142)
+void production_alignment(Ctx& c) {
+    PairState& s = *c.st;
+    if (s.flags & CONV_LOWTMSC) return;
+    const int m = s.nalnposs;
+    for (auto& r : s.rows) r.clear();
+    int idts = 0, psts = 0, gaps = 0;
+    uint32_t bcrds = 0, ecrds = 0;
+    auto push = [&](int qch, int mch, int tch, int qss, int tss) {
+        s.rows[0].push_back((char)qch); s.rows[1].push_back((char)mch); s.rows[2].push_back((char)tch);
+        s.rows[3].push_back((char)qss); s.rows[4].push_back((char)tss);
+    };
+    // matched pairs in alignment order (stored reversed)
+    for (int k = 0; k < m; k++) {
+        int idx = m - 1 - k;
+        int qp = (int)s.cqp[idx], rp = (int)s.crp[idx];
+        float d2 = s.cd2[idx];
+        ecrds = ((uint32_t)(rp + 1) << 16) | (uint32_t)(qp + 1);
+        if (k == 0) bcrds = ecrds;
+        if (d2 < EQUIV_D2) psts++;
+        char qc = c.q.rsd[qp], tc = c.r.rsd[rp];
+        int mid = (d2 < EQUIV_D2) ? '+' : ' ';
+        if (qc != '-' && qc == tc) { idts++; mid = (mid == '+') ? qc : (qc | 32); }
+        push(qc, mid, tc, c.q.ss[qp], c.r.ss[rp]);
+        if (k + 1 < m) {
+            int qn = (int)s.cqp[idx - 1], rn = (int)s.crp[idx - 1];
+            for (int p = qp + 1; p < qn; p++) { push(c.q.rsd[p], ' ', '-', c.q.ss[p], ' '); gaps++; }
+            if (!c.o->nodeletions)
+                for (int p = rp + 1; p < rn; p++) { push('-', ' ', c.r.rsd[p], ' ', c.r.ss[p]); gaps++; }
+        }
+    }
+    float fb, fe;
+    std::memcpy(&fb, &bcrds, 4); std::memcpy(&fe, &ecrds, 4);
+    s.out[GTA_ALN_BEGCOORDS] = fb;
+    s.out[GTA_ALN_ENDCOORDS] = fe;
+    s.out[GTA_ALN_ALNLENGTH] = (float)s.rows[0].size();
+    s.out[GTA_ALN_PSTVS] = (float)psts;
+    s.out[GTA_ALN_IDNTS] = (float)idts;
+    s.out[GTA_ALN_NGAPS] = (float)gaps;
+}
+
+void init_pair(Ctx& c) {
+    PairState& s = *c.st;
+    s = PairState();
+    set_identity(s.tfm_ibest);
+    set_identity(s.tfm_grand);
+    for (int f = 0; f < GTA_ALN_NFIELDS; f++) s.out[f] = 0.f;
+    s.out[GTA_ALN_RMSD] = 9999.9f;
+    // Preinitialize1Kernel: molecule-type compatibility (MpStage1.cpp:110-125)
+    s.flags = (c.q.na == c.r.na) ? 0 : CONV_LOWTMSC;
+}
+
+// the whole stage for one pair (MpBatch.cpp:283-485 with frg3 and the
+// sequence-similarity filter off, their defaults for this stage)
+void process_pair(Ctx& c) {
+    const gta_options& o = *c.o;
+    init_pair(c);
+    // stage 1 (MpStage1.h:78-85)
+    find_frag(c);
+    refine_frag_init(c);
+    dp_refine(c, true, true, 2, o.prescore);
+    // stage 2, two passes (MpBatch.cpp:477-483)
+    stage2(c, false, true, true, 2, false, o.scorethld);
+    stage2(c, true, false, false, o.convergence, true, o.scorethld);
+    // final stage (MpStageFin.h:66-90)
+    dp_align_grand(c, 0.0f, true, false, false, false);
+    fin_phase1(c);
+    if (!(o.depth == GTA_DEPTH_SHALLOW || o.refinement == GTA_REF_COARSE)) {
+        if (o.refinement == GTA_REF_FULLA) {
+            // Phase2_fullsearch is not part of the default path
+            fin_phase2(c);
+        } else {
+            fin_phase2(c);
+        }
+    }
+    dp_align_grand(c, 0.0f, true, false, false, true);
+    production_alignment(c);
+    production_phase1(c);
+    if (o.refinement == GTA_REF_ONE || o.refinement == GTA_REF_COARSE) production_phase2(c);
+    if (o.referenced && !(c.st->flags & CONV_LOWTMSC)) {
+        // RevertTfmMatricesKernel (MpStageFin.cpp:1835-1880): inverse transform
+        float* t = c.st->tfm_grand;
+        std::swap(t[1], t[3]); std::swap(t[2], t[6]); std::swap(t[5], t[7]);
+        float t0 = t[9], t1 = t[10], t2 = t[11];
+        t[9] = -(t[0] * t0 + t[1] * t1 + t[2] * t2);
+        t[10] = -(t[3] * t0 + t[4] * t1 + t[5] * t2);
+        t[11] = -(t[6] * t0 + t[7] * t1 + t[8] * t2);
+    }
+}
+
+ChunkGeom make_geom(const gta_chunk_view* q, const gta_chunk_view* r, const gta_options* o) {
+    ChunkGeom g;
+    g.qystr1len = q->len[0]; g.dbstr1len = r->len[0];
+    g.qystrnlen = q->len[q->nstrs - 1]; g.dbstrnlen = r->len[r->nstrs - 1];
+    g.minfraglen = GetMinFragLengthForAln(4000);
+    g.stepinit = (o->depth == GTA_DEPTH_SHALLOW) ? 5 : 1;
+    int minlenmin = std::min(g.qystrnlen, g.dbstrnlen);
+    int minalnmin = std::max(minlenmin >> 1, 5);
+    g.n1 = minalnmin - g.dbstr1len;
+    g.n2 = g.qystr1len - minalnmin;
+    int maxalnmax = std::min(g.qystr1len, g.dbstr1len);
+    g.nlocsteps = GetMaxNFragSteps(maxalnmax, FRAGREF_SFRAGSTEP, g.minfraglen) * FRAGREF_NMAXSUBFRAGS;
+    g.maxnfragfcts = std::min(2 * maxalnmax, FIN_MAX_NPOSITIONS);
+    return g;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t gta_oracle_alignment_section_size(int nq, int nqposs, int nr, int nrposs) {
+    return ((size_t)nrposs * nq + (size_t)nr * ((size_t)nqposs + nq)) * GTA_ALN_NROWS;
+}
+
+// The whole stage on CPU.  alndata: nq*nr*15, tfms: nq*nr*12, alns: the
+// alignment section (gta_oracle_alignment_section_size bytes), ss_q/ss_r:
+// optional outputs of the SS pass.  Returns the number of DP cells updated.
+double gta_oracle_process_block(const gta_chunk_view* q, const gta_chunk_view* r,
+                                const gta_options* o, float* alndata, float* tfms, char* alns,
+                                char* ss_q, char* ss_r) {
+    std::vector<char> qss, rss;
+    calc_secstr(q, qss);
+    calc_secstr(r, rss);
+    if (ss_q) std::memcpy(ss_q, qss.data(), q->nposs);
+    if (ss_r) std::memcpy(ss_r, rss.data(), r->nposs);
+    ChunkGeom g = make_geom(q, r, o);
+    const int nq = q->nstrs, nr = r->nstrs;
+    const size_t alnsize = gta_oracle_alignment_section_size(nq, q->nposs, nr, r->nposs);
+    if (alns) std::memset(alns, 0, alnsize);
+    double cells = 0.0;
+    for (int qi = 0; qi < nq; qi++) {
+        for (int ri = 0; ri < nr; ri++) {
+            PairState st;
+            Ctx c;
+            c.o = o; c.g = g;
+            c.q = get_str(q, qss, qi);
+            c.r = get_str(r, rss, ri);
+            c.st = &st;
+            process_pair(c);
+            cells += c.cells;
+            const size_t p = (size_t)qi * nr + ri;
+            std::memcpy(alndata + p * GTA_ALN_NFIELDS, st.out, sizeof(st.out));
+            std::memcpy(tfms + p * GTA_TFM_NFIELDS, st.tfm_grand, 12 * sizeof(float));
+            if (alns && !(st.flags & CONV_LOWTMSC)) {
+                // layout of production_match2aln.cu:130-133 / MpDPHub.cpp:1980-1984
+                const size_t alnofff = ((size_t)qi * r->nposs + (size_t)nr * (q->dst[qi] + qi)) * GTA_ALN_NROWS;
+                const size_t dbalnlen = (size_t)r->nposs + (size_t)nr * (q->len[qi] + 1);
+                const size_t dbalnbeg = (size_t)r->dst[ri] + (size_t)ri * (q->len[qi] + 1);
+                const size_t n = st.rows[0].size();
+                for (int f = 0; f < GTA_ALN_NROWS; f++) {
+                    char* dstp = alns + alnofff + dbalnbeg + dbalnlen * f;
+                    std::memcpy(dstp, st.rows[f].data(), n);
+                    dstp[n] = 0;
+                }
+            }
+        }
+    }
+    return cells;
+}
+
+// ---- kernel-level entry points used by the parity tests --------------------
+
+// DP fill + backtrace for one pair under transform tfm. bt receives Lq*Lr
+// codes; qpos/rpos the matched positions in walk order; returns #matched.
+int gta_oracle_dp(const float* qx, const float* qy, const float* qz, const char* qss, int Lq,
+                  const float* rx, const float* ry, const float* rz, const char* rss, int Lr,
+                  const float* tfm, float d02, float gapopen, int gap0, int usess, float ssw,
+                  unsigned char* bt_out, int* qpos, int* rpos) {
+    Str q, r;
+    q.len = Lq; q.x = qx; q.y = qy; q.z = qz; q.ss = qss; q.rsd = nullptr;
+    r.len = Lr; r.x = rx; r.y = ry; r.z = rz; r.ss = rss; r.rsd = nullptr;
+    std::vector<uint8_t> bt;
+    double cells = 0;
+    dp_fill(q, r, tfm, d02, gapopen, gap0 != 0, usess != 0, ssw, bt, cells);
+    if (bt_out) std::memcpy(bt_out, bt.data(), bt.size());
+    std::vector<int> qp, rp;
+    backtrace(bt, Lq, Lr, qp, rp);
+    for (size_t k = 0; k < qp.size(); k++) { qpos[k] = qp[k]; rpos[k] = rp[k]; }
+    return (int)qp.size();
+}
+
+// Kabsch superposition of n pairs (x -> y) in the reference's FP32 arithmetic.
+void gta_oracle_kabsch(const float* x, const float* y, int n, float* tfm) {
+    float ccm[NCCX] = {0};
+    for (int k = 0; k < n; k++) {
+        const float* a = x + 3 * k; const float* b = y + 3 * k;
+        for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) ccm[3 * i + j] += a[i] * b[j];
+        for (int i = 0; i < 3; i++) { ccm[CVq0 + i] += a[i]; ccm[CVr0 + i] += b[i]; }
+    }
+    ccm[NALN] = (float)n;
+    CalcTfm(0, 0, ccm, false);
+    std::memcpy(tfm, ccm, 12 * sizeof(float));
+}
+
+// RMSD of the optimal superposition from Kabsch's E0 identity (FP32).
+float gta_oracle_rmsd(const float* x, const float* y, int n) {
+    float ccm[NCCX] = {0};
+    for (int k = 0; k < n; k++) {
+        const float* a = x + 3 * k; const float* b = y + 3 * k;
+        for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) ccm[3 * i + j] += a[i] * b[j];
+        for (int i = 0; i < 3; i++) {
+            ccm[CVq0 + i] += a[i]; ccm[CVr0 + i] += b[i];
+            ccm[CV2q0 + i] += a[i] * a[i]; ccm[CV2r0 + i] += b[i] * b[i];
+        }
+    }
+    ccm[NALN] = (float)n;
+    return CalcRMSD(ccm);
+}
+
+void gta_oracle_d0(int qlen, int rlen, float* out) {
+    out[0] = GetD0(qlen, rlen);
+    out[1] = GetD02_dpscan(qlen, rlen);
+    out[2] = GetD0finProtein(qlen, rlen);
+    out[3] = GetD82(qlen, rlen);
+    out[4] = GetD0s(GetD0(qlen, rlen));
+}
+
+int gta_oracle_fraglen(int qlen, int rlen, int qpos, int rpos, int ndx) {
+    return GetFragLength(qlen, rlen, qpos, rpos, ndx);
+}
+
+}  // extern "C"

@@ -1,0 +1,151 @@
+"""ctypes binding of the oracle (TEST INFRASTRUCTURE ONLY).
+
+libgta_oracle.so is the CPU restatement of the reference path
+(oracle/gta_oracle.cpp); _ref/libgta_ref.so is the reference's own CPU path
+compiled from its sources (oracle/Makefile), present only where the
+reference tree was available at build time.  Only tests/, the smoke check
+and bench.py's CPU baseline use this module.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from gtalign_alpha_amd.capi import (ChunkView, Options, GTA_ALN_NFIELDS, GTA_TFM_NFIELDS,
+                                    alignment_section_size, default_options)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_LIB = os.path.join(HERE, "libgta_oracle.so")
+REF_LIB = os.path.join(HERE, "_ref", "libgta_ref.so")
+
+_orc = None
+_ref = None
+_fp = C.POINTER(C.c_float)
+
+
+def lib():
+    global _orc
+    if _orc is None:
+        _orc = C.CDLL(ORACLE_LIB)
+        _orc.gta_oracle_process_block.argtypes = [C.POINTER(ChunkView), C.POINTER(ChunkView),
+                                                  C.POINTER(Options), C.c_void_p, C.c_void_p,
+                                                  C.c_void_p, C.c_void_p, C.c_void_p]
+        _orc.gta_oracle_process_block.restype = C.c_double
+        _orc.gta_oracle_dp.argtypes = ([C.c_void_p] * 4 + [C.c_int] + [C.c_void_p] * 4 +
+                                       [C.c_int, C.c_void_p, C.c_float, C.c_float, C.c_int, C.c_int,
+                                        C.c_float, C.c_void_p, C.c_void_p, C.c_void_p])
+        _orc.gta_oracle_dp.restype = C.c_int
+        _orc.gta_oracle_kabsch.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
+        _orc.gta_oracle_rmsd.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
+        _orc.gta_oracle_rmsd.restype = C.c_float
+        _orc.gta_oracle_d0.argtypes = [C.c_int, C.c_int, C.c_void_p]
+        _orc.gta_oracle_fraglen.argtypes = [C.c_int] * 5
+        _orc.gta_oracle_fraglen.restype = C.c_int
+    return _orc
+
+
+def ref_available():
+    return os.path.exists(REF_LIB)
+
+
+def ref_lib():
+    global _ref
+    if _ref is None:
+        _ref = C.CDLL(REF_LIB)
+        _ref.gta_ref_process_block.argtypes = [C.POINTER(ChunkView), C.POINTER(ChunkView),
+                                               C.POINTER(Options), C.c_int, C.c_void_p, C.c_void_p,
+                                               C.c_void_p, C.c_size_t, C.POINTER(C.c_double)]
+        _ref.gta_ref_process_block.restype = C.c_int
+    return _ref
+
+
+class BlockResult:
+    def __init__(self, nq, nr, alndata, tfms, alns, seconds=0.0, cells=0.0, ss_q=None, ss_r=None):
+        self.nq, self.nr = nq, nr
+        self.alndata = alndata.reshape(nq, nr, GTA_ALN_NFIELDS)
+        self.tfms = tfms.reshape(nq, nr, GTA_TFM_NFIELDS)
+        self.alns = alns
+        self.seconds = seconds
+        self.cells = cells
+        self.ss_q, self.ss_r = ss_q, ss_r
+
+
+def _buffers(q, r):
+    nq, nr = q.nstrs, r.nstrs
+    aln = np.zeros(nq * nr * GTA_ALN_NFIELDS, dtype=np.float32)
+    tfm = np.zeros(nq * nr * GTA_TFM_NFIELDS, dtype=np.float32)
+    alns = np.zeros(alignment_section_size(nq, q.nposs, nr, r.nposs), dtype=np.uint8)
+    return aln, tfm, alns
+
+
+def process_block(q, r, opts=None):
+    """The whole stage on the oracle."""
+    opts = opts or default_options()
+    aln, tfm, alns = _buffers(q, r)
+    ssq = np.zeros(q.nposs, dtype=np.uint8)
+    ssr = np.zeros(r.nposs, dtype=np.uint8)
+    import time
+    t0 = time.perf_counter()
+    cells = lib().gta_oracle_process_block(C.byref(q.view()), C.byref(r.view()), C.byref(opts),
+                                           aln.ctypes.data, tfm.ctypes.data, alns.ctypes.data,
+                                           ssq.ctypes.data, ssr.ctypes.data)
+    dt = time.perf_counter() - t0
+    return BlockResult(q.nstrs, r.nstrs, aln, tfm, alns, dt, cells, ssq, ssr)
+
+
+def ref_process_block(q, r, opts=None, nthreads=1):
+    """The whole stage on the reference's own CPU code (oracle/_ref)."""
+    opts = opts or default_options()
+    aln, tfm, alns = _buffers(q, r)
+    secs = C.c_double(0)
+    rc = ref_lib().gta_ref_process_block(C.byref(q.view()), C.byref(r.view()), C.byref(opts),
+                                         nthreads, aln.ctypes.data, tfm.ctypes.data,
+                                         alns.ctypes.data, alns.nbytes, C.byref(secs))
+    if rc != 0:
+        raise RuntimeError("reference CPU path failed (%d)" % rc)
+    return BlockResult(q.nstrs, r.nstrs, aln, tfm, alns, secs.value)
+
+
+def dp(qxyz, rxyz, tfm, d02, gapopen=0.0, gap0=True, qss=None, rss=None, ssw=0.0):
+    """DP fill + backtrace for one pair; returns (btck[Lq,Lr], matched pairs in walk order)."""
+    qxyz = np.ascontiguousarray(qxyz, np.float32)
+    rxyz = np.ascontiguousarray(rxyz, np.float32)
+    Lq, Lr = len(qxyz), len(rxyz)
+    qc = [np.ascontiguousarray(qxyz[:, i]) for i in range(3)]
+    rc = [np.ascontiguousarray(rxyz[:, i]) for i in range(3)]
+    usess = qss is not None
+    qs = np.frombuffer(qss.encode(), np.uint8).copy() if usess else np.zeros(Lq, np.uint8)
+    rs = np.frombuffer(rss.encode(), np.uint8).copy() if usess else np.zeros(Lr, np.uint8)
+    bt = np.zeros((Lq, Lr), dtype=np.uint8)
+    qp = np.zeros(Lq + Lr, dtype=np.int32)
+    rp = np.zeros(Lq + Lr, dtype=np.int32)
+    t = np.ascontiguousarray(tfm, np.float32)
+    n = lib().gta_oracle_dp(qc[0].ctypes.data, qc[1].ctypes.data, qc[2].ctypes.data, qs.ctypes.data,
+                            Lq, rc[0].ctypes.data, rc[1].ctypes.data, rc[2].ctypes.data,
+                            rs.ctypes.data, Lr, t.ctypes.data, d02, gapopen, int(gap0), int(usess),
+                            ssw, bt.ctypes.data, qp.ctypes.data, rp.ctypes.data)
+    return bt, np.stack([qp[:n], rp[:n]], axis=1)
+
+
+def kabsch(x, y):
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.ascontiguousarray(y, np.float32)
+    t = np.zeros(12, np.float32)
+    lib().gta_oracle_kabsch(x.ctypes.data, y.ctypes.data, len(x), t.ctypes.data)
+    return t
+
+
+def rmsd(x, y):
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.ascontiguousarray(y, np.float32)
+    return float(lib().gta_oracle_rmsd(x.ctypes.data, y.ctypes.data, len(x)))
+
+
+def d0(qlen, rlen):
+    out = np.zeros(5, np.float32)
+    lib().gta_oracle_d0(qlen, rlen, out.ctypes.data)
+    return dict(d0=out[0], d02_dpscan=out[1], d0fin=out[2], d82=out[3], d0s=out[4])
+
+
+def fraglen(qlen, rlen, qpos, rpos, ndx):
+    return lib().gta_oracle_fraglen(qlen, rlen, qpos, rpos, ndx)
