@@ -1,0 +1,122 @@
+"""Host-side mirror of the reference's GPU batch object, over the C ABI.
+
+CuBatch (src/libmycu/cubatch/CuBatch.cuh:38-241) is constructed per device,
+receives a query chunk (TransferQueryPMDataToDevice, CuBatch.cuh:54-62) and a
+reference chunk (TransferCPMDataToDevice, CuBatch.cuh:64-72), and
+ProcessBlock (CuBatch.cuh:74-85) computes per pair TM-scores, RMSD, the
+superposition and the alignment.  GtaBatch keeps those names and that call
+order; errors from the library raise GtaError (the reference throws
+myruntime_error).  The HIP library is required: there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from .capi import (ChunkView, Options, Results, STATUS, GTA_ALN_NFIELDS, GTA_TFM_NFIELDS,
+                   default_options)
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libgta_amd.so")
+_lib = None
+
+
+class GtaError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s: %s" % (STATUS.get(code, code), msg))
+        self.code = code
+
+
+def lib():
+    """The HIP library (fails loudly when it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("%s is not built; run __graft_entry__.build()" % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        L.gta_default_options.argtypes = [C.POINTER(Options)]
+        L.gta_create.argtypes = [C.c_int, C.POINTER(Options), C.c_int, C.c_int, C.c_int, C.c_int,
+                                 C.POINTER(C.c_void_p)]
+        L.gta_create.restype = C.c_int
+        L.gta_destroy.argtypes = [C.c_void_p]
+        L.gta_set_queries.argtypes = [C.c_void_p, C.POINTER(ChunkView)]
+        L.gta_set_queries.restype = C.c_int
+        L.gta_set_references.argtypes = [C.c_void_p, C.POINTER(ChunkView)]
+        L.gta_set_references.restype = C.c_int
+        L.gta_process.argtypes = [C.c_void_p, C.POINTER(Results)]
+        L.gta_process.restype = C.c_int
+        L.gta_process_resident.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.gta_process_resident.restype = C.c_int
+        L.gta_last_error.argtypes = [C.c_void_p]
+        L.gta_last_error.restype = C.c_char_p
+        L.gta_alignment_section_size.argtypes = [C.c_int] * 4
+        L.gta_alignment_section_size.restype = C.c_size_t
+        _lib = L
+    return _lib
+
+
+class BlockResults:
+    """Host copy of the three result sections of one ProcessBlock."""
+
+    def __init__(self, res):
+        nq, nr = res.nqrys, res.nrefs
+        self.nq, self.nr = nq, nr
+        self.alndata = np.ctypeslib.as_array(res.alndata, (nq * nr * GTA_ALN_NFIELDS,)).copy() \
+            .reshape(nq, nr, GTA_ALN_NFIELDS)
+        self.tfms = np.ctypeslib.as_array(res.tfms, (nq * nr * GTA_TFM_NFIELDS,)).copy() \
+            .reshape(nq, nr, GTA_TFM_NFIELDS)
+        self.alns = np.frombuffer(C.string_at(res.alns, res.alnsize), dtype=np.uint8).copy()
+        self.seconds = res.seconds
+        self.cells = res.dp_cells
+
+
+class GtaBatch:
+    def __init__(self, device=0, options=None, max_qrys=100, max_qry_poss=50000, max_refs=1 << 20,
+                 max_ref_poss=1 << 27):
+        self.options = options or default_options()
+        h = C.c_void_p()
+        rc = lib().gta_create(device, C.byref(self.options), max_qrys, max_qry_poss, max_refs,
+                              max_ref_poss, C.byref(h))
+        if rc != 0:
+            raise GtaError(rc, lib().gta_last_error(None).decode())
+        self._h = h
+        self._q = self._r = None
+
+    def _check(self, rc):
+        if rc != 0:
+            raise GtaError(rc, lib().gta_last_error(self._h).decode())
+
+    def TransferQueryPMDataToDevice(self, chunk):
+        self._check(lib().gta_set_queries(self._h, C.byref(chunk.view())))
+        self._q = chunk
+
+    def TransferCPMDataToDevice(self, chunk):
+        self._check(lib().gta_set_references(self._h, C.byref(chunk.view())))
+        self._r = chunk
+
+    def ProcessBlock(self):
+        res = Results()
+        self._check(lib().gta_process(self._h, C.byref(res)))
+        return BlockResults(res)
+
+    def process_resident(self):
+        """Run the stage leaving results in HBM; returns (device seconds, DP cells)."""
+        s, cells = C.c_double(0), C.c_double(0)
+        self._check(lib().gta_process_resident(self._h, C.byref(s), C.byref(cells)))
+        return s.value, cells.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().gta_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

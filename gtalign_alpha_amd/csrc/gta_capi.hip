@@ -1,0 +1,407 @@
+// gta_capi.hip -- implementation of the C ABI (include/gta/gtalign_amd.h).
+//
+// Host orchestration of the stage on one HIP stream: the sequence of
+// MpBatch::ProcessBlock / CuBatch::ProcessBlock (MpBatch.cpp:283-485,
+// CuBatch.cu:464-681) -- stage 1, stage 2 in two passes, final stage -- with
+// every per-pair quantity resident in HBM between kernels.  The sequence-
+// similarity pre-filter and the frg3 search are off, their defaults for this
+// stage; pairs failing the pre-score test are flagged and skipped by every
+// later kernel instead of being compacted out of the chunk.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "gta/gtalign_amd.h"
+#include "gta_launch.h"
+
+using namespace gta;
+
+namespace {
+
+thread_local std::string g_tls_error;
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    hipError_t ensure(size_t n) {
+        if (n <= bytes && p) return hipSuccess;
+        if (p) { (void)hipFree(p); p = nullptr; bytes = 0; }
+        hipError_t e = hipMalloc(&p, std::max<size_t>(n, 256));
+        if (e == hipSuccess) bytes = std::max<size_t>(n, 256);
+        return e;
+    }
+    void release() { if (p) (void)hipFree(p); p = nullptr; bytes = 0; }
+    template <typename T> T* as() const { return (T*)p; }
+};
+
+struct ChunkDev {
+    int nstrs = 0, nposs = 0;
+    std::vector<int> len;
+    std::vector<unsigned> dst;
+    std::vector<int> na;
+    DevBuf b_len, b_dst, b_na, b_x, b_y, b_z, b_rsd, b_ss;
+    DevChunk view() const {
+        DevChunk c;
+        c.nstrs = nstrs; c.nposs = nposs;
+        c.len = b_len.as<int>(); c.dst = b_dst.as<unsigned>(); c.na = b_na.as<int>();
+        c.x = b_x.as<float>(); c.y = b_y.as<float>(); c.z = b_z.as<float>();
+        c.rsd = b_rsd.as<char>(); c.ss = b_ss.as<char>();
+        return c;
+    }
+    void release() {
+        b_len.release(); b_dst.release(); b_na.release(); b_x.release(); b_y.release();
+        b_z.release(); b_rsd.release(); b_ss.release();
+    }
+};
+
+int GetMaxNFragSteps(int maxalnlen, int step, int fraglen) {
+    return (int)ceilf((float)(maxalnlen + step - fraglen) / step);
+}
+
+}  // namespace
+
+struct gta_ctx {
+    int device = 0;
+    gta_options opts;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int max_qrys = 0, max_qposs = 0, max_refs = 0, max_rposs = 0;
+    size_t lds_max = 65536;
+    ChunkDev q, r;
+    bool have_q = false, have_r = false;
+    // pair state
+    DevBuf s_flags, s_grand, s_best0, s_prev, s_naln, s_qpos, s_rpos, s_fndx, s_fpos;
+    DevBuf s_tibest, s_tgrand, s_out, s_aln, s_btck, s_btckoff, s_rowbuf, s_cells, s_alns;
+    // host results
+    std::vector<float> h_out, h_tfm;
+    std::vector<char> h_alns;
+    std::string err;
+};
+
+static int fail(gta_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg; else g_tls_error = msg;
+    return code;
+}
+#define HIPCHK(ctx, expr)                                                                   \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail(ctx, GTA_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+extern "C" {
+
+void gta_default_options(gta_options* o) {
+    if (!o) return;
+    o->prescore = 0.4f;
+    o->scorethld = 0.5f;
+    o->convergence = 18;
+    o->depth = GTA_DEPTH_MEDIUM;
+    o->refinement = GTA_REF_ONE;
+    o->symmetric = 0;
+    o->nodeletions = 0;
+    o->referenced = 0;
+    o->precision = GTA_PREC_FP32;
+}
+
+size_t gta_alignment_section_size(int nq, int nqposs, int nr, int nrposs) {
+    return ((size_t)nrposs * nq + (size_t)nr * ((size_t)nqposs + nq)) * GTA_ALN_NROWS;
+}
+
+const char* gta_last_error(const gta_ctx* ctx) {
+    return ctx ? ctx->err.c_str() : g_tls_error.c_str();
+}
+
+static int check_options(const gta_options* o) {
+    if (!(o->prescore >= 0.f && o->prescore < 1.f)) return 0;
+    if (!(o->scorethld >= 0.f && o->scorethld < 1.f)) return 0;
+    if (o->convergence < 1 || o->convergence > 30) return 0;
+    if (o->depth < 0 || o->depth > GTA_DEPTH_SHALLOW) return 0;
+    if (o->refinement < 0 || o->refinement > GTA_REF_FULLA) return 0;
+    if (o->precision != GTA_PREC_FP32 && o->precision != GTA_PREC_MIXED) return 0;
+    return 1;
+}
+
+int gta_create(int device, const gta_options* opts, int max_qrys, int max_qry_poss, int max_refs,
+               int max_ref_poss, gta_ctx** out) {
+    if (!out) return fail(nullptr, GTA_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    gta_options o;
+    gta_default_options(&o);
+    if (opts) o = *opts;
+    if (!check_options(&o)) return fail(nullptr, GTA_ERR_ARG, "option out of range");
+    if (max_qrys < 1 || max_qry_poss < 1 || max_refs < 1 || max_ref_poss < 1)
+        return fail(nullptr, GTA_ERR_ARG, "invalid capacity");
+    if (max_qrys > 100) return fail(nullptr, GTA_ERR_TOOMANY, "more than 100 queries per chunk");
+    hipError_t e = hipSetDevice(device);
+    if (e != hipSuccess) return fail(nullptr, GTA_ERR_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
+    gta_ctx* c = new gta_ctx();
+    c->device = device;
+    c->opts = o;
+    c->max_qrys = max_qrys; c->max_qposs = max_qry_poss;
+    c->max_refs = max_refs; c->max_rposs = max_ref_poss;
+    int lds = 0;
+    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
+        c->lds_max = (size_t)lds;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        delete c;
+        return fail(nullptr, GTA_ERR_HIP, "stream/event creation failed");
+    }
+    *out = c;
+    return GTA_OK;
+}
+
+void gta_destroy(gta_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    c->q.release(); c->r.release();
+    DevBuf* bufs[] = {&c->s_flags, &c->s_grand, &c->s_best0, &c->s_prev, &c->s_naln, &c->s_qpos,
+                      &c->s_rpos, &c->s_fndx, &c->s_fpos, &c->s_tibest, &c->s_tgrand, &c->s_out,
+                      &c->s_aln, &c->s_btck, &c->s_btckoff, &c->s_rowbuf, &c->s_cells, &c->s_alns};
+    for (DevBuf* b : bufs) b->release();
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+static int upload_chunk(gta_ctx* c, ChunkDev& d, const gta_chunk_view* v, int maxstrs, int maxposs) {
+    if (!v || !v->len || !v->dst || !v->x || !v->y || !v->z || !v->rsd)
+        return fail(c, GTA_ERR_ARG, "null chunk field");
+    if (v->nstrs < 1 || v->nposs < 1) return fail(c, GTA_ERR_EMPTY, "no structures in the chunk");
+    if (v->nstrs > maxstrs || v->nposs > maxposs)
+        return fail(c, GTA_ERR_SIZE, "chunk larger than the context capacity");
+    long long tot = 0;
+    for (int i = 0; i < v->nstrs; i++) {
+        if (v->len[i] < 1) return fail(c, GTA_ERR_ARG, "empty structure");
+        if (v->len[i] > 65535) return fail(c, GTA_ERR_LENGTH, "structure longer than 65535 residues");
+        if (v->dst[i] != (unsigned)tot) return fail(c, GTA_ERR_ARG, "structure offsets are not prefix sums");
+        if (i && v->len[i] > v->len[i - 1]) return fail(c, GTA_ERR_ARG, "structures not sorted by descending length");
+        tot += v->len[i];
+    }
+    if (tot != v->nposs) return fail(c, GTA_ERR_ARG, "nposs does not match the lengths");
+    (void)hipSetDevice(c->device);
+    d.nstrs = v->nstrs; d.nposs = v->nposs;
+    d.len.assign(v->len, v->len + v->nstrs);
+    d.dst.assign(v->dst, v->dst + v->nstrs);
+    d.na.assign(v->nstrs, 0);
+    if (v->ins_ch_ord)
+        for (int i = 0; i < v->nstrs; i++) d.na[i] = v->ins_ch_ord[v->dst[i]] > 0;
+    const size_t ns = v->nstrs, np = v->nposs;
+    HIPCHK(c, d.b_len.ensure(ns * 4));
+    HIPCHK(c, d.b_dst.ensure(ns * 4));
+    HIPCHK(c, d.b_na.ensure(ns * 4));
+    HIPCHK(c, d.b_x.ensure(np * 4));
+    HIPCHK(c, d.b_y.ensure(np * 4));
+    HIPCHK(c, d.b_z.ensure(np * 4));
+    HIPCHK(c, d.b_rsd.ensure(np));
+    HIPCHK(c, d.b_ss.ensure(np));
+    HIPCHK(c, hipMemcpy(d.b_len.p, d.len.data(), ns * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(d.b_dst.p, d.dst.data(), ns * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(d.b_na.p, d.na.data(), ns * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(d.b_x.p, v->x, np * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(d.b_y.p, v->y, np * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(d.b_z.p, v->z, np * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(d.b_rsd.p, v->rsd, np, hipMemcpyHostToDevice));
+    return GTA_OK;
+}
+
+int gta_set_queries(gta_ctx* c, const gta_chunk_view* q) {
+    if (!c) return fail(nullptr, GTA_ERR_ARG, "null context");
+    if (q && q->nstrs > 100) return fail(c, GTA_ERR_TOOMANY, "more than 100 queries per chunk");
+    int rc = upload_chunk(c, c->q, q, c->max_qrys, c->max_qposs);
+    c->have_q = (rc == GTA_OK);
+    return rc;
+}
+
+int gta_set_references(gta_ctx* c, const gta_chunk_view* r) {
+    if (!c) return fail(nullptr, GTA_ERR_ARG, "null context");
+    int rc = upload_chunk(c, c->r, r, c->max_refs, c->max_rposs);
+    c->have_r = (rc == GTA_OK);
+    return rc;
+}
+
+// grid sizes of the chunk (MpStage1.cpp:284-296,506-516; MpStageFin.cpp:304-309)
+static Geom make_geom(const gta_ctx* c) {
+    Geom g;
+    const int qystr1len = c->q.len.front(), qystrnlen = c->q.len.back();
+    const int dbstr1len = c->r.len.front(), dbstrnlen = c->r.len.back();
+    const int minlenmin = std::min(qystrnlen, dbstrnlen);
+    const int minalnmin = std::max(minlenmin >> 1, 5);
+    g.stepinit = (c->opts.depth == GTA_DEPTH_SHALLOW) ? 5 : 1;
+    g.n1 = minalnmin - dbstr1len;
+    const int n2 = qystr1len - minalnmin;
+    g.nseeds = (n2 - g.n1) / g.stepinit + 1;
+    const int maxalnmax = std::min(qystr1len, dbstr1len);
+    const int minfraglen = 4;  // GetMinFragLengthForAln(GetMaxAlnLength()) for any length >= 64
+    g.nlocsteps = GetMaxNFragSteps(maxalnmax, SFRAGSTEP, minfraglen) * NMAXSUBFRAGS;
+    g.maxnfragfcts = std::min(2 * maxalnmax, 128);
+    return g;
+}
+
+static int run_stage(gta_ctx* c, double* seconds, double* cells) {
+    if (!c) return fail(nullptr, GTA_ERR_ARG, "null context");
+    if (!c->have_q || !c->have_r) return fail(c, GTA_ERR_STATE, "queries and references must be set first");
+    HIPCHK(c, hipSetDevice(c->device));
+    const int nq = c->q.nstrs, nr = c->r.nstrs;
+    const size_t np = (size_t)nq * nr;
+    const int prec = c->opts.precision;
+    const int dinv = c->opts.symmetric;
+
+    // pair state
+    HIPCHK(c, c->s_flags.ensure(np * 4));
+    HIPCHK(c, c->s_grand.ensure(np * 4));
+    HIPCHK(c, c->s_best0.ensure(np * 4));
+    HIPCHK(c, c->s_prev.ensure(np * 4));
+    HIPCHK(c, c->s_naln.ensure(np * 4));
+    HIPCHK(c, c->s_qpos.ensure(np * 4));
+    HIPCHK(c, c->s_rpos.ensure(np * 4));
+    HIPCHK(c, c->s_fndx.ensure(np * 4));
+    HIPCHK(c, c->s_fpos.ensure(np * 4));
+    HIPCHK(c, c->s_tibest.ensure(np * 12 * 4));
+    HIPCHK(c, c->s_tgrand.ensure(np * 12 * 4));
+    HIPCHK(c, c->s_out.ensure(np * 15 * 4));
+    const size_t alnplane = (size_t)nq * c->r.nposs;
+    HIPCHK(c, c->s_aln.ensure(alnplane * 9 * 4));
+    HIPCHK(c, c->s_cells.ensure(8));
+    // backtrace: per pair ceil(Lq/64) strips x ceil((Lr+63)/16) blocks x 64 words
+    std::vector<unsigned long long> off(np);
+    unsigned long long tot = 0;
+    int maxlr = 0;
+    for (int ri = 0; ri < nr; ri++) maxlr = std::max(maxlr, c->r.len[ri]);
+    for (int qi = 0; qi < nq; qi++) {
+        const unsigned long long ns = (unsigned long long)((c->q.len[qi] + 63) / 64);
+        for (int ri = 0; ri < nr; ri++) {
+            off[(size_t)qi * nr + ri] = tot;
+            tot += ns * (unsigned long long)((c->r.len[ri] + 63 + 15) / 16) * 64ull;
+        }
+    }
+    HIPCHK(c, c->s_btck.ensure(tot * 4));
+    HIPCHK(c, c->s_btckoff.ensure(np * 8));
+    HIPCHK(c, hipMemcpyAsync(c->s_btckoff.p, off.data(), np * 8, hipMemcpyHostToDevice, c->stream));
+    const size_t dp_lds = (size_t)maxlr * 20;
+    const bool dp_in_lds = dp_lds <= std::min<size_t>(c->lds_max, 65536);
+    if (!dp_in_lds) HIPCHK(c, c->s_rowbuf.ensure(alnplane * 4));
+    const size_t alnsize = gta_alignment_section_size(nq, c->q.nposs, nr, c->r.nposs);
+    HIPCHK(c, c->s_alns.ensure(alnsize));
+
+    DevState S;
+    S.nq = nq; S.nr = nr;
+    S.flags = c->s_flags.as<int>(); S.grand = c->s_grand.as<float>(); S.best0 = c->s_best0.as<float>();
+    S.prevbest0 = c->s_prev.as<float>(); S.nalnposs = c->s_naln.as<int>();
+    S.qrypos = c->s_qpos.as<int>(); S.rfnpos = c->s_rpos.as<int>();
+    S.sfragndx = c->s_fndx.as<int>(); S.sfragpos = c->s_fpos.as<int>();
+    S.tfm_ibest = c->s_tibest.as<float>(); S.tfm_grand = c->s_tgrand.as<float>();
+    S.out = c->s_out.as<float>();
+    S.aln = c->s_aln.as<float>(); S.alnplane = alnplane;
+    S.btck = c->s_btck.as<unsigned>(); S.btckoff = c->s_btckoff.as<unsigned long long>();
+    S.rowbuf = dp_in_lds ? nullptr : c->s_rowbuf.as<float>();
+    S.cells = c->s_cells.as<unsigned long long>();
+
+    const DevChunk Q = c->q.view(), R = c->r.view();
+    const Geom G = make_geom(c);
+    const gta_options& o = c->opts;
+    hipStream_t st = c->stream;
+
+    int maxaln = std::min(c->q.len.front(), c->r.len.front());
+    RefineLaunch RL{dinv, 0, 0, 0, 0.f, std::min(maxaln, 1536)};
+    auto dp = [&](int mode, int gap0, float gapopen, int usess, int dpscan) -> hipError_t {
+        DpLaunch L{mode, gap0, gapopen, usess, usess ? 0.5f : 0.0f, dpscan, dp_in_lds ? dp_lds : 0};
+        return launch_dp(Q, R, S, L, st);
+    };
+    auto refine = [&](int mode, int epi, int dpi, int maxdpi, float prescore) -> hipError_t {
+        RefineLaunch L = RL;
+        L.epi = epi; L.dpi = dpi; L.maxndpiters = maxdpi; L.prescore = prescore;
+        return launch_refine_dp(prec, mode, Q, R, S, G, L, st);
+    };
+    // DPRefine<GAP0,PRESCREEN> (MpStage1.cpp:455-510)
+    auto dprefine = [&](int gap0, float prescore, int maxdpi) -> int {
+        for (int dpi = 0; dpi < maxdpi; dpi++) {
+            HIPCHK(c, dp(DPM_IBEST_L, gap0, gap0 ? 0.0f : -0.6f, 0, 0));
+            HIPCHK(c, refine(RM_DPREFINE_L, 1, dpi, maxdpi, prescore));
+        }
+        return GTA_OK;
+    };
+
+    HIPCHK(c, hipMemsetAsync(c->s_cells.p, 0, 8, st));
+    HIPCHK(c, hipMemsetAsync(c->s_alns.p, 0, alnsize, st));
+    HIPCHK(c, hipEventRecord(c->ev0, st));
+    HIPCHK(c, launch_secstr(Q, st));
+    HIPCHK(c, launch_secstr(R, st));
+    HIPCHK(c, launch_init(Q, R, S, st));
+    // stage 1 (MpStage1.h:78-85)
+    HIPCHK(c, launch_findfrag(prec, Q, R, S, G, dinv, st));
+    HIPCHK(c, launch_refine_gapless(prec, Q, R, S, G, dinv, st));
+    int rc;
+    if ((rc = dprefine(1, o.prescore, 2)) != GTA_OK) return rc;
+    // stage 2, pass 1: SS-weighted DP with gap cost (MpStage2.h:60-80)
+    HIPCHK(c, dp(DPM_GRAND_L, 0, -1.0f, 1, 1));
+    HIPCHK(c, refine(RM_DPREFINE_L, 0, 0, 0, 0.f));
+    if ((rc = dprefine(0, 0.f, 2)) != GTA_OK) return rc;
+    if ((rc = dprefine(1, 0.f, 2)) != GTA_OK) return rc;
+    // stage 2, pass 2: no gap cost, convergence iterations
+    HIPCHK(c, dp(DPM_GRAND_L, 1, 0.0f, 0, 0));
+    HIPCHK(c, refine(RM_DPREFINE_L, 0, 0, 0, 0.f));
+    if ((rc = dprefine(0, 0.f, 2)) != GTA_OK) return rc;
+    if ((rc = dprefine(1, 0.f, o.convergence)) != GTA_OK) return rc;
+    if (0.0f < o.scorethld) HIPCHK(c, launch_setlow(Q, R, S, o.scorethld, st));
+    // final stage (MpStageFin.h:66-90)
+    HIPCHK(c, dp(DPM_GRAND_L, 1, 0.0f, 0, 0));
+    HIPCHK(c, refine(RM_FIN1_L, 0, 0, 0, 0.f));
+    if (!(o.depth == GTA_DEPTH_SHALLOW || o.refinement == GTA_REF_COARSE))
+        HIPCHK(c, refine(RM_FIN2_L, 0, 0, 0, 0.f));
+    HIPCHK(c, dp(DPM_CONSTRAINED_L, 1, 0.0f, 0, 0));
+    HIPCHK(c, launch_prod_aln(Q, R, S, c->s_alns.as<char>(), o.nodeletions, st));
+    HIPCHK(c, refine(RM_PROD1_L, 0, 0, 0, 0.f));
+    if (o.refinement == GTA_REF_ONE || o.refinement == GTA_REF_COARSE)
+        HIPCHK(c, refine(RM_PROD2_L, 0, 0, 0, 0.f));
+    if (o.referenced) HIPCHK(c, launch_revert(S, st));
+    HIPCHK(c, hipEventRecord(c->ev1, st));
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    unsigned long long ncells = 0;
+    HIPCHK(c, hipMemcpy(&ncells, c->s_cells.p, 8, hipMemcpyDeviceToHost));
+    if (seconds) *seconds = ms * 1e-3;
+    if (cells) *cells = (double)ncells;
+    return GTA_OK;
+}
+
+int gta_process_resident(gta_ctx* c, double* seconds, double* dp_cells) {
+    return run_stage(c, seconds, dp_cells);
+}
+
+int gta_process(gta_ctx* c, gta_results* out) {
+    if (!out) return fail(c, GTA_ERR_ARG, "null results pointer");
+    double secs = 0, cells = 0;
+    int rc = run_stage(c, &secs, &cells);
+    if (rc != GTA_OK) return rc;
+    const int nq = c->q.nstrs, nr = c->r.nstrs;
+    const size_t np = (size_t)nq * nr;
+    const size_t alnsize = gta_alignment_section_size(nq, c->q.nposs, nr, c->r.nposs);
+    c->h_out.resize(np * 15);
+    c->h_tfm.resize(np * 12);
+    c->h_alns.resize(alnsize);
+    HIPCHK(c, hipMemcpy(c->h_out.data(), c->s_out.p, np * 15 * 4, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(c->h_tfm.data(), c->s_tgrand.p, np * 12 * 4, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(c->h_alns.data(), c->s_alns.p, alnsize, hipMemcpyDeviceToHost));
+    out->nqrys = nq;
+    out->nrefs = nr;
+    out->alndata = c->h_out.data();
+    out->tfms = c->h_tfm.data();
+    out->alns = c->h_alns.data();
+    out->alnsize = alnsize;
+    out->seconds = secs;
+    out->dp_cells = cells;
+    return GTA_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,255 @@
+// gta_device.h -- device-side data model and math of the alignment stage.
+//
+// Arithmetic that decides DP alignments (transform, squared distance, pair
+// score) is written operation by operation in the order the reference CPU
+// path evaluates it (transformbase.h:40-109, scoringbase.h:61-65), and the
+// translation unit is compiled with FP contraction off, so that for identical
+// superpositions the DP matrices -- and hence the alignments -- are bit-exact.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define GTA_WAVE 64
+
+namespace gta {
+
+// ---- constants (reference file:line) ----------------------------------------
+constexpr float CP_LARGEDST = 999999.9f;          // stagecnsts.cuh:17
+constexpr float CP_LARGEDST_cmp = 399999.9f;      // stagecnsts.cuh:19
+constexpr float RM_CONVEPSILON = 1.e-5f;          // stagecnsts.cuh:42
+constexpr float SCORE_CONVEPSILON = 1.e-6f;       // scoringbase.h:36
+constexpr float TFMEPSILON = 1.e-8f;              // transformbase.h:16
+constexpr float TFMTOL = 0.01f;                   // transformbase.h:17
+constexpr int NMAXSUBFRAGS = 6;                   // mpprocconfbase.h:28
+constexpr int SFRAGSTEP = 40;                     // mpprocconfbase.h:30
+constexpr int NMAXCONVIT = 20;                    // mpprocconfbase.h:33
+constexpr float DEFCOORD_QRY_cmp = 99990.0f;      // mpdpbase.h:26
+constexpr float DEFCOORD_RFN_cmp = -99990.0f;     // mpdpbase.h:29
+constexpr float EQUIV_D2 = 25.0f;                 // gproc.h:10, squared
+
+enum { CONV_SCOREDP = 2, CONV_LOWTMSC = 8 };      // gproc.h:39-49
+enum { BT_STOP = 0, BT_LEFT = 1, BT_UP = 2, BT_DIAG = 3 };  // gproc.h:180-185
+
+// covariance record: 9 cross terms, query sum, reference sum, count
+enum { CCM00 = 0, CVq0 = 9, CVr0 = 12, NALN = 15, NCC = 16 };
+
+// ---- chunk and pair-state views passed to every kernel -----------------------
+struct DevChunk {
+    int nstrs, nposs;
+    const int* len;
+    const unsigned* dst;
+    const int* na;          // per structure: 1 = nucleic acid
+    const float* x;
+    const float* y;
+    const float* z;
+    const char* rsd;
+    char* ss;
+};
+
+struct Geom {
+    int n1, stepinit, nseeds;   // seed diagonals n1 + k*stepinit, k < nseeds (MpStage1.cpp:286-296)
+    int nlocsteps;              // refinement slots (MpStage1.cpp:512-516)
+    int maxnfragfcts;           // final-stage positions (MpStageFin.cpp:308-309)
+};
+
+struct DevState {
+    int nq, nr;
+    int* flags;
+    float* grand;
+    float* best0;
+    float* prevbest0;
+    int* nalnposs;
+    int* qrypos;
+    int* rfnpos;
+    int* sfragndx;
+    int* sfragpos;
+    float* tfm_ibest;       // [pair][12]
+    float* tfm_grand;       // [pair][12]
+    float* out;             // [pair][15]
+    // matched pairs, reverse alignment order, region of pair (q,r) starts at
+    // q*nrposs + dst[r]; planes: qx qy qz rx ry rz d2 qpos rpos
+    float* aln;
+    size_t alnplane;
+    unsigned* btck;         // packed backtrace codes
+    const unsigned long long* btckoff;  // per-pair offsets (in words)
+    float* rowbuf;          // strip boundary rows for long references, laid out like aln planes
+    unsigned long long* cells;   // DP cells updated (device counter)
+};
+
+__device__ __forceinline__ float sqrd(float x) { return x * x; }
+
+// ---- d0 / d8 (stagecnsts.cuh:48-175) -----------------------------------------
+__device__ __forceinline__ float GetLnorm(int q, int r) { return (float)min(q, r); }
+__device__ __forceinline__ float GetD82(int q, int r) {
+    float d8 = 1.5f * powf(GetLnorm(q, r), 0.3f) + 3.5f;
+    return d8 * d8;
+}
+__device__ __forceinline__ float GetD0(int q, int r) {
+    float l = GetLnorm(q, r), d0 = 0.168f;
+    if (l > 19.f) d0 = 1.24f * cbrtf(l - 15.f) - 1.8f;
+    return d0 + 0.8f;
+}
+__device__ __forceinline__ float GetD0finP(int q, int r) {
+    float l = GetLnorm(q, r), d0 = 0.5f;
+    if (l > 21.f) d0 = 1.24f * cbrtf(l - 15.f) - 1.8f;
+    if (d0 < 0.5f) d0 = 0.5f;
+    return d0;
+}
+__device__ __forceinline__ float GetD0finNA(int q, int r) {
+    float l = GetLnorm(q, r), d0 = 0.3f;
+    if (30.f <= l) d0 = 0.6f * sqrtf(l - 0.5f) - 2.5f;
+    else if (23.f < l) d0 = 0.7f;
+    else if (19.f < l) d0 = 0.6f;
+    else if (15.f < l) d0 = 0.5f;
+    else if (11.f < l) d0 = 0.4f;
+    return fmaxf(0.3f, d0);
+}
+__device__ __forceinline__ float GetD0fin(int q, int r, int na) {
+    return na ? GetD0finNA(q, r) : GetD0finP(q, r);
+}
+__device__ __forceinline__ float GetD0s(float d0) { return fmaxf(fminf(d0, 8.f), 4.5f); }
+
+// ---- fragment geometry (fragment.cuh:19-109) --------------------------------
+__device__ __forceinline__ int GetFragLength(int qlen, int rlen, int qpos, int rpos, int ndx) {
+    int aln = min(qlen - qpos, rlen - rpos);
+    if (aln < 4 && ndx == 0) return aln;
+    aln >>= ndx;
+    if (aln == 2 && 0 < ndx && ((aln >> (ndx - 1)) & 1)) aln = 4;
+    if (aln < 3) return 0;
+    if (aln < 4) aln = 4;
+    if (ndx + 1 == NMAXSUBFRAGS) aln = 4;
+    return aln;
+}
+__device__ __forceinline__ void UpdateLengths(int& qlen, int& rlen, int& qpos, int& rpos, int fl) {
+    int qe = qpos + fl, re = rpos + fl;
+    int delta = max(qe - qlen, re - rlen);
+    if (0 < delta) { qpos -= delta; rpos -= delta; }
+    else { qlen = qe; rlen = re; }
+}
+
+// ---- transform / distance / score, reference evaluation order --------------
+__device__ __forceinline__ void tfm_point(const float* t, float& x0, float& x1, float& x2) {
+    float x = x0, y = x1, z = x2;
+    x0 = __fadd_rn(__fadd_rn(__fadd_rn(__fmul_rn(t[0], x), __fmul_rn(t[1], y)), __fmul_rn(t[2], z)), t[9]);
+    x1 = __fadd_rn(__fadd_rn(__fadd_rn(__fmul_rn(t[3], x), __fmul_rn(t[4], y)), __fmul_rn(t[5], z)), t[10]);
+    x2 = __fadd_rn(__fadd_rn(__fadd_rn(__fmul_rn(t[6], x), __fmul_rn(t[7], y)), __fmul_rn(t[8], z)), t[11]);
+}
+__device__ __forceinline__ float dist2(float a0, float a1, float a2, float b0, float b1, float b2) {
+    float d0 = __fsub_rn(a0, b0), d1 = __fsub_rn(a1, b1), d2 = __fsub_rn(a2, b2);
+    return __fadd_rn(__fadd_rn(__fmul_rn(d0, d0), __fmul_rn(d1, d1)), __fmul_rn(d2, d2));
+}
+__device__ __forceinline__ float tfm_dist2(const float* t, float qx, float qy, float qz, float rx,
+                                           float ry, float rz) {
+    tfm_point(t, qx, qy, qz);
+    return dist2(qx, qy, qz, rx, ry, rz);
+}
+// correctly rounded quotient, as the reference's IEEE division
+__device__ __forceinline__ float pair_score(float d02, float d2) {
+    return __fdiv_rn(d02, __fadd_rn(d02, d2));
+}
+
+// ---- wave64 collectives -------------------------------------------------------
+__device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, GTA_WAVE);
+    return v;
+}
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, GTA_WAVE);
+    return v;
+}
+__device__ __forceinline__ int wave_sum_i(int v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, GTA_WAVE);
+    return v;
+}
+
+// three smallest values of a multiset, kept sorted per lane
+struct Min3 {
+    float a, b, c;
+    __device__ void init() { a = b = c = CP_LARGEDST; }
+    __device__ void put(float d) {
+        if (d < a) { c = b; b = a; a = d; }
+        else if (d < b) { c = b; b = d; }
+        else if (d < c) c = d;
+    }
+    __device__ float reduce_third() {
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) {
+            float x = __shfl_xor(a, o, GTA_WAVE), y = __shfl_xor(b, o, GTA_WAVE), z = __shfl_xor(c, o, GTA_WAVE);
+            put(x); put(y); put(z);
+        }
+        return c;
+    }
+};
+// three smallest distinct values
+struct Min3d {
+    float a, b, c;
+    __device__ void init() { a = b = c = CP_LARGEDST; }
+    __device__ void put(float d) {
+        if (d == a || d == b || d == c) return;
+        if (d < a) { c = b; b = a; a = d; }
+        else if (d < b) { c = b; b = d; }
+        else if (d < c) c = d;
+    }
+    __device__ float reduce_third() {
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) {
+            float x = __shfl_xor(a, o, GTA_WAVE), y = __shfl_xor(b, o, GTA_WAVE), z = __shfl_xor(c, o, GTA_WAVE);
+            put(x); put(y); put(z);
+        }
+        return c;
+    }
+};
+
+// ---- covariance accumulators ----------------------------------------------------
+// plain FP32 (the reference's arithmetic)
+struct AccF32 {
+    float s[15];
+    __device__ void clear() {
+#pragma unroll
+        for (int i = 0; i < 15; i++) s[i] = 0.f;
+    }
+    __device__ void add(float qx, float qy, float qz, float rx, float ry, float rz) {
+        s[0] += qx * rx; s[1] += qx * ry; s[2] += qx * rz;
+        s[3] += qy * rx; s[4] += qy * ry; s[5] += qy * rz;
+        s[6] += qz * rx; s[7] += qz * ry; s[8] += qz * rz;
+        s[9] += qx; s[10] += qy; s[11] += qz;
+        s[12] += rx; s[13] += ry; s[14] += rz;
+    }
+    __device__ void reduce(float* out) {
+#pragma unroll
+        for (int i = 0; i < 15; i++) out[i] = wave_sum(s[i]);
+    }
+};
+// compensated FP32 per lane (Kahan), combined across lanes in FP64
+struct AccKahan {
+    float s[15], c[15];
+    __device__ void clear() {
+#pragma unroll
+        for (int i = 0; i < 15; i++) { s[i] = 0.f; c[i] = 0.f; }
+    }
+    __device__ __forceinline__ void k(int i, float v) {
+        float y = __fsub_rn(v, c[i]);
+        float t = __fadd_rn(s[i], y);
+        c[i] = __fsub_rn(__fsub_rn(t, s[i]), y);
+        s[i] = t;
+    }
+    __device__ void add(float qx, float qy, float qz, float rx, float ry, float rz) {
+        k(0, qx * rx); k(1, qx * ry); k(2, qx * rz);
+        k(3, qy * rx); k(4, qy * ry); k(5, qy * rz);
+        k(6, qz * rx); k(7, qz * ry); k(8, qz * rz);
+        k(9, qx); k(10, qy); k(11, qz);
+        k(12, rx); k(13, ry); k(14, rz);
+    }
+    __device__ void reduce(double* out) {
+#pragma unroll
+        for (int i = 0; i < 15; i++) out[i] = wave_sum((double)s[i] - (double)c[i]);
+    }
+};
+
+}  // namespace gta

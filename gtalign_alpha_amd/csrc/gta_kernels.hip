@@ -1,0 +1,978 @@
+// gta_kernels.hip -- CDNA4 (gfx950) kernels of the structure-alignment stage.
+//
+// Every kernel works on one (query, reference) pair per workgroup; pair p is
+// (p / nr, p % nr).  Per-pair state lives in HBM (DevState) between kernels,
+// so the host never synchronises inside a chunk.
+//
+// Reference correspondence (kernel -> reference functions it replaces):
+//   k_secstr          MpSecStr::ssk_kernel_helper            MpSecStr.cpp:20-110
+//   k_init            MpStage1::Preinitialize1Kernel         MpStage1.cpp:28-170
+//   k_findfrag        MpStage1::FindFragKernel               MpStage1.cpp:275-440
+//   k_refine_gapless  MpStage1::RefineFragInitKernel         MpStage1.cpp:470-640
+//   k_dp              MpDPHub::ExecDPwBtck128xKernel,        MpDPHub.cpp:73-388
+//                     ExecDPTFMSSwBtck128xKernel,            MpDPHub.cpp:687-984
+//                     BtckToMatched128xKernel,               MpDPHub.cpp:1500-1610
+//                     ConstrainedBtckToMatched128xKernel     MpDPHub.cpp:1620-1810
+//   k_refine_dp       MpStage1::RefineFragDPKernel (+ the    MpStage1.cpp:780-960,1070-1215
+//                     DPRefine bookkeeping kernels),
+//                     MpStageFin Final...Phase1/Phase2,      MpStageFin.cpp:44-470
+//                     Production...Phase1/Phase2             MpStageFin.cpp:747-1470
+//   k_prod_aln        MpDPHub::ProductionMatchToAlignment... MpDPHub.cpp:1919-2142
+//   k_setlow          MpStage1::SetLowScoreConvergenceFlag   MpStage1.cpp:1175-1215
+//   k_revert          MpStageFin::RevertTfmMatricesKernel    MpStageFin.cpp:1835-1880
+//
+// Precision P: 0 = FP32 throughout (the reference's arithmetic), 1 = mixed
+// (per-lane compensated FP32 accumulation combined in FP64, FP64 3x3 solve,
+// FP64 RMSD).  DP cell arithmetic is FP32 in both and rounds exactly like the
+// reference's scalar code.
+
+#include "gta_device.h"
+#include "gta_kabsch.h"
+
+namespace gta {
+
+constexpr int RWAVES = 4;                 // waves per workgroup in the refinement kernels
+constexpr int RTHREADS = RWAVES * GTA_WAVE;
+
+__device__ __forceinline__ int pair_q(const DevState& S, int p) { return p / S.nr; }
+__device__ __forceinline__ int pair_r(const DevState& S, int p) { return p % S.nr; }
+
+template <int P> struct Prec;
+template <> struct Prec<0> { typedef float T; typedef AccF32 Acc; };
+template <> struct Prec<1> { typedef double T; typedef AccKahan Acc; };
+
+// sum of per-lane FP32 partial sums; P=1 keeps a Kahan compensation per lane
+template <int P> struct ScoreAcc;
+template <> struct ScoreAcc<0> {
+    float s;
+    __device__ void clear() { s = 0.f; }
+    __device__ void add(float v) { s += v; }
+    __device__ float reduce() { return wave_sum(s); }
+};
+template <> struct ScoreAcc<1> {
+    float s, c;
+    __device__ void clear() { s = 0.f; c = 0.f; }
+    __device__ void add(float v) {
+        float y = __fsub_rn(v, c), t = __fadd_rn(s, y);
+        c = __fsub_rn(__fsub_rn(t, s), y);
+        s = t;
+    }
+    __device__ float reduce() { return (float)wave_sum((double)s - (double)c); }
+};
+
+// convergence of the covariance record (CheckConvergenceRefined_Complete,
+// MpStageBase.h:262-277): all 16 fields changed by less than 1e-5 relative
+template <typename T>
+__device__ __forceinline__ bool cc_converged(const T* a, T na, const T* b, T nb) {
+    int n = 0;
+#pragma unroll
+    for (int f = 0; f < 15; f++) {
+        T d = a[f] - b[f];
+        T m = t_abs(a[f]) < t_abs(b[f]) ? t_abs(a[f]) : t_abs(b[f]);
+        if (t_abs(d) < m * (T)RM_CONVEPSILON) n++;
+    }
+    {
+        T d = na - nb;
+        T m = t_abs(na) < t_abs(nb) ? t_abs(na) : t_abs(nb);
+        if (t_abs(d) < m * (T)RM_CONVEPSILON) n++;
+    }
+    return n >= 16;
+}
+
+// ============================================================================
+// secondary structure (protein C-alpha distance rule, MpSecStr.h:150-180)
+__global__ void k_secstr(DevChunk C) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= C.nposs) return;
+    int lo = 0, hi = C.nstrs - 1;
+    while (lo < hi) {  // last structure whose offset is <= g
+        const int mid = (lo + hi + 1) >> 1;
+        if ((int)C.dst[mid] <= g) lo = mid; else hi = mid - 1;
+    }
+    const int s = lo;
+    const int L = C.len[s];
+    const int D = (int)C.dst[s];
+    const int j = g - D;
+    if (C.na[s]) return;
+    char v = ' ';
+    if (2 <= j && j < L - 2) {
+        auto d = [&](int a, int b) {
+            return sqrtf(dist2(C.x[D + a], C.y[D + a], C.z[D + a], C.x[D + b], C.y[D + b], C.z[D + b]));
+        };
+        float d13 = d(j - 2, j), d14 = d(j - 2, j + 1), d15 = d(j - 2, j + 2);
+        float d24 = d(j - 1, j + 1), d25 = d(j - 1, j + 2), d35 = d(j, j + 2);
+        float e = 2.1f;
+        if (fabsf(d15 - 6.37f) < e && fabsf(d14 - 5.18f) < e && fabsf(d25 - 5.18f) < e &&
+            fabsf(d13 - 5.45f) < e && fabsf(d24 - 5.45f) < e && fabsf(d35 - 5.45f) < e)
+            v = 'h';
+        else {
+            e = 1.42f;
+            if (fabsf(d15 - 13.0f) < e && fabsf(d14 - 10.4f) < e && fabsf(d25 - 10.4f) < e &&
+                fabsf(d13 - 6.1f) < e && fabsf(d24 - 6.1f) < e && fabsf(d35 - 6.1f) < e)
+                v = 'e';
+            else if (d15 < 8.0f)
+                v = 't';
+        }
+    }
+    C.ss[D + j] = v;
+}
+
+// ============================================================================
+__global__ void k_init(DevChunk Q, DevChunk R, DevState S) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= S.nq * S.nr) return;
+    const int qi = pair_q(S, p), ri = pair_r(S, p);
+    S.flags[p] = (Q.na[qi] == R.na[ri]) ? 0 : CONV_LOWTMSC;
+    S.grand[p] = 0.f; S.best0[p] = 0.f; S.prevbest0[p] = 0.f;
+    S.nalnposs[p] = 0; S.qrypos[p] = 0; S.rfnpos[p] = 0; S.sfragndx[p] = 0; S.sfragpos[p] = 0;
+    for (int i = 0; i < 12; i++) {
+        float v = (i == 0 || i == 4 || i == 8) ? 1.f : 0.f;
+        S.tfm_ibest[p * 12 + i] = v;
+        S.tfm_grand[p * 12 + i] = v;
+    }
+    for (int i = 0; i < 15; i++) S.out[p * 15 + i] = 0.f;
+    S.out[p * 15 + 8] = 9999.9f;
+}
+
+// ============================================================================
+// gapless passes over structure coordinates (stage 1)
+struct Coords {
+    const float* x;
+    const float* y;
+    const float* z;
+};
+
+template <int P>
+__device__ void cc_gapless(const Coords& q, const Coords& r, int qpos, int rpos, int n,
+                           typename Prec<P>::T* cc) {
+    typename Prec<P>::Acc acc;
+    acc.clear();
+    for (int k = lane_id(); k < n; k += GTA_WAVE) {
+        int a = qpos + k, b = rpos + k;
+        acc.add(q.x[a], q.y[a], q.z[a], r.x[b], r.y[b], r.z[b]);
+    }
+    acc.reduce(cc);
+}
+// pairs with d2 (under tprev) <= thr; returns the count
+template <int P>
+__device__ int cc_gapless_ext(const Coords& q, const Coords& r, int qpos, int rpos, int n,
+                              const float* tprev, float thr, typename Prec<P>::T* cc) {
+    typename Prec<P>::Acc acc;
+    acc.clear();
+    int cnt = 0;
+    for (int k = lane_id(); k < n; k += GTA_WAVE) {
+        int a = qpos + k, b = rpos + k;
+        float qx = q.x[a], qy = q.y[a], qz = q.z[a], rx = r.x[b], ry = r.y[b], rz = r.z[b];
+        float d = tfm_dist2(tprev, qx, qy, qz, rx, ry, rz);
+        if (thr < d) continue;
+        acc.add(qx, qy, qz, rx, ry, rz);
+        cnt++;
+    }
+    acc.reduce(cc);
+    return wave_sum_i(cnt);
+}
+
+// seed-diagonal scores (CalcScoresUnrl_Complete, MpStage1.h:700-790):
+// all pairs scored; next threshold from the 3rd smallest distinct distance
+template <int P>
+__device__ float score_gapless_seed(const Coords& q, const Coords& r, int qpos, int rpos, int n,
+                                    const float* tfm, float d0, float d02, bool calc2, bool want_thr,
+                                    float* thr) {
+    ScoreAcc<P> sc;
+    sc.clear();
+    Min3d m3;
+    m3.init();
+    for (int k = lane_id(); k < n; k += GTA_WAVE) {
+        int a = qpos + k, b = rpos + k;
+        float d = tfm_dist2(tfm, q.x[a], q.y[a], q.z[a], r.x[b], r.y[b], r.z[b]);
+        sc.add(pair_score(d02, d));
+        if (want_thr) m3.put(d);
+    }
+    float s = sc.reduce();
+    if (want_thr) {
+        float min3 = m3.reduce_third();
+        float d0s = GetD0s(d0);
+        float d02s = d0s * d0s;
+        if (calc2) d02s += 1.0f;
+        if (CP_LARGEDST_cmp < min3 || min3 < d02s || n <= 3) min3 = d02s;
+        else min3 = d02s + ceilf((min3 - d02s) * 2.0f) * 0.5f;
+        *thr = min3;
+    }
+    return s;
+}
+
+// refinement scores (CalcScoresUnrlRefined_Complete / _DPRefined_Complete):
+// pairs within d8 (when chk) scored; threshold from the multiset 3rd smallest
+// distance, rounded up to a 0.5 A step above d0s-1
+__device__ __forceinline__ float refined_threshold(float min3, float d0, int n) {
+    float d0s = GetD0s(d0) - 1.0f;
+    float d02s = d0s * d0s;
+    if (CP_LARGEDST_cmp < min3 || min3 < d02s || n <= 3) return d02s;
+    float t = d0s + ceilf((sqrtf(min3) - d0s) * 2.0f) * 0.5f;
+    return t * t;
+}
+
+template <int P>
+__device__ float score_gapless_refined(const Coords& q, const Coords& r, int qpos, int rpos, int n,
+                                       const float* tfm, float d0, float d02, float d82, bool want_thr,
+                                       float* thr) {
+    ScoreAcc<P> sc;
+    sc.clear();
+    Min3 m3;
+    m3.init();
+    for (int k = lane_id(); k < n; k += GTA_WAVE) {
+        int a = qpos + k, b = rpos + k;
+        float d = tfm_dist2(tfm, q.x[a], q.y[a], q.z[a], r.x[b], r.y[b], r.z[b]);
+        if (d <= d82) sc.add(pair_score(d02, d));
+        if (want_thr) m3.put(d);
+    }
+    float s = sc.reduce();
+    if (want_thr) *thr = refined_threshold(m3.reduce_third(), d0, n);
+    return s;
+}
+
+__device__ __forceinline__ void copy12(float* d, const float* s) {
+#pragma unroll
+    for (int i = 0; i < 12; i++) d[i] = s[i];
+}
+
+// cross-wave argmax of the slot bests: highest score, lowest slot on ties
+struct SlotBest {
+    float best;
+    int slot;
+    float tfm[12];
+    float gbest;
+    int ndx, pos;
+};
+
+__device__ void block_best(SlotBest& mine, SlotBest* lds, SlotBest& out) {
+    const int w = threadIdx.x / GTA_WAVE;
+    if (lane_id() == 0) lds[w] = mine;
+    __syncthreads();
+    out = lds[0];
+    for (int i = 1; i < RWAVES; i++) {
+        const SlotBest& c = lds[i];
+        if (c.best > out.best || (c.best == out.best && c.slot < out.slot)) out = c;
+    }
+}
+
+// ============================================================================
+// stage 1: best gapless diagonal (seed) per pair
+template <int P>
+__global__ __launch_bounds__(RTHREADS) void k_findfrag(DevChunk Q, DevChunk R, DevState S, Geom G,
+                                                        int dinv_on) {
+    __shared__ SlotBest lds[RWAVES];
+    const int p = blockIdx.x;
+    const int qi = pair_q(S, p), ri = pair_r(S, p);
+    const int qlen = Q.len[qi], rlen = R.len[ri];
+    const int qd = Q.dst[qi], rd = R.dst[ri];
+    if (S.flags[p] & CONV_LOWTMSC) {
+        if (threadIdx.x == 0) S.best0[p] = 0.f;
+        return;
+    }
+    typedef typename Prec<P>::T T;
+    const Coords q{Q.x + qd, Q.y + qd, Q.z + qd}, r{R.x + rd, R.y + rd, R.z + rd};
+    const bool dinv = dinv_on && (qlen < rlen);
+    const float d0 = GetD0(qlen, rlen), d02 = d0 * d0;
+    const int prv = max(min(qlen, rlen) >> 1, 5);
+    SlotBest mine;
+    mine.best = 0.f; mine.slot = 0x7fffffff; mine.ndx = 0; mine.pos = 0;
+    for (int si = threadIdx.x / GTA_WAVE; si < G.nseeds; si += RWAVES) {
+        const int abspos = G.n1 + si * G.stepinit;
+        const int qpos = max(0, abspos), rpos = max(-abspos, 0);
+        if (qlen <= qpos || rlen <= rpos) continue;
+        if (qlen - qpos < prv || rlen - rpos < prv) continue;
+        const int n = min(qlen - qpos, rlen - rpos);
+        T cc[15];
+        float tfm[12], thr = CP_LARGEDST, best = 0.f;
+        for (int round = 0; round < 3; round++) {
+            int cnt;
+            if (round == 0) {
+                cc_gapless<P>(q, r, qpos, rpos, n, cc);
+                cnt = n;
+            } else {
+                float tprev[12];
+                copy12(tprev, tfm);
+                cnt = cc_gapless_ext<P>(q, r, qpos, rpos, n, tprev, thr, cc);
+                if (cnt < 1 || cnt == n) break;
+            }
+            kabsch<T>(cc, (T)cnt, dinv, tfm);
+            float sc = score_gapless_seed<P>(q, r, qpos, rpos, n, tfm, d0, d02, round > 0, round < 2, &thr);
+            if (best < sc) best = sc;
+        }
+        if (best > mine.best) { mine.best = best; mine.slot = si; mine.ndx = qpos; mine.pos = rpos; }
+    }
+    SlotBest b;
+    block_best(mine, lds, b);
+    if (threadIdx.x == 0) {
+        S.best0[p] = b.best;
+        if (b.best > 0.f) { S.qrypos[p] = b.ndx; S.rfnpos[p] = b.pos; }
+    }
+}
+
+// ============================================================================
+// stage 1: refinement of fragments of the best gapless alignment
+template <int P>
+__global__ __launch_bounds__(RTHREADS) void k_refine_gapless(DevChunk Q, DevChunk R, DevState S, Geom G,
+                                                              int dinv_on) {
+    __shared__ SlotBest lds[RWAVES];
+    const int p = blockIdx.x;
+    const int qi = pair_q(S, p), ri = pair_r(S, p);
+    const int qlen = Q.len[qi], rlen = R.len[ri];
+    const int qd = Q.dst[qi], rd = R.dst[ri];
+    if (S.flags[p] & CONV_LOWTMSC) {
+        if (threadIdx.x == 0) S.best0[p] = 0.f;
+        return;
+    }
+    typedef typename Prec<P>::T T;
+    const Coords q{Q.x + qd, Q.y + qd, Q.z + qd}, r{R.x + rd, R.y + rd, R.z + rd};
+    const bool dinv = dinv_on && (qlen < rlen);
+    const int qpos = S.qrypos[p], rpos = S.rfnpos[p];
+    const float d0 = GetD0(qlen, rlen), d02 = d0 * d0, d82 = GetD82(qlen, rlen);
+    const float d0s1 = GetD0s(d0) + 1.0f, d02s1 = d0s1 * d0s1;
+    SlotBest mine;
+    mine.best = 0.f; mine.slot = 0x7fffffff;
+    if (!(qlen <= qpos || rlen <= rpos)) {
+        const int nall = min(qlen - qpos, rlen - rpos);
+        for (int si = threadIdx.x / GTA_WAVE; si < G.nlocsteps; si += RWAVES) {
+            const int sfragfct = si / NMAXSUBFRAGS, sfragndx = si - sfragfct * NMAXSUBFRAGS;
+            const int sfragpos = sfragfct * SFRAGSTEP;
+            const int fl = GetFragLength(qlen, rlen, qpos, rpos, sfragndx);
+            if (fl < 1) continue;
+            if (qlen + SFRAGSTEP <= qpos + sfragpos + fl || rlen + SFRAGSTEP <= rpos + sfragpos + fl) continue;
+            T cc[15], last[15], nlast = 0;
+            T ncc;
+            {
+                int ql = qlen, rl = rlen, qp = qpos + sfragpos, rp = rpos + sfragpos;
+                UpdateLengths(ql, rl, qp, rp, fl);
+                cc_gapless<P>(q, r, qp, rp, fl, cc);
+                ncc = (T)fl;
+            }
+            float tfm[12], tfmBest[12], thr = CP_LARGEDST, best = 0.f;
+            for (int cit = 0; cit < NMAXCONVIT + 2; cit++) {
+                if (0 < cit) {
+                    float tprev[12];
+                    copy12(tprev, tfm);
+                    int cnt = cc_gapless_ext<P>(q, r, qpos, rpos, nall, tprev, (cit < 2) ? thr : d02s1, cc);
+                    ncc = (T)cnt;
+                    if (cc_converged<T>(cc, ncc, last, nlast)) break;
+                }
+                if (ncc < (T)1) break;
+#pragma unroll
+                for (int f = 0; f < 15; f++) last[f] = cc[f];
+                nlast = ncc;
+                kabsch<T>(cc, ncc, dinv, tfm);
+                float sc = score_gapless_refined<P>(q, r, qpos, rpos, nall, tfm, d0, d02, d82, cit < 1, &thr);
+                if (best < sc) { best = sc; copy12(tfmBest, tfm); }
+            }
+            if (best > mine.best) { mine.best = best; mine.slot = si; copy12(mine.tfm, tfmBest); }
+        }
+    }
+    SlotBest b;
+    block_best(mine, lds, b);
+    if (threadIdx.x == 0) {
+        const bool have = b.best > 0.f;
+        S.best0[p] = have ? b.best : 0.f;
+        if (have) {
+            copy12(S.tfm_ibest + p * 12, b.tfm);
+            if (S.grand[p] < b.best) { S.grand[p] = b.best; copy12(S.tfm_grand + p * 12, b.tfm); }
+        }
+    }
+}
+
+// ============================================================================
+// DP fill with backtrace and extraction of matched pairs, one wave per pair.
+//
+// The Lq x Lr matrix is swept in strips of 64 query rows; lane l owns row
+// 64*s + l and at step t computes column t - l, so every step is one
+// anti-diagonal of the strip.  The up and diagonal predecessors come from lane
+// l-1 through a cross-lane shift; lane 0 reads the previous strip's last row
+// from a row buffer that lane 63 fills.  Backtrace codes (2 bits) are packed
+// with two ballots per step: 16 steps of a strip make 64 words, one per lane,
+// stored as one coalesced 256-byte write.  The walk from the bottom-right
+// cell reads runs of 64 cells along the current direction in parallel.
+enum { DPM_IBEST = 0, DPM_GRAND = 1, DPM_CONSTRAINED = 2 };
+
+struct DpParams {
+    int mode;
+    int gap0;
+    float gapopen;
+    int usess;
+    float ssw;
+    int dpscan;
+};
+
+__device__ __forceinline__ unsigned read_code(const unsigned* bt, int nblk, int i, int j) {
+    const int s = i >> 6, l = i & 63, t = j + l;
+    const size_t base = ((size_t)(s * nblk + (t >> 4)) << 6) + ((t & 15) << 2);
+    const unsigned w0 = bt[base + (l >> 5)], w1 = bt[base + 2 + (l >> 5)];
+    const int b = l & 31;
+    return ((w0 >> b) & 1u) | (((w1 >> b) & 1u) << 1);
+}
+
+template <bool LDSROW>
+__global__ __launch_bounds__(GTA_WAVE) void k_dp(DevChunk Q, DevChunk R, DevState S, DpParams prm) {
+    extern __shared__ float4 lds_dp[];
+    const int p = blockIdx.x;
+    const int qi = pair_q(S, p), ri = pair_r(S, p);
+    const int Lq = Q.len[qi], Lr = R.len[ri];
+    const int qd = Q.dst[qi], rd = R.dst[ri];
+    const int lane = lane_id();
+    const int flags = S.flags[p];
+    if (prm.mode == DPM_IBEST) {
+        if (flags & (CONV_LOWTMSC | CONV_SCOREDP)) { if (lane == 0) S.nalnposs[p] = 0; return; }
+    } else if (flags & CONV_LOWTMSC) {
+        if (prm.mode == DPM_GRAND && lane == 0) S.nalnposs[p] = 0;
+        return;
+    }
+    float tfm[12];
+    const float* tsrc = (prm.mode == DPM_IBEST ? S.tfm_ibest : S.tfm_grand) + (size_t)p * 12;
+#pragma unroll
+    for (int i = 0; i < 12; i++) tfm[i] = tsrc[i];
+    float d0 = GetD0(Lq, Lr);
+    if (prm.dpscan) d0 += 1.5f;
+    const float d02 = d0 * d0;
+    const bool gap0 = prm.gap0 != 0, usess = prm.usess != 0;
+    const float gapopen = prm.gapopen, ssw = prm.ssw;
+
+    // reference coordinates (+ss) and the strip-boundary row
+    float4* rc;
+    float* row;
+    float4* ldsrc = lds_dp;
+    float* ldsrow = (float*)(lds_dp + Lr);
+    if (LDSROW) {
+        rc = ldsrc;
+        row = ldsrow;
+        for (int j = lane; j < Lr; j += GTA_WAVE)
+            rc[j] = make_float4(R.x[rd + j], R.y[rd + j], R.z[rd + j], (float)R.ss[rd + j]);
+        __syncthreads();
+    } else {
+        rc = nullptr;
+        row = S.rowbuf + (size_t)qi * (size_t)R.nposs + (size_t)rd;
+    }
+    unsigned* bt = S.btck + S.btckoff[p];
+    const int nstrips = (Lq + 63) >> 6;
+    const int nsteps = Lr + 63;
+    const int nblk = (nsteps + 15) >> 4;
+
+    for (int s = 0; s < nstrips; s++) {
+        const int i = (s << 6) + lane;
+        const bool rowok = i < Lq;
+        float qx = 0.f, qy = 0.f, qz = 0.f;
+        int qss = 0;
+        if (rowok) {
+            qx = Q.x[qd + i]; qy = Q.y[qd + i]; qz = Q.z[qd + i];
+            tfm_point(tfm, qx, qy, qz);
+            qss = Q.ss[qd + i];
+        }
+        float val = 0.f, upprev = 0.f;
+        unsigned word = 0;
+        for (int t = 0; t < nsteps; t++) {
+            const int j = t - lane;
+            float recv = __shfl_up(val, 1, GTA_WAVE);
+            if (lane == 0) recv = (s > 0 && t < Lr) ? row[t] : 0.f;
+            const float up = recv, diag = upprev;
+            upprev = up;
+            unsigned code = 0;
+            if (j >= 0 && j < Lr && rowok) {
+                float rx, ry, rz;
+                int rss;
+                if (LDSROW) {
+                    const float4 r4 = rc[j];
+                    rx = r4.x; ry = r4.y; rz = r4.z; rss = (int)r4.w;
+                } else {
+                    rx = R.x[rd + j]; ry = R.y[rd + j]; rz = R.z[rd + j]; rss = R.ss[rd + j];
+                }
+                float v = 0.0f;
+                if (qx < DEFCOORD_QRY_cmp && DEFCOORD_RFN_cmp < rx) v = pair_score(d02, dist2(qx, qy, qz, rx, ry, rz));
+                if (usess) v = __fadd_rn(v, (qss == rss) ? ssw : 0.0f);
+                v = __fadd_rn(v, gap0 ? diag : fabsf(diag));
+                unsigned bk = BT_DIAG;
+                float w = val;
+                if (!gap0 && w < 0.0f) w = __fsub_rn(gapopen, w);
+                if (v < w) { v = w; bk = BT_LEFT; }
+                w = up;
+                if (!gap0 && w < 0.0f) w = __fsub_rn(gapopen, w);
+                if (v < w) { v = w; bk = BT_UP; }
+                val = (gap0 || bk != BT_DIAG) ? v : -v;
+                code = bk;
+            } else if (j < 0) {
+                val = 0.f;
+            }
+            if (lane == 63 && j >= 0 && j < Lr) row[j] = val;
+            const unsigned long long b0 = __ballot(code & 1u);
+            const unsigned long long b1 = __ballot(code & 2u);
+            const int k = t & 15;
+            if ((lane >> 2) == k) {
+                const int sel = lane & 3;
+                const unsigned long long m = (sel < 2) ? b0 : b1;
+                word = (sel & 1) ? (unsigned)(m >> 32) : (unsigned)m;
+            }
+            if (k == 15 || t == nsteps - 1) {
+                bt[((size_t)(s * nblk + (t >> 4)) << 6) + lane] = word;
+                word = 0;
+            }
+        }
+        if (!LDSROW) __threadfence_block();
+    }
+    if (lane == 0) atomicAdd(S.cells, (unsigned long long)Lq * (unsigned long long)Lr);
+    __threadfence();
+
+    // ---- backtrace walk --------------------------------------------------------
+    if (prm.mode == DPM_GRAND && (flags & CONV_SCOREDP)) {
+        if (lane == 0) S.nalnposs[p] = 0;
+        return;
+    }
+    const size_t base = (size_t)qi * (size_t)R.nposs + (size_t)rd;
+    float* aqx = S.aln + base;
+    float* aqy = aqx + S.alnplane;
+    float* aqz = aqy + S.alnplane;
+    float* arx = aqz + S.alnplane;
+    float* ary = arx + S.alnplane;
+    float* arz = ary + S.alnplane;
+    float* ad2 = arz + S.alnplane;
+    float* aqp = ad2 + S.alnplane;
+    float* arp = aqp + S.alnplane;
+    const bool constrained = prm.mode == DPM_CONSTRAINED;
+    const float d82 = GetD82(Lq, Lr);
+    int y = Lq - 1, x = Lr - 1, k = 0;
+    unsigned dir = BT_DIAG;
+    while (y >= 0 && x >= 0) {
+        const int dy = (dir == BT_LEFT) ? 0 : 1, dx = (dir == BT_UP) ? 0 : 1;
+        const int yy = y - lane * dy, xx = x - lane * dx;
+        const bool inr = yy >= 0 && xx >= 0;
+        const unsigned code = inr ? read_code(bt, nblk, yy, xx) : (unsigned)BT_STOP;
+        const unsigned long long brk = __ballot(!(inr && code == dir));
+        const int run = brk ? (int)__builtin_ctzll(brk) : GTA_WAVE;
+        if (dir == BT_DIAG) {
+            const bool take = lane < run;
+            float qx = 0, qy = 0, qz = 0, rx = 0, ry = 0, rz = 0, d2 = 0;
+            bool keep = take;
+            if (take) {
+                qx = Q.x[qd + yy]; qy = Q.y[qd + yy]; qz = Q.z[qd + yy];
+                rx = R.x[rd + xx]; ry = R.y[rd + xx]; rz = R.z[rd + xx];
+                if (constrained) {
+                    d2 = tfm_dist2(tfm, qx, qy, qz, rx, ry, rz);
+                    keep = d2 <= d82;
+                }
+            }
+            const unsigned long long km = __ballot(keep);
+            const int before = __popcll(km & ((1ull << lane) - 1ull));
+            if (keep) {
+                const int o = k + before;
+                aqx[o] = qx; aqy[o] = qy; aqz[o] = qz;
+                arx[o] = rx; ary[o] = ry; arz[o] = rz;
+                if (constrained) { ad2[o] = d2; aqp[o] = (float)yy; arp[o] = (float)xx; }
+            }
+            k += __popcll(km);
+        }
+        y -= run * dy;
+        x -= run * dx;
+        if (run < GTA_WAVE) {
+            const unsigned c = __shfl(code, run, GTA_WAVE);
+            const int in = __shfl((int)inr, run, GTA_WAVE);
+            if (!in || c == BT_STOP) break;
+            dir = c;
+        }
+    }
+    if (lane == 0) S.nalnposs[p] = k;
+}
+
+// ============================================================================
+// refinement over the DP alignment (reverse-ordered matched pairs)
+enum { RM_DPREFINE = 0, RM_FIN1 = 1, RM_FIN2 = 2, RM_PROD1 = 3, RM_PROD2 = 4 };
+
+struct RefineParams {
+    int dinv_on;
+    // DPRefine bookkeeping (MpStage1.cpp:455-510); epi=0 disables it
+    int epi;
+    int dpi;
+    int maxndpiters;
+    float prescore;        // > 0 with prescreen at the last iteration
+};
+
+struct Aln {
+    const float* qx; const float* qy; const float* qz;
+    const float* rx; const float* ry; const float* rz;
+};
+
+// forward alignment position f is stored at L-1-f
+template <int P>
+__device__ void cc_dp_frag(const Aln& a, int L, int fragpos, int fl, typename Prec<P>::T* cc,
+                           typename Prec<P>::T* sq) {
+    int ql = L, rl = L, qp = fragpos, rp = fragpos;
+    UpdateLengths(ql, rl, qp, rp, fl);
+    typename Prec<P>::Acc acc;
+    acc.clear();
+    typedef typename Prec<P>::T T;
+    T s2[6] = {0, 0, 0, 0, 0, 0};
+    for (int k = lane_id(); k < fl; k += GTA_WAVE) {
+        const int idx = L - 1 - (rp + k);
+        const float qx = a.qx[idx], qy = a.qy[idx], qz = a.qz[idx], rx = a.rx[idx], ry = a.ry[idx], rz = a.rz[idx];
+        acc.add(qx, qy, qz, rx, ry, rz);
+        if (sq) {
+            s2[0] += (T)(qx * qx); s2[1] += (T)(qy * qy); s2[2] += (T)(qz * qz);
+            s2[3] += (T)(rx * rx); s2[4] += (T)(ry * ry); s2[5] += (T)(rz * rz);
+        }
+    }
+    acc.reduce(cc);
+    if (sq) {
+        for (int i = 0; i < 6; i++) sq[i] = wave_sum(s2[i]);
+    }
+}
+template <int P>
+__device__ int cc_dp_ext(const Aln& a, int L, const float* tprev, float thr, typename Prec<P>::T* cc) {
+    typename Prec<P>::Acc acc;
+    acc.clear();
+    int cnt = 0;
+    for (int f = lane_id(); f < L; f += GTA_WAVE) {
+        const int idx = L - 1 - f;
+        const float qx = a.qx[idx], qy = a.qy[idx], qz = a.qz[idx], rx = a.rx[idx], ry = a.ry[idx], rz = a.rz[idx];
+        const float d = tfm_dist2(tprev, qx, qy, qz, rx, ry, rz);
+        if (thr < d) continue;
+        acc.add(qx, qy, qz, rx, ry, rz);
+        cnt++;
+    }
+    acc.reduce(cc);
+    return wave_sum_i(cnt);
+}
+template <int P>
+__device__ float score_dp(const Aln& a, int L, const float* tfm, float d0, float d02, float d82, bool chk,
+                          bool want_thr, float* thr) {
+    ScoreAcc<P> sc;
+    sc.clear();
+    Min3 m3;
+    m3.init();
+    for (int f = lane_id(); f < L; f += GTA_WAVE) {
+        const int idx = L - 1 - f;
+        const float d = tfm_dist2(tfm, a.qx[idx], a.qy[idx], a.qz[idx], a.rx[idx], a.ry[idx], a.rz[idx]);
+        if (!chk || d <= d82) sc.add(pair_score(d02, d));
+        if (want_thr) m3.put(d);
+    }
+    float s = sc.reduce();
+    if (want_thr) *thr = refined_threshold(m3.reduce_third(), d0, L);
+    return s;
+}
+
+// one slot's superposition iterations over the DP alignment
+template <int P>
+__device__ float refine_slot_dp(const Aln& a, int L, int fl, int fragpos, float d0, float d02, float d82,
+                                bool chk, float bestinit, bool dinv, float* tfmBest,
+                                const typename Prec<P>::T* ccinit) {
+    typedef typename Prec<P>::T T;
+    T cc[15], last[15], ncc, nlast = 0;
+    if (ccinit) {
+        for (int f = 0; f < 15; f++) cc[f] = ccinit[f];
+    } else {
+        cc_dp_frag<P>(a, L, fragpos, fl, cc, nullptr);
+    }
+    ncc = (T)fl;
+    const float d0s1 = GetD0s(d0) + 1.0f, d02s1 = d0s1 * d0s1;
+    float tfm[12], thr = CP_LARGEDST, best = bestinit;
+    for (int i = 0; i < 12; i++) { tfm[i] = 0.f; tfmBest[i] = (i == 0 || i == 4 || i == 8) ? 1.f : 0.f; }
+    for (int cit = 0; cit < NMAXCONVIT + 2; cit++) {
+        if (0 < cit) {
+            float tprev[12];
+            copy12(tprev, tfm);
+            int cnt = cc_dp_ext<P>(a, L, tprev, (cit < 2) ? thr : d02s1, cc);
+            ncc = (T)cnt;
+            if (cc_converged<T>(cc, ncc, last, nlast)) break;
+        }
+        if (ncc < (T)1) break;
+        for (int f = 0; f < 15; f++) last[f] = cc[f];
+        nlast = ncc;
+        kabsch<T>(cc, ncc, dinv, tfm);
+        float sc = score_dp<P>(a, L, tfm, d0, d02, d82, chk, cit < 1, &thr);
+        if (best < sc) { best = sc; copy12(tfmBest, tfm); }
+    }
+    return best;
+}
+
+template <int P, int MODE>
+__global__ __launch_bounds__(RTHREADS) void k_refine_dp(DevChunk Q, DevChunk R, DevState S, Geom G,
+                                                         RefineParams prm, int ldscap) {
+    extern __shared__ float lds_rf[];
+    __shared__ SlotBest slds[RWAVES];
+    typedef typename Prec<P>::T T;
+    const int p = blockIdx.x;
+    const int qi = pair_q(S, p), ri = pair_r(S, p);
+    const int qlen = Q.len[qi], rlen = R.len[ri];
+    const int flags = S.flags[p];
+    const bool dinv = prm.dinv_on && (qlen < rlen);
+
+    bool skip;
+    if (MODE == RM_DPREFINE) skip = flags != 0;
+    else skip = (flags & CONV_LOWTMSC) != 0;
+    if (skip && MODE != RM_DPREFINE) return;
+
+    const int L = skip ? 0 : S.nalnposs[p];
+    const size_t base = (size_t)qi * (size_t)R.nposs + (size_t)R.dst[ri];
+    Aln a;
+    const float* g = S.aln + base;
+    const size_t pl = S.alnplane;
+    if (!skip && L <= ldscap) {
+        for (int k = threadIdx.x; k < L; k += RTHREADS) {
+            lds_rf[k] = g[k]; lds_rf[ldscap + k] = g[pl + k]; lds_rf[2 * ldscap + k] = g[2 * pl + k];
+            lds_rf[3 * ldscap + k] = g[3 * pl + k]; lds_rf[4 * ldscap + k] = g[4 * pl + k];
+            lds_rf[5 * ldscap + k] = g[5 * pl + k];
+        }
+        __syncthreads();
+        a = Aln{lds_rf, lds_rf + ldscap, lds_rf + 2 * ldscap, lds_rf + 3 * ldscap, lds_rf + 4 * ldscap,
+                lds_rf + 5 * ldscap};
+    } else {
+        a = Aln{g, g + pl, g + 2 * pl, g + 3 * pl, g + 4 * pl, g + 5 * pl};
+    }
+
+    const bool prod = (MODE == RM_PROD1 || MODE == RM_PROD2);
+    const int rna = R.na[ri];
+    const float d0 = prod ? GetD0fin(qlen, rlen, rna) : GetD0(qlen, rlen);
+    const float d02 = d0 * d0, d82 = GetD82(qlen, rlen);
+    const int glen = max(qlen, rlen);
+    const float g0 = GetD0fin(glen, glen, rna), g02 = g0 * g0;
+    const bool chk = !prod;
+    const float bestinit = (MODE == RM_DPREFINE) ? 0.f : -1.f;
+    const bool phase2 = (MODE == RM_FIN2 || MODE == RM_PROD2);
+    const int nslots = skip ? 0 : (phase2 ? G.maxnfragfcts : G.nlocsteps);
+    const int base_ndx = phase2 ? S.sfragndx[p] : 0;
+    const int base_pos = phase2 ? S.sfragpos[p] : 0;
+
+    SlotBest mine;
+    mine.best = 0.f; mine.slot = 0x7fffffff; mine.gbest = 0.f; mine.ndx = 0; mine.pos = 0;
+    for (int si = threadIdx.x / GTA_WAVE; si < nslots; si += RWAVES) {
+        int sfragndx, sfragpos, step;
+        if (phase2) {
+            sfragndx = base_ndx;
+            if (sfragndx == 0) sfragndx++;
+            sfragpos = base_pos + (si - (G.maxnfragfcts >> 1));
+            step = 1;
+        } else {
+            const int sfragfct = si / NMAXSUBFRAGS;
+            sfragndx = si - sfragfct * NMAXSUBFRAGS;
+            sfragpos = sfragfct * SFRAGSTEP;
+            step = SFRAGSTEP;
+        }
+        const int fl = GetFragLength(L, L, 0, 0, sfragndx);
+        if (fl < 1) continue;
+        if (sfragpos < 0 || L + step <= sfragpos + fl) continue;
+        T ccinit[15];
+        const T* ccp = nullptr;
+        if (MODE == RM_PROD1 && si == 0) {
+            T sq[6];
+            cc_dp_frag<P>(a, L, sfragpos, fl, ccinit, sq);
+            const float rmsd = rmsd_e0<T>(ccinit, sq, (T)fl);
+            if (lane_id() == 0) S.out[(size_t)p * 15 + 8] = rmsd;
+            ccp = ccinit;
+        }
+        float tfmBest[12];
+        float best = refine_slot_dp<P>(a, L, fl, sfragpos, d0, d02, d82, chk, bestinit, dinv, tfmBest, ccp);
+        float gbest = best;
+        if (prod && qlen != rlen) gbest = score_dp<P>(a, L, tfmBest, g0, g02, d82, false, false, nullptr);
+        if (best > 0.f && best > mine.best) {
+            mine.best = best; mine.slot = si; copy12(mine.tfm, tfmBest);
+            mine.gbest = gbest; mine.ndx = sfragndx; mine.pos = sfragpos;
+        }
+    }
+    SlotBest b;
+    block_best(mine, slds, b);
+    if (threadIdx.x != 0) return;
+    const bool have = b.best > 0.f;
+    float* tg = S.tfm_grand + (size_t)p * 12;
+    if (!prod) {
+        S.best0[p] = have ? b.best : 0.f;
+        if (have) copy12(S.tfm_ibest + (size_t)p * 12, b.tfm);
+        if (MODE == RM_FIN1) { S.sfragndx[p] = have ? b.ndx : 0; S.sfragpos[p] = have ? b.pos : 0; }
+        if (have && S.grand[p] < b.best) { S.grand[p] = b.best; copy12(tg, b.tfm); }
+    } else {
+        const float grand = (MODE == RM_PROD2) ? S.grand[p] : 0.f;
+        if (have && grand < b.best) {
+            S.grand[p] = b.best;
+            if (MODE == RM_PROD1) { S.sfragndx[p] = b.ndx; S.sfragpos[p] = b.pos; }
+            float sq = b.best, sr = b.gbest;
+            if (rlen < qlen) { float t = sq; sq = sr; sr = t; }
+            float* o = S.out + (size_t)p * 15;
+            o[9] = sq / (float)qlen;
+            o[10] = sr / (float)rlen;
+            o[11] = GetD0fin(qlen, qlen, Q.na[qi]);
+            o[12] = GetD0fin(rlen, rlen, rna);
+            copy12(tg, b.tfm);
+        }
+    }
+    if (MODE == RM_DPREFINE && prm.epi) {
+        int fl = S.flags[p];
+        if (0 < prm.dpi && !(fl & (CONV_SCOREDP | CONV_LOWTMSC)) &&
+            fabsf(S.best0[p] - S.prevbest0[p]) < SCORE_CONVEPSILON)
+            fl |= CONV_SCOREDP;
+        if (prm.dpi + 1 < prm.maxndpiters && !(fl & (CONV_SCOREDP | CONV_LOWTMSC))) S.prevbest0[p] = S.best0[p];
+        if (prm.maxndpiters <= prm.dpi + 1) {
+            if (0.0f < prm.prescore && S.grand[p] < prm.prescore * (float)min(qlen, rlen)) fl |= CONV_LOWTMSC;
+            fl &= ~CONV_SCOREDP;
+        }
+        S.flags[p] = fl;
+    }
+}
+
+// ============================================================================
+// production alignment rows and statistics, one wave per pair
+__global__ __launch_bounds__(GTA_WAVE) void k_prod_aln(DevChunk Q, DevChunk R, DevState S, char* alns,
+                                                       int nodel) {
+    const int p = blockIdx.x;
+    const int qi = pair_q(S, p), ri = pair_r(S, p);
+    if (S.flags[p] & CONV_LOWTMSC) return;
+    const int lane = lane_id();
+    const int qlen = Q.len[qi];
+    const int qd = Q.dst[qi], rd = R.dst[ri];
+    const int m = S.nalnposs[p];
+    const size_t base = (size_t)qi * (size_t)R.nposs + (size_t)rd;
+    const float* ad2 = S.aln + base + 6 * S.alnplane;
+    const float* aqp = ad2 + S.alnplane;
+    const float* arp = aqp + S.alnplane;
+    const size_t alnofff = ((size_t)qi * R.nposs + (size_t)S.nr * (size_t)(qd + qi)) * 5;
+    const size_t dbalnlen = (size_t)R.nposs + (size_t)S.nr * (size_t)(qlen + 1);
+    const size_t dbalnbeg = (size_t)rd + (size_t)ri * (size_t)(qlen + 1);
+    char* row0 = alns + alnofff + dbalnbeg;
+    auto put = [&](int col, char a, char mid, char b, char sa, char sb) {
+        row0[col] = a; row0[dbalnlen + col] = mid; row0[2 * dbalnlen + col] = b;
+        row0[3 * dbalnlen + col] = sa; row0[4 * dbalnlen + col] = sb;
+    };
+    int qp0 = 0, rp0 = 0, qpl = 0, rpl = 0;
+    if (m > 0) {
+        qp0 = (int)aqp[m - 1]; rp0 = (int)arp[m - 1];
+        qpl = (int)aqp[0]; rpl = (int)arp[0];
+    }
+    int psts = 0, idts = 0;
+    for (int k = lane; k < m; k += GTA_WAVE) {
+        const int idx = m - 1 - k;
+        const int qp = (int)aqp[idx], rp = (int)arp[idx];
+        const float d2 = ad2[idx];
+        const int col = nodel ? (qp - qp0) : (qp - qp0) + (rp - rp0) - k;
+        const char qc = Q.rsd[qd + qp], tc = R.rsd[rd + rp];
+        char mid = (d2 < EQUIV_D2) ? '+' : ' ';
+        if (d2 < EQUIV_D2) psts++;
+        if (qc != '-' && qc == tc) { idts++; mid = (mid == '+') ? qc : (char)(qc | 32); }
+        put(col, qc, mid, tc, Q.ss[qd + qp], R.ss[rd + rp]);
+        if (k + 1 < m) {
+            const int qn = (int)aqp[idx - 1], rn = (int)arp[idx - 1];
+            int c = col + 1;
+            for (int q = qp + 1; q < qn; q++, c++) put(c, Q.rsd[qd + q], ' ', '-', Q.ss[qd + q], ' ');
+            if (!nodel)
+                for (int r = rp + 1; r < rn; r++, c++) put(c, '-', ' ', R.rsd[rd + r], ' ', R.ss[rd + r]);
+        }
+    }
+    psts = wave_sum_i(psts);
+    idts = wave_sum_i(idts);
+    const int total = (m > 0) ? ((nodel ? (qpl - qp0) : (qpl - qp0) + (rpl - rp0) - (m - 1)) + 1) : 0;
+    if (lane < 5) row0[(size_t)lane * dbalnlen + total] = 0;
+    if (lane == 0) {
+        float* o = S.out + (size_t)p * 15;
+        unsigned bc = (m > 0) ? (((unsigned)(rp0 + 1) << 16) | (unsigned)(qp0 + 1)) : 0u;
+        unsigned ec = (m > 0) ? (((unsigned)(rpl + 1) << 16) | (unsigned)(qpl + 1)) : 0u;
+        o[2] = __uint_as_float(bc);
+        o[3] = __uint_as_float(ec);
+        o[4] = (float)total;
+        o[5] = (float)psts;
+        o[6] = (float)idts;
+        o[7] = (float)(total - m);
+    }
+}
+
+// ============================================================================
+__global__ void k_setlow(DevChunk Q, DevChunk R, DevState S, float thr) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= S.nq * S.nr) return;
+    const int qi = pair_q(S, p), ri = pair_r(S, p);
+    if (S.grand[p] < thr * (float)min(Q.len[qi], R.len[ri])) S.flags[p] |= CONV_LOWTMSC;
+}
+
+__global__ void k_revert(DevState S) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= S.nq * S.nr) return;
+    float* t = S.tfm_grand + (size_t)p * 12;
+    float u[12];
+    copy12(u, t);
+    float r[9] = {u[0], u[3], u[6], u[1], u[4], u[7], u[2], u[5], u[8]};
+    for (int i = 0; i < 9; i++) t[i] = r[i];
+    t[9] = -(r[0] * u[9] + r[1] * u[10] + r[2] * u[11]);
+    t[10] = -(r[3] * u[9] + r[4] * u[10] + r[5] * u[11]);
+    t[11] = -(r[6] * u[9] + r[7] * u[10] + r[8] * u[11]);
+}
+
+}  // namespace gta
+
+// ============================================================================
+// launchers (host side), called by the C ABI implementation
+#include "gta_launch.h"
+
+namespace gta {
+
+static inline int blocks_for(long long n, int t) { return (int)((n + t - 1) / t); }
+
+hipError_t launch_secstr(const DevChunk& C, hipStream_t st) {
+    hipLaunchKernelGGL(k_secstr, dim3(blocks_for(C.nposs, 256)), dim3(256), 0, st, C);
+    return hipGetLastError();
+}
+hipError_t launch_init(const DevChunk& Q, const DevChunk& R, const DevState& S, hipStream_t st) {
+    const long long np = (long long)S.nq * S.nr;
+    hipLaunchKernelGGL(k_init, dim3(blocks_for(np, 256)), dim3(256), 0, st, Q, R, S);
+    return hipGetLastError();
+}
+hipError_t launch_findfrag(int prec, const DevChunk& Q, const DevChunk& R, const DevState& S, const Geom& G,
+                           int dinv, hipStream_t st) {
+    const int np = S.nq * S.nr;
+    if (prec) hipLaunchKernelGGL(k_findfrag<1>, dim3(np), dim3(RTHREADS), 0, st, Q, R, S, G, dinv);
+    else hipLaunchKernelGGL(k_findfrag<0>, dim3(np), dim3(RTHREADS), 0, st, Q, R, S, G, dinv);
+    return hipGetLastError();
+}
+hipError_t launch_refine_gapless(int prec, const DevChunk& Q, const DevChunk& R, const DevState& S,
+                                 const Geom& G, int dinv, hipStream_t st) {
+    const int np = S.nq * S.nr;
+    if (prec) hipLaunchKernelGGL(k_refine_gapless<1>, dim3(np), dim3(RTHREADS), 0, st, Q, R, S, G, dinv);
+    else hipLaunchKernelGGL(k_refine_gapless<0>, dim3(np), dim3(RTHREADS), 0, st, Q, R, S, G, dinv);
+    return hipGetLastError();
+}
+hipError_t launch_dp(const DevChunk& Q, const DevChunk& R, const DevState& S, const DpLaunch& L,
+                     hipStream_t st) {
+    const int np = S.nq * S.nr;
+    DpParams prm{L.mode, L.gap0, L.gapopen, L.usess, L.ssw, L.dpscan};
+    if (L.lds_bytes > 0)
+        hipLaunchKernelGGL(k_dp<true>, dim3(np), dim3(GTA_WAVE), L.lds_bytes, st, Q, R, S, prm);
+    else
+        hipLaunchKernelGGL(k_dp<false>, dim3(np), dim3(GTA_WAVE), 0, st, Q, R, S, prm);
+    return hipGetLastError();
+}
+hipError_t launch_refine_dp(int prec, int mode, const DevChunk& Q, const DevChunk& R, const DevState& S,
+                            const Geom& G, const RefineLaunch& L, hipStream_t st) {
+    const int np = S.nq * S.nr;
+    RefineParams prm{L.dinv, L.epi, L.dpi, L.maxndpiters, L.prescore};
+    const size_t lds = (size_t)L.ldscap * 6 * sizeof(float);
+#define GTA_RL(PP, MM) hipLaunchKernelGGL((k_refine_dp<PP, MM>), dim3(np), dim3(RTHREADS), lds, st, Q, R, S, G, prm, L.ldscap)
+#define GTA_RLM(PP)                                      \
+    switch (mode) {                                      \
+        case RM_DPREFINE: GTA_RL(PP, RM_DPREFINE); break; \
+        case RM_FIN1: GTA_RL(PP, RM_FIN1); break;         \
+        case RM_FIN2: GTA_RL(PP, RM_FIN2); break;         \
+        case RM_PROD1: GTA_RL(PP, RM_PROD1); break;       \
+        default: GTA_RL(PP, RM_PROD2); break;             \
+    }
+    if (prec) { GTA_RLM(1) } else { GTA_RLM(0) }
+#undef GTA_RLM
+#undef GTA_RL
+    return hipGetLastError();
+}
+hipError_t launch_prod_aln(const DevChunk& Q, const DevChunk& R, const DevState& S, char* alns, int nodel,
+                           hipStream_t st) {
+    const int np = S.nq * S.nr;
+    hipLaunchKernelGGL(k_prod_aln, dim3(np), dim3(GTA_WAVE), 0, st, Q, R, S, alns, nodel);
+    return hipGetLastError();
+}
+hipError_t launch_setlow(const DevChunk& Q, const DevChunk& R, const DevState& S, float thr, hipStream_t st) {
+    const long long np = (long long)S.nq * S.nr;
+    hipLaunchKernelGGL(k_setlow, dim3(blocks_for(np, 256)), dim3(256), 0, st, Q, R, S, thr);
+    return hipGetLastError();
+}
+hipError_t launch_revert(const DevState& S, hipStream_t st) {
+    const long long np = (long long)S.nq * S.nr;
+    hipLaunchKernelGGL(k_revert, dim3(blocks_for(np, 256)), dim3(256), 0, st, S);
+    return hipGetLastError();
+}
+
+}  // namespace gta
