@@ -20,6 +20,11 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libgta_amd.
 _lib = None
 
 
+class _KStat(C.Structure):
+    _fields_ = [("name", C.c_char_p), ("launches", C.c_int), ("seconds", C.c_double),
+                ("alg_bytes", C.c_double)]
+
+
 class GtaError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__("%s: %s" % (STATUS.get(code, code), msg))
@@ -50,6 +55,10 @@ def lib():
         L.gta_last_error.restype = C.c_char_p
         L.gta_alignment_section_size.argtypes = [C.c_int] * 4
         L.gta_alignment_section_size.restype = C.c_size_t
+        L.gta_set_profiling.argtypes = [C.c_void_p, C.c_int]
+        L.gta_set_profiling.restype = C.c_int
+        L.gta_kernel_stats.argtypes = [C.c_void_p, C.POINTER(_KStat), C.c_int]
+        L.gta_kernel_stats.restype = C.c_int
         _lib = L
     return _lib
 
@@ -104,6 +113,17 @@ class GtaBatch:
         self._check(lib().gta_process_resident(self._h, C.byref(s), C.byref(cells)))
         return s.value, cells.value
 
+    def set_profiling(self, on):
+        """Bracket every kernel launch with HIP events (see kernel_stats)."""
+        self._check(lib().gta_set_profiling(self._h, 1 if on else 0))
+
+    def kernel_stats(self):
+        """Per kernel class of the last step: launches, device seconds, algorithmic bytes."""
+        arr = (_KStat * 16)()
+        n = lib().gta_kernel_stats(self._h, arr, 16)
+        return [{"name": arr[i].name.decode(), "launches": arr[i].launches,
+                 "seconds": arr[i].seconds, "alg_bytes": arr[i].alg_bytes} for i in range(n)]
+
     def close(self):
         if getattr(self, "_h", None):
             lib().gta_destroy(self._h)
@@ -120,3 +140,4 @@ class GtaBatch:
 
     def __exit__(self, *a):
         self.close()
+

@@ -77,7 +77,14 @@ struct gta_ctx {
     bool have_q = false, have_r = false;
     // pair state
     DevBuf s_flags, s_grand, s_best0, s_prev, s_naln, s_qpos, s_rpos, s_fndx, s_fpos;
-    DevBuf s_tibest, s_tgrand, s_out, s_aln, s_btck, s_btckoff, s_rowbuf, s_cells, s_alns;
+    DevBuf s_tibest, s_tgrand, s_out, s_aln, s_btck, s_btckoff, s_rowbuf, s_cells, s_alns, s_algb;
+    // measurement
+    int profiling = 0;
+    std::vector<hipEvent_t> evpool;
+    std::vector<std::pair<int, int>> evrec;   // (kernel class, first event index)
+    int kc_launches[KC_COUNT] = {0};
+    double kc_seconds[KC_COUNT] = {0};
+    double kc_bytes[KC_COUNT] = {0};
     // host results
     std::vector<float> h_out, h_tfm;
     std::vector<char> h_alns;
@@ -165,8 +172,10 @@ void gta_destroy(gta_ctx* c) {
     c->q.release(); c->r.release();
     DevBuf* bufs[] = {&c->s_flags, &c->s_grand, &c->s_best0, &c->s_prev, &c->s_naln, &c->s_qpos,
                       &c->s_rpos, &c->s_fndx, &c->s_fpos, &c->s_tibest, &c->s_tgrand, &c->s_out,
-                      &c->s_aln, &c->s_btck, &c->s_btckoff, &c->s_rowbuf, &c->s_cells, &c->s_alns};
+                      &c->s_aln, &c->s_btck, &c->s_btckoff, &c->s_rowbuf, &c->s_cells, &c->s_alns,
+                      &c->s_algb};
     for (DevBuf* b : bufs) b->release();
+    for (hipEvent_t e : c->evpool) (void)hipEventDestroy(e);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -272,6 +281,7 @@ static int run_stage(gta_ctx* c, double* seconds, double* cells) {
     const size_t alnplane = (size_t)nq * c->r.nposs;
     HIPCHK(c, c->s_aln.ensure(alnplane * 9 * 4));
     HIPCHK(c, c->s_cells.ensure(8));
+    HIPCHK(c, c->s_algb.ensure(KC_COUNT * 8));
     // backtrace: per pair ceil(Lq/64) strips x ceil((Lr+63)/16) blocks x 64 words
     std::vector<unsigned long long> off(np);
     unsigned long long tot = 0;
@@ -305,12 +315,39 @@ static int run_stage(gta_ctx* c, double* seconds, double* cells) {
     S.btck = c->s_btck.as<unsigned>(); S.btckoff = c->s_btckoff.as<unsigned long long>();
     S.rowbuf = dp_in_lds ? nullptr : c->s_rowbuf.as<float>();
     S.cells = c->s_cells.as<unsigned long long>();
+    S.algbytes = c->s_algb.as<unsigned long long>();
 
     const DevChunk Q = c->q.view(), R = c->r.view();
     const Geom G = make_geom(c);
     const gta_options& o = c->opts;
     hipStream_t st = c->stream;
 
+    c->evrec.clear();
+    size_t evnext = 0;
+    // brackets one launch with events when profiling (kernel class kc)
+    auto ev_begin = [&](int kc) -> hipError_t {
+        if (!c->profiling) return hipSuccess;
+        while (c->evpool.size() < evnext + 2) {
+            hipEvent_t e;
+            hipError_t er = hipEventCreate(&e);
+            if (er != hipSuccess) return er;
+            c->evpool.push_back(e);
+        }
+        c->evrec.push_back({kc, (int)evnext});
+        hipError_t er = hipEventRecord(c->evpool[evnext], st);
+        evnext += 2;
+        return er;
+    };
+    auto ev_end = [&]() -> hipError_t {
+        if (!c->profiling) return hipSuccess;
+        return hipEventRecord(c->evpool[c->evrec.back().second + 1], st);
+    };
+#define GTA_TIMED(kc, expr)                  \
+    do {                                     \
+        HIPCHK(c, ev_begin(kc));             \
+        HIPCHK(c, (expr));                   \
+        HIPCHK(c, ev_end());                 \
+    } while (0)
     int maxaln = std::min(c->q.len.front(), c->r.len.front());
     RefineLaunch RL{dinv, 0, 0, 0, 0.f, std::min(maxaln, 1536)};
     auto dp = [&](int mode, int gap0, float gapopen, int usess, int dpscan) -> hipError_t {
@@ -325,45 +362,47 @@ static int run_stage(gta_ctx* c, double* seconds, double* cells) {
     // DPRefine<GAP0,PRESCREEN> (MpStage1.cpp:455-510)
     auto dprefine = [&](int gap0, float prescore, int maxdpi) -> int {
         for (int dpi = 0; dpi < maxdpi; dpi++) {
-            HIPCHK(c, dp(DPM_IBEST_L, gap0, gap0 ? 0.0f : -0.6f, 0, 0));
-            HIPCHK(c, refine(RM_DPREFINE_L, 1, dpi, maxdpi, prescore));
+            GTA_TIMED(KC_DP, dp(DPM_IBEST_L, gap0, gap0 ? 0.0f : -0.6f, 0, 0));
+            GTA_TIMED(KC_REFINE_DP, refine(RM_DPREFINE_L, 1, dpi, maxdpi, prescore));
         }
         return GTA_OK;
     };
 
     HIPCHK(c, hipMemsetAsync(c->s_cells.p, 0, 8, st));
+    HIPCHK(c, hipMemsetAsync(c->s_algb.p, 0, KC_COUNT * 8, st));
     HIPCHK(c, hipMemsetAsync(c->s_alns.p, 0, alnsize, st));
     HIPCHK(c, hipEventRecord(c->ev0, st));
-    HIPCHK(c, launch_secstr(Q, st));
-    HIPCHK(c, launch_secstr(R, st));
-    HIPCHK(c, launch_init(Q, R, S, st));
+    GTA_TIMED(KC_SECSTR, launch_secstr(Q, st));
+    GTA_TIMED(KC_SECSTR, launch_secstr(R, st));
+    GTA_TIMED(KC_INIT, launch_init(Q, R, S, st));
     // stage 1 (MpStage1.h:78-85)
-    HIPCHK(c, launch_findfrag(prec, Q, R, S, G, dinv, st));
-    HIPCHK(c, launch_refine_gapless(prec, Q, R, S, G, dinv, st));
+    GTA_TIMED(KC_FINDFRAG, launch_findfrag(prec, Q, R, S, G, dinv, st));
+    GTA_TIMED(KC_REFINE_GAPLESS, launch_refine_gapless(prec, Q, R, S, G, dinv, st));
     int rc;
     if ((rc = dprefine(1, o.prescore, 2)) != GTA_OK) return rc;
     // stage 2, pass 1: SS-weighted DP with gap cost (MpStage2.h:60-80)
-    HIPCHK(c, dp(DPM_GRAND_L, 0, -1.0f, 1, 1));
-    HIPCHK(c, refine(RM_DPREFINE_L, 0, 0, 0, 0.f));
+    GTA_TIMED(KC_DP, dp(DPM_GRAND_L, 0, -1.0f, 1, 1));
+    GTA_TIMED(KC_REFINE_DP, refine(RM_DPREFINE_L, 0, 0, 0, 0.f));
     if ((rc = dprefine(0, 0.f, 2)) != GTA_OK) return rc;
     if ((rc = dprefine(1, 0.f, 2)) != GTA_OK) return rc;
     // stage 2, pass 2: no gap cost, convergence iterations
-    HIPCHK(c, dp(DPM_GRAND_L, 1, 0.0f, 0, 0));
-    HIPCHK(c, refine(RM_DPREFINE_L, 0, 0, 0, 0.f));
+    GTA_TIMED(KC_DP, dp(DPM_GRAND_L, 1, 0.0f, 0, 0));
+    GTA_TIMED(KC_REFINE_DP, refine(RM_DPREFINE_L, 0, 0, 0, 0.f));
     if ((rc = dprefine(0, 0.f, 2)) != GTA_OK) return rc;
     if ((rc = dprefine(1, 0.f, o.convergence)) != GTA_OK) return rc;
-    if (0.0f < o.scorethld) HIPCHK(c, launch_setlow(Q, R, S, o.scorethld, st));
+    if (0.0f < o.scorethld) GTA_TIMED(KC_MISC, launch_setlow(Q, R, S, o.scorethld, st));
     // final stage (MpStageFin.h:66-90)
-    HIPCHK(c, dp(DPM_GRAND_L, 1, 0.0f, 0, 0));
-    HIPCHK(c, refine(RM_FIN1_L, 0, 0, 0, 0.f));
+    GTA_TIMED(KC_DP, dp(DPM_GRAND_L, 1, 0.0f, 0, 0));
+    GTA_TIMED(KC_REFINE_DP, refine(RM_FIN1_L, 0, 0, 0, 0.f));
     if (!(o.depth == GTA_DEPTH_SHALLOW || o.refinement == GTA_REF_COARSE))
-        HIPCHK(c, refine(RM_FIN2_L, 0, 0, 0, 0.f));
-    HIPCHK(c, dp(DPM_CONSTRAINED_L, 1, 0.0f, 0, 0));
-    HIPCHK(c, launch_prod_aln(Q, R, S, c->s_alns.as<char>(), o.nodeletions, st));
-    HIPCHK(c, refine(RM_PROD1_L, 0, 0, 0, 0.f));
+        GTA_TIMED(KC_REFINE_DP, refine(RM_FIN2_L, 0, 0, 0, 0.f));
+    GTA_TIMED(KC_DP, dp(DPM_CONSTRAINED_L, 1, 0.0f, 0, 0));
+    GTA_TIMED(KC_PROD_ALN, launch_prod_aln(Q, R, S, c->s_alns.as<char>(), o.nodeletions, st));
+    GTA_TIMED(KC_REFINE_DP, refine(RM_PROD1_L, 0, 0, 0, 0.f));
     if (o.refinement == GTA_REF_ONE || o.refinement == GTA_REF_COARSE)
-        HIPCHK(c, refine(RM_PROD2_L, 0, 0, 0, 0.f));
-    if (o.referenced) HIPCHK(c, launch_revert(S, st));
+        GTA_TIMED(KC_REFINE_DP, refine(RM_PROD2_L, 0, 0, 0, 0.f));
+    if (o.referenced) GTA_TIMED(KC_MISC, launch_revert(S, st));
+#undef GTA_TIMED
     HIPCHK(c, hipEventRecord(c->ev1, st));
     HIPCHK(c, hipEventSynchronize(c->ev1));
     float ms = 0.f;
@@ -372,7 +411,38 @@ static int run_stage(gta_ctx* c, double* seconds, double* cells) {
     HIPCHK(c, hipMemcpy(&ncells, c->s_cells.p, 8, hipMemcpyDeviceToHost));
     if (seconds) *seconds = ms * 1e-3;
     if (cells) *cells = (double)ncells;
+    for (int k = 0; k < KC_COUNT; k++) { c->kc_launches[k] = 0; c->kc_seconds[k] = 0; }
+    if (c->profiling) {
+        for (const auto& e : c->evrec) {
+            float t = 0.f;
+            HIPCHK(c, hipEventElapsedTime(&t, c->evpool[e.second], c->evpool[e.second + 1]));
+            c->kc_launches[e.first]++;
+            c->kc_seconds[e.first] += t * 1e-3;
+        }
+    }
+    unsigned long long ab[KC_COUNT];
+    HIPCHK(c, hipMemcpy(ab, c->s_algb.p, sizeof(ab), hipMemcpyDeviceToHost));
+    for (int k = 0; k < KC_COUNT; k++) c->kc_bytes[k] = (double)ab[k];
     return GTA_OK;
+}
+
+int gta_set_profiling(gta_ctx* c, int on) {
+    if (!c) return fail(nullptr, GTA_ERR_ARG, "null context");
+    c->profiling = on ? 1 : 0;
+    return GTA_OK;
+}
+
+int gta_kernel_stats(const gta_ctx* c, gta_kernel_stat* out, int n) {
+    static const char* names[KC_COUNT] = {"secstr", "init", "findfrag", "refine_gapless", "dp",
+                                          "refine_dp", "prod_aln", "misc"};
+    if (!c) return GTA_ERR_ARG;
+    for (int k = 0; k < KC_COUNT && k < n; k++) {
+        out[k].name = names[k];
+        out[k].launches = c->kc_launches[k];
+        out[k].seconds = c->kc_seconds[k];
+        out[k].alg_bytes = c->kc_bytes[k];
+    }
+    return KC_COUNT;
 }
 
 int gta_process_resident(gta_ctx* c, double* seconds, double* dp_cells) {

@@ -75,7 +75,17 @@ struct DevState {
     const unsigned long long* btckoff;  // per-pair offsets (in words)
     float* rowbuf;          // strip boundary rows for long references, laid out like aln planes
     unsigned long long* cells;   // DP cells updated (device counter)
+    unsigned long long* algbytes;  // algorithmic bytes per kernel class (KC_*)
 };
+
+// kernel classes for the measurement counters
+enum { KC_SECSTR = 0, KC_INIT, KC_FINDFRAG, KC_REFINE_GAPLESS, KC_DP, KC_REFINE_DP, KC_PROD_ALN,
+       KC_MISC, KC_COUNT };
+
+// algorithmic bytes per aligned pair and pass, and per slot iteration
+// (covariance pass 24 B, TM-score pass and thresholded covariance pass 28 B,
+// RMSD pass 24 B; 64 B covariance record + 48 B transform per iteration)
+constexpr unsigned long long AB_CC = 24, AB_TM = 28, AB_EXT = 28, AB_RMSD = 24, AB_SLOT = 112;
 
 __device__ __forceinline__ float sqrd(float x) { return x * x; }
 

@@ -277,6 +277,7 @@ __global__ __launch_bounds__(RTHREADS) void k_findfrag(DevChunk Q, DevChunk R, D
     const int prv = max(min(qlen, rlen) >> 1, 5);
     SlotBest mine;
     mine.best = 0.f; mine.slot = 0x7fffffff; mine.ndx = 0; mine.pos = 0;
+    unsigned long long ab = 0;
     for (int si = threadIdx.x / GTA_WAVE; si < G.nseeds; si += RWAVES) {
         const int abspos = G.n1 + si * G.stepinit;
         const int qpos = max(0, abspos), rpos = max(-abspos, 0);
@@ -290,18 +291,22 @@ __global__ __launch_bounds__(RTHREADS) void k_findfrag(DevChunk Q, DevChunk R, D
             if (round == 0) {
                 cc_gapless<P>(q, r, qpos, rpos, n, cc);
                 cnt = n;
+                ab += AB_CC * n;
             } else {
                 float tprev[12];
                 copy12(tprev, tfm);
                 cnt = cc_gapless_ext<P>(q, r, qpos, rpos, n, tprev, thr, cc);
+                ab += AB_EXT * n;
                 if (cnt < 1 || cnt == n) break;
             }
+            ab += AB_TM * n + AB_SLOT;
             kabsch<T>(cc, (T)cnt, dinv, tfm);
             float sc = score_gapless_seed<P>(q, r, qpos, rpos, n, tfm, d0, d02, round > 0, round < 2, &thr);
             if (best < sc) best = sc;
         }
         if (best > mine.best) { mine.best = best; mine.slot = si; mine.ndx = qpos; mine.pos = rpos; }
     }
+    if (lane_id() == 0 && ab) atomicAdd(S.algbytes + KC_FINDFRAG, ab);
     SlotBest b;
     block_best(mine, lds, b);
     if (threadIdx.x == 0) {
@@ -332,6 +337,7 @@ __global__ __launch_bounds__(RTHREADS) void k_refine_gapless(DevChunk Q, DevChun
     const float d0s1 = GetD0s(d0) + 1.0f, d02s1 = d0s1 * d0s1;
     SlotBest mine;
     mine.best = 0.f; mine.slot = 0x7fffffff;
+    unsigned long long ab = 0;
     if (!(qlen <= qpos || rlen <= rpos)) {
         const int nall = min(qlen - qpos, rlen - rpos);
         for (int si = threadIdx.x / GTA_WAVE; si < G.nlocsteps; si += RWAVES) {
@@ -347,6 +353,7 @@ __global__ __launch_bounds__(RTHREADS) void k_refine_gapless(DevChunk Q, DevChun
                 UpdateLengths(ql, rl, qp, rp, fl);
                 cc_gapless<P>(q, r, qp, rp, fl, cc);
                 ncc = (T)fl;
+                ab += AB_CC * fl;
             }
             float tfm[12], tfmBest[12], thr = CP_LARGEDST, best = 0.f;
             for (int cit = 0; cit < NMAXCONVIT + 2; cit++) {
@@ -355,6 +362,7 @@ __global__ __launch_bounds__(RTHREADS) void k_refine_gapless(DevChunk Q, DevChun
                     copy12(tprev, tfm);
                     int cnt = cc_gapless_ext<P>(q, r, qpos, rpos, nall, tprev, (cit < 2) ? thr : d02s1, cc);
                     ncc = (T)cnt;
+                    ab += AB_EXT * nall;
                     if (cc_converged<T>(cc, ncc, last, nlast)) break;
                 }
                 if (ncc < (T)1) break;
@@ -363,11 +371,13 @@ __global__ __launch_bounds__(RTHREADS) void k_refine_gapless(DevChunk Q, DevChun
                 nlast = ncc;
                 kabsch<T>(cc, ncc, dinv, tfm);
                 float sc = score_gapless_refined<P>(q, r, qpos, rpos, nall, tfm, d0, d02, d82, cit < 1, &thr);
+                ab += AB_TM * nall + AB_SLOT;
                 if (best < sc) { best = sc; copy12(tfmBest, tfm); }
             }
             if (best > mine.best) { mine.best = best; mine.slot = si; copy12(mine.tfm, tfmBest); }
         }
     }
+    if (lane_id() == 0 && ab) atomicAdd(S.algbytes + KC_REFINE_GAPLESS, ab);
     SlotBest b;
     block_best(mine, lds, b);
     if (threadIdx.x == 0) {
@@ -515,7 +525,11 @@ __global__ __launch_bounds__(GTA_WAVE) void k_dp(DevChunk Q, DevChunk R, DevStat
         }
         if (!LDSROW) __threadfence_block();
     }
-    if (lane == 0) atomicAdd(S.cells, (unsigned long long)Lq * (unsigned long long)Lr);
+    if (lane == 0) {
+        const unsigned long long cells = (unsigned long long)Lq * (unsigned long long)Lr;
+        atomicAdd(S.cells, cells);
+        atomicAdd(S.algbytes + KC_DP, cells / 4 + 12ull * (unsigned long long)(Lq + Lr));
+    }
     __threadfence();
 
     // ---- backtrace walk --------------------------------------------------------
@@ -658,7 +672,7 @@ __device__ float score_dp(const Aln& a, int L, const float* tfm, float d0, float
 template <int P>
 __device__ float refine_slot_dp(const Aln& a, int L, int fl, int fragpos, float d0, float d02, float d82,
                                 bool chk, float bestinit, bool dinv, float* tfmBest,
-                                const typename Prec<P>::T* ccinit) {
+                                const typename Prec<P>::T* ccinit, unsigned long long& ab) {
     typedef typename Prec<P>::T T;
     T cc[15], last[15], ncc, nlast = 0;
     if (ccinit) {
@@ -666,6 +680,7 @@ __device__ float refine_slot_dp(const Aln& a, int L, int fl, int fragpos, float 
     } else {
         cc_dp_frag<P>(a, L, fragpos, fl, cc, nullptr);
     }
+    ab += AB_CC * fl;
     ncc = (T)fl;
     const float d0s1 = GetD0s(d0) + 1.0f, d02s1 = d0s1 * d0s1;
     float tfm[12], thr = CP_LARGEDST, best = bestinit;
@@ -676,6 +691,7 @@ __device__ float refine_slot_dp(const Aln& a, int L, int fl, int fragpos, float 
             copy12(tprev, tfm);
             int cnt = cc_dp_ext<P>(a, L, tprev, (cit < 2) ? thr : d02s1, cc);
             ncc = (T)cnt;
+            ab += AB_EXT * L;
             if (cc_converged<T>(cc, ncc, last, nlast)) break;
         }
         if (ncc < (T)1) break;
@@ -683,6 +699,7 @@ __device__ float refine_slot_dp(const Aln& a, int L, int fl, int fragpos, float 
         nlast = ncc;
         kabsch<T>(cc, ncc, dinv, tfm);
         float sc = score_dp<P>(a, L, tfm, d0, d02, d82, chk, cit < 1, &thr);
+        ab += AB_TM * L + AB_SLOT;
         if (best < sc) { best = sc; copy12(tfmBest, tfm); }
     }
     return best;
@@ -738,6 +755,7 @@ __global__ __launch_bounds__(RTHREADS) void k_refine_dp(DevChunk Q, DevChunk R, 
 
     SlotBest mine;
     mine.best = 0.f; mine.slot = 0x7fffffff; mine.gbest = 0.f; mine.ndx = 0; mine.pos = 0;
+    unsigned long long ab = 0;
     for (int si = threadIdx.x / GTA_WAVE; si < nslots; si += RWAVES) {
         int sfragndx, sfragpos, step;
         if (phase2) {
@@ -764,14 +782,18 @@ __global__ __launch_bounds__(RTHREADS) void k_refine_dp(DevChunk Q, DevChunk R, 
             ccp = ccinit;
         }
         float tfmBest[12];
-        float best = refine_slot_dp<P>(a, L, fl, sfragpos, d0, d02, d82, chk, bestinit, dinv, tfmBest, ccp);
+        float best = refine_slot_dp<P>(a, L, fl, sfragpos, d0, d02, d82, chk, bestinit, dinv, tfmBest, ccp, ab);
         float gbest = best;
-        if (prod && qlen != rlen) gbest = score_dp<P>(a, L, tfmBest, g0, g02, d82, false, false, nullptr);
+        if (prod && qlen != rlen) {
+            gbest = score_dp<P>(a, L, tfmBest, g0, g02, d82, false, false, nullptr);
+            ab += AB_TM * L;
+        }
         if (best > 0.f && best > mine.best) {
             mine.best = best; mine.slot = si; copy12(mine.tfm, tfmBest);
             mine.gbest = gbest; mine.ndx = sfragndx; mine.pos = sfragpos;
         }
     }
+    if (lane_id() == 0 && ab) atomicAdd(S.algbytes + KC_REFINE_DP, ab);
     SlotBest b;
     block_best(mine, slds, b);
     if (threadIdx.x != 0) return;

@@ -142,6 +142,25 @@ int gta_process(gta_ctx* ctx, gta_results* out);
  * with inputs and outputs resident). */
 int gta_process_resident(gta_ctx* ctx, double* seconds, double* dp_cells);
 
+/* ---- measurement (no counterpart in the reference, which only times the
+ * whole ProcessBlock with a wall clock, CuBatch.cu:450,684) ---------------
+ * With profiling on, every kernel launch of gta_process* is bracketed by HIP
+ * events on the stage stream; per kernel class the library reports the
+ * number of launches, the summed device time and the algorithmic bytes
+ * moved (per aligned pair and pass: 24 B covariance, 28 B TM-score,
+ * 28 B thresholded covariance; 112 B per slot iteration; DP: 2 bits per
+ * cell + 12 B per residue). */
+typedef struct gta_kernel_stat {
+    const char* name;
+    int launches;
+    double seconds;
+    double alg_bytes;
+} gta_kernel_stat;
+
+int gta_set_profiling(gta_ctx* ctx, int on);
+/* fills up to n entries; returns the number of kernel classes */
+int gta_kernel_stats(const gta_ctx* ctx, gta_kernel_stat* out, int n);
+
 /* message of the last error on ctx (thread-local when ctx is NULL) */
 const char* gta_last_error(const gta_ctx* ctx);
 

@@ -13,6 +13,7 @@
 
 #include <omp.h>
 #include <chrono>
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -68,8 +69,81 @@ void fill(const gta_chunk_view* c, ChunkFields& f) {
 
 extern "C" {
 
+// Runs one reference sub-chunk (which the reference's memory layout must hold).
+static int run_subchunk(const gta_chunk_view* q, const gta_chunk_view* r, const gta_options* o,
+                        MpGlobalMemory* gmem, float* alndata, float* tfms, char* alns) {
+    ChunkFields qf, rf;
+    fill(q, qf);
+    fill(r, rf);
+
+    const size_t nqystrs = q->nstrs, ndbCstrs = r->nstrs;
+    const size_t nqyposs = q->nposs, ndbCposs = r->nposs;
+    const size_t qystr1len = q->len[0], dbstr1len = r->len[0];
+    const size_t qystrnlen = q->len[nqystrs - 1], dbstrnlen = r->len[ndbCstrs - 1];
+    const size_t dbxpad = ALIGN_UP(ndbCposs, CUL2CLINESIZE) - ndbCposs;
+    if (gmem->GetCurrentMaxDbPos() < ndbCposs + dbxpad) return -4;
+    auto sec = [&](int s) { return gmem->GetHeap() + gmem->GetHeapSectionOffset(0, s); };
+
+    const uint maxnsteps = MpGlobalMemory::GetMaxNFragSteps();
+    const uint minfraglen = GetMinFragLengthForAln(MpGlobalMemory::GetMaxAlnLength());
+    const int stepinit = (o->depth == CLOptions::csdShallow) ? 5 : 1;
+
+    float* scores = (float*)sec(MpGlobalMemory::ddsBegOfMtxScores);
+    float* tmpdpdiagbuffers = (float*)sec(MpGlobalMemory::ddsBegOfDPDiagScores);
+    float* tmpdpbotbuffer = (float*)sec(MpGlobalMemory::ddsBegOfDPBottomScores);
+    float* tmpdpalnpossbuffer = (float*)sec(MpGlobalMemory::ddsBegOfDPAlignedPoss);
+    uint* maxcoordsbuf = (uint*)sec(MpGlobalMemory::ddsBegOfDPMaxCoords);
+    char* btckdata = sec(MpGlobalMemory::ddsBegOfDPBackTckData);
+    float* wrkmemory = (float*)sec(MpGlobalMemory::ddsBegOfWrkMemory);
+    float* wrkmemoryccd = (float*)sec(MpGlobalMemory::ddsBegOfWrkMemoryCCD);
+    float* wrkmemorytm = (float*)sec(MpGlobalMemory::ddsBegOfWrkMemoryTM);
+    float* wrkmemorytmibest = (float*)sec(MpGlobalMemory::ddsBegOfWrkMemoryTMibest);
+    float* auxwrkmemory = (float*)sec(MpGlobalMemory::ddsBegOfAuxWrkMemory);
+    float* wrkmemory2 = (float*)sec(MpGlobalMemory::ddsBegOfWrkMemory2);
+    float* alndatamemory = (float*)sec(MpGlobalMemory::ddsBegOfDP2AlnData);
+    float* tfmmemory = (float*)sec(MpGlobalMemory::ddsBegOfTfmMatrices);
+    char* alnsmemory = sec(MpGlobalMemory::ddsBegOfDP2Alns);
+    uint* globvarsbuf = (uint*)sec(MpGlobalMemory::ddsBegOfGlobVars);
+
+    char** qb = qf.beg; char** qe = qf.end;
+    char** rb = rf.beg; char** re = rf.end;
+    const size_t szalns = gmem->GetSizeOfDP2Alns(nqystrs, nqyposs, ndbCposs, ndbCstrs);
+    std::memset(alnsmemory, 0, szalns);
+
+    MpStage1 stg1(2, maxnsteps, minfraglen, o->prescore, stepinit, qb, qe, rb, re,
+                  nqystrs, ndbCstrs, nqyposs, ndbCposs, qystr1len, dbstr1len, qystrnlen, dbstrnlen,
+                  dbxpad, scores, tmpdpdiagbuffers, tmpdpbotbuffer, tmpdpalnpossbuffer, maxcoordsbuf,
+                  btckdata, wrkmemory, wrkmemoryccd, wrkmemorytm, wrkmemorytmibest, auxwrkmemory,
+                  wrkmemory2, alndatamemory, tfmmemory, globvarsbuf);
+    stg1.Preinitialize1(false);
+    MpSecStr(tmpdpdiagbuffers, qb, qe, rb, re, nqystrs, ndbCstrs, nqyposs, ndbCposs, qystr1len,
+             dbstr1len, qystrnlen, dbstrnlen, dbxpad).Run();
+    stg1.Run();
+    MpStage2 stg2(2, maxnsteps, minfraglen, o->prescore, stepinit, qb, qe, rb, re,
+                  nqystrs, ndbCstrs, nqyposs, ndbCposs, qystr1len, dbstr1len, qystrnlen, dbstrnlen,
+                  dbxpad, scores, tmpdpdiagbuffers, tmpdpbotbuffer, tmpdpalnpossbuffer, maxcoordsbuf,
+                  btckdata, wrkmemory, wrkmemoryccd, wrkmemorytm, wrkmemorytmibest, auxwrkmemory,
+                  wrkmemory2, alndatamemory, tfmmemory, globvarsbuf);
+    stg2.RunSpecialized<false, true, D02IND_DPSCAN>(2, false, o->scorethld);
+    stg2.RunSpecialized<true, false, D02IND_SEARCH>(o->convergence, true, o->scorethld);
+    MpStageFin(EQUIVALENCE_DISTANCE * EQUIVALENCE_DISTANCE, maxnsteps, minfraglen, qb, qe, rb, re,
+               nqystrs, ndbCstrs, nqyposs, ndbCposs, qystr1len, dbstr1len, qystrnlen, dbstrnlen,
+               dbxpad, scores, tmpdpdiagbuffers, tmpdpbotbuffer, tmpdpalnpossbuffer, maxcoordsbuf,
+               btckdata, wrkmemory, wrkmemoryccd, wrkmemorytm, wrkmemorytmibest, auxwrkmemory,
+               wrkmemory2, alndatamemory, tfmmemory, alnsmemory, globvarsbuf).Run();
+
+    const size_t np = nqystrs * ndbCstrs;
+    std::memcpy(alndata, alndatamemory, np * nTDP2OutputAlnData * sizeof(float));
+    std::memcpy(tfms, tfmmemory, np * nTTranformMatrix * sizeof(float));
+    std::memcpy(alns, alnsmemory, szalns);
+    return 0;
+}
+
 // Options are read once per process by the reference (function-local
 // statics), so a process must use one option set and one thread count.
+// References are processed in sub-chunks that fit the reference's memory
+// layout, as its task scheduler does; results are returned for the whole
+// reference chunk in the layout of the caller's chunk.
 int gta_ref_process_block(const gta_chunk_view* q, const gta_chunk_view* r, const gta_options* o,
                           int nthreads, float* alndata, float* tfms, char* alns, size_t alnsize,
                           double* seconds) {
@@ -85,82 +159,100 @@ int gta_ref_process_block(const gta_chunk_view* q, const gta_chunk_view* r, cons
         CLOptions::AssignCLOptO_REFERENCED(o->referenced);
         CLOptions::AssignCLOptC_NODETAILEDSEARCH(1);
 
-        ChunkFields qf, rf;
-        fill(q, qf);
-        fill(r, rf);
-
-        const size_t nqystrs = q->nstrs, ndbCstrs = r->nstrs;
-        const size_t nqyposs = q->nposs, ndbCposs = r->nposs;
-        const size_t qystr1len = q->len[0], dbstr1len = r->len[0];
-        const size_t qystrnlen = q->len[nqystrs - 1], dbstrnlen = r->len[ndbCstrs - 1];
-        const size_t dbxpad = ALIGN_UP(ndbCposs, CUL2CLINESIZE) - ndbCposs;
-
         static MpGlobalMemory* gmem = nullptr;
-        static size_t gmem_qposs = 0;
-        if (!gmem || gmem_qposs < nqyposs) {
-            delete gmem;
-            gmem = new MpGlobalMemory((size_t)8 * ONEG, 1);
-            gmem->CalcMaxDbDataChunkSize(nqyposs);
-            gmem_qposs = nqyposs;
+        if (!gmem) gmem = new MpGlobalMemory((size_t)8 * ONEG, 1);
+        // the reference's working memory holds at most GetMaxQueriesPerChunk()
+        // (--dev-queries-per-chunk, default 2) queries; group them as its
+        // scheduler does
+        const int qgroup = (int)CuMemoryBase::GetMaxQueriesPerChunk();
+        const int nq = q->nstrs, nr = r->nstrs;
+        std::vector<float> sa, st;
+        std::vector<char> sl;
+        if (alns) std::memset(alns, 0, alnsize);
+        double total = 0.0;
+        for (int q0 = 0; q0 < nq; q0 += qgroup) {
+            const int q1 = std::min(nq, q0 + qgroup);
+            gta_chunk_view qs = *q;
+            const unsigned qbase = q->dst[q0];
+            std::vector<unsigned> qdst(q1 - q0);
+            int qposs = 0;
+            for (int i = q0; i < q1; i++) { qdst[i - q0] = q->dst[i] - qbase; qposs += q->len[i]; }
+            qs.nstrs = q1 - q0;
+            qs.nposs = qposs;
+            qs.len = q->len + q0;
+            qs.dst = qdst.data();
+            qs.type = q->type ? q->type + q0 : nullptr;
+            qs.x = q->x + qbase; qs.y = q->y + qbase; qs.z = q->z + qbase;
+            qs.ins_ch_ord = q->ins_ch_ord ? q->ins_ch_ord + qbase : nullptr;
+            qs.resnum = q->resnum ? q->resnum + qbase : nullptr;
+            qs.rsd = q->rsd + qbase;
+            qs.ss = q->ss ? q->ss + qbase : nullptr;
+            gmem->CalcMaxDbDataChunkSize(qposs);
+            const size_t maxpos = gmem->GetCurrentMaxDbPos();
+            const size_t maxstrs = gmem->GetCurrentMaxNDbStrs();
+            const int nqs = qs.nstrs;
+            for (int r0 = 0; r0 < nr;) {
+                // greedy sub-chunk [r0, r1) within the memory layout's limits
+                int r1 = r0;
+                size_t pos = 0;
+                while (r1 < nr && (size_t)(r1 - r0) < maxstrs) {
+                    size_t np = pos + r->len[r1];
+                    if (ALIGN_UP(np, CUL2CLINESIZE) > maxpos) break;
+                    pos = np;
+                    r1++;
+                }
+                if (r1 == r0) return -4;
+                gta_chunk_view sub = *r;
+                const unsigned base = r->dst[r0];
+                std::vector<unsigned> sdst(r1 - r0);
+                for (int i = r0; i < r1; i++) sdst[i - r0] = r->dst[i] - base;
+                sub.nstrs = r1 - r0;
+                sub.nposs = (int)pos;
+                sub.len = r->len + r0;
+                sub.dst = sdst.data();
+                sub.type = r->type ? r->type + r0 : nullptr;
+                sub.x = r->x + base; sub.y = r->y + base; sub.z = r->z + base;
+                sub.ins_ch_ord = r->ins_ch_ord ? r->ins_ch_ord + base : nullptr;
+                sub.resnum = r->resnum ? r->resnum + base : nullptr;
+                sub.rsd = r->rsd + base;
+                sub.ss = r->ss ? r->ss + base : nullptr;
+                const size_t nps = (size_t)nqs * sub.nstrs;
+                sa.assign(nps * nTDP2OutputAlnData, 0.f);
+                st.assign(nps * nTTranformMatrix, 0.f);
+                sl.assign(gmem->GetSizeOfDP2Alns((size_t)nqs, (size_t)qs.nposs, (size_t)sub.nposs,
+                                                 (size_t)sub.nstrs), 0);
+                auto t0 = std::chrono::steady_clock::now();
+                int rc = run_subchunk(&qs, &sub, o, gmem, sa.data(), st.data(), sl.data());
+                total += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+                if (rc) return rc;
+                for (int qi = q0; qi < q1; qi++) {
+                    const int ql = qi - q0;
+                    for (int ri = r0; ri < r1; ri++) {
+                        const int rl = ri - r0;
+                        std::memcpy(alndata + ((size_t)qi * nr + ri) * nTDP2OutputAlnData,
+                                    sa.data() + ((size_t)ql * sub.nstrs + rl) * nTDP2OutputAlnData,
+                                    nTDP2OutputAlnData * sizeof(float));
+                        std::memcpy(tfms + ((size_t)qi * nr + ri) * nTTranformMatrix,
+                                    st.data() + ((size_t)ql * sub.nstrs + rl) * nTTranformMatrix,
+                                    nTTranformMatrix * sizeof(float));
+                        if (!alns) continue;
+                        // alignment rows: production_match2aln.cu:130-133 layout in both chunks
+                        const size_t qlen = q->len[qi];
+                        const size_t seg = qlen + r->len[ri] + 1;
+                        const size_t sofff = ((size_t)ql * sub.nposs + (size_t)sub.nstrs * (qdst[ql] + ql)) * 5;
+                        const size_t slen = (size_t)sub.nposs + (size_t)sub.nstrs * (qlen + 1);
+                        const size_t sbeg = (size_t)sdst[rl] + (size_t)rl * (qlen + 1);
+                        const size_t dofff = ((size_t)qi * r->nposs + (size_t)nr * (q->dst[qi] + qi)) * 5;
+                        const size_t dlen = (size_t)r->nposs + (size_t)nr * (qlen + 1);
+                        const size_t dbeg = (size_t)r->dst[ri] + (size_t)ri * (qlen + 1);
+                        for (int f = 0; f < 5; f++)
+                            std::memcpy(alns + dofff + dbeg + dlen * f, sl.data() + sofff + sbeg + slen * f, seg);
+                    }
+                }
+                r0 = r1;
+            }
         }
-        if (gmem->GetCurrentMaxDbPos() < ndbCposs + dbxpad) return -4;
-        auto sec = [&](int s) { return gmem->GetHeap() + gmem->GetHeapSectionOffset(0, s); };
-
-        const uint maxnsteps = MpGlobalMemory::GetMaxNFragSteps();
-        const uint minfraglen = GetMinFragLengthForAln(MpGlobalMemory::GetMaxAlnLength());
-        const int stepinit = (o->depth == CLOptions::csdShallow) ? 5 : 1;
-
-        float* scores = (float*)sec(MpGlobalMemory::ddsBegOfMtxScores);
-        float* tmpdpdiagbuffers = (float*)sec(MpGlobalMemory::ddsBegOfDPDiagScores);
-        float* tmpdpbotbuffer = (float*)sec(MpGlobalMemory::ddsBegOfDPBottomScores);
-        float* tmpdpalnpossbuffer = (float*)sec(MpGlobalMemory::ddsBegOfDPAlignedPoss);
-        uint* maxcoordsbuf = (uint*)sec(MpGlobalMemory::ddsBegOfDPMaxCoords);
-        char* btckdata = sec(MpGlobalMemory::ddsBegOfDPBackTckData);
-        float* wrkmemory = (float*)sec(MpGlobalMemory::ddsBegOfWrkMemory);
-        float* wrkmemoryccd = (float*)sec(MpGlobalMemory::ddsBegOfWrkMemoryCCD);
-        float* wrkmemorytm = (float*)sec(MpGlobalMemory::ddsBegOfWrkMemoryTM);
-        float* wrkmemorytmibest = (float*)sec(MpGlobalMemory::ddsBegOfWrkMemoryTMibest);
-        float* auxwrkmemory = (float*)sec(MpGlobalMemory::ddsBegOfAuxWrkMemory);
-        float* wrkmemory2 = (float*)sec(MpGlobalMemory::ddsBegOfWrkMemory2);
-        float* alndatamemory = (float*)sec(MpGlobalMemory::ddsBegOfDP2AlnData);
-        float* tfmmemory = (float*)sec(MpGlobalMemory::ddsBegOfTfmMatrices);
-        char* alnsmemory = sec(MpGlobalMemory::ddsBegOfDP2Alns);
-        uint* globvarsbuf = (uint*)sec(MpGlobalMemory::ddsBegOfGlobVars);
-
-        char** qb = qf.beg; char** qe = qf.end;
-        char** rb = rf.beg; char** re = rf.end;
-        const size_t szalns = gmem->GetSizeOfDP2Alns(nqystrs, nqyposs, ndbCposs, ndbCstrs);
-        std::memset(alnsmemory, 0, szalns);
-
-        auto t0 = std::chrono::steady_clock::now();
-        MpStage1 stg1(2, maxnsteps, minfraglen, o->prescore, stepinit, qb, qe, rb, re,
-                      nqystrs, ndbCstrs, nqyposs, ndbCposs, qystr1len, dbstr1len, qystrnlen, dbstrnlen,
-                      dbxpad, scores, tmpdpdiagbuffers, tmpdpbotbuffer, tmpdpalnpossbuffer, maxcoordsbuf,
-                      btckdata, wrkmemory, wrkmemoryccd, wrkmemorytm, wrkmemorytmibest, auxwrkmemory,
-                      wrkmemory2, alndatamemory, tfmmemory, globvarsbuf);
-        stg1.Preinitialize1(false);
-        MpSecStr(tmpdpdiagbuffers, qb, qe, rb, re, nqystrs, ndbCstrs, nqyposs, ndbCposs, qystr1len,
-                 dbstr1len, qystrnlen, dbstrnlen, dbxpad).Run();
-        stg1.Run();
-        MpStage2 stg2(2, maxnsteps, minfraglen, o->prescore, stepinit, qb, qe, rb, re,
-                      nqystrs, ndbCstrs, nqyposs, ndbCposs, qystr1len, dbstr1len, qystrnlen, dbstrnlen,
-                      dbxpad, scores, tmpdpdiagbuffers, tmpdpbotbuffer, tmpdpalnpossbuffer, maxcoordsbuf,
-                      btckdata, wrkmemory, wrkmemoryccd, wrkmemorytm, wrkmemorytmibest, auxwrkmemory,
-                      wrkmemory2, alndatamemory, tfmmemory, globvarsbuf);
-        stg2.RunSpecialized<false, true, D02IND_DPSCAN>(2, false, o->scorethld);
-        stg2.RunSpecialized<true, false, D02IND_SEARCH>(o->convergence, true, o->scorethld);
-        MpStageFin(EQUIVALENCE_DISTANCE * EQUIVALENCE_DISTANCE, maxnsteps, minfraglen, qb, qe, rb, re,
-                   nqystrs, ndbCstrs, nqyposs, ndbCposs, qystr1len, dbstr1len, qystrnlen, dbstrnlen,
-                   dbxpad, scores, tmpdpdiagbuffers, tmpdpbotbuffer, tmpdpalnpossbuffer, maxcoordsbuf,
-                   btckdata, wrkmemory, wrkmemoryccd, wrkmemorytm, wrkmemorytmibest, auxwrkmemory,
-                   wrkmemory2, alndatamemory, tfmmemory, alnsmemory, globvarsbuf).Run();
-        auto t1 = std::chrono::steady_clock::now();
-        if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
-
-        const size_t np = nqystrs * ndbCstrs;
-        std::memcpy(alndata, alndatamemory, np * nTDP2OutputAlnData * sizeof(float));
-        std::memcpy(tfms, tfmmemory, np * nTTranformMatrix * sizeof(float));
-        if (alns) std::memcpy(alns, alnsmemory, std::min(alnsize, szalns));
+        if (seconds) *seconds = total;
         return 0;
     } catch (...) {
         return -1;
