@@ -59,6 +59,9 @@ def lib():
         L.gta_set_profiling.restype = C.c_int
         L.gta_kernel_stats.argtypes = [C.c_void_p, C.POINTER(_KStat), C.c_int]
         L.gta_kernel_stats.restype = C.c_int
+        L.gta_test_dp.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_float, C.c_int, C.c_int,
+                                  C.c_void_p, C.c_void_p, C.c_void_p]
+        L.gta_test_dp.restype = C.c_int
         _lib = L
     return _lib
 
@@ -123,6 +126,25 @@ class GtaBatch:
         n = lib().gta_kernel_stats(self._h, arr, 16)
         return [{"name": arr[i].name.decode(), "launches": arr[i].launches,
                  "seconds": arr[i].seconds, "alg_bytes": arr[i].alg_bytes} for i in range(n)]
+
+    def dp_under(self, tfms, gap0=True, gapopen=0.0, usess=False, dpscan=False):
+        """The DP kernel alone under the given per-pair transforms (testing hook,
+        include/gta/gtalign_amd_testing.h): returns per pair the matched
+        (query, reference) positions in backtrace walk order."""
+        nq, nr = self._q.nstrs, self._r.nstrs
+        t = np.ascontiguousarray(tfms, np.float32).reshape(nq * nr * GTA_TFM_NFIELDS)
+        n = np.zeros(nq * nr, np.int32)
+        qp = np.zeros(nq * self._r.nposs, np.int32)
+        rp = np.zeros(nq * self._r.nposs, np.int32)
+        self._check(lib().gta_test_dp(self._h, t.ctypes.data, int(gap0), gapopen, int(usess),
+                                      int(dpscan), n.ctypes.data, qp.ctypes.data, rp.ctypes.data))
+        out = {}
+        for qi in range(nq):
+            for ri in range(nr):
+                k = int(n[qi * nr + ri])
+                o = qi * self._r.nposs + int(self._r.dst[ri])
+                out[qi, ri] = np.stack([qp[o:o + k], rp[o:o + k]], axis=1)
+        return out
 
     def close(self):
         if getattr(self, "_h", None):

@@ -47,8 +47,8 @@ def build_synth(out=SYNTH):
     os.replace(tmp, out)
 
 
-def build_oracle():
-    _run(["make", "-C", os.path.join(ROOT, "oracle"), "-j8", "all"])
+def build_oracle(force=False):
+    _run(["make", "-C", os.path.join(ROOT, "oracle"), "-j8"] + (["-B"] if force else []) + ["all"])
 
 
 def build_all():

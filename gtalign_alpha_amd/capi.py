@@ -18,6 +18,7 @@ STATUS = {
     0: "GTA_OK", -1: "GTA_ERR_ARG", -2: "GTA_ERR_EMPTY", -3: "GTA_ERR_TOOMANY",
     -4: "GTA_ERR_SIZE", -5: "GTA_ERR_LENGTH", -6: "GTA_ERR_HIP", -7: "GTA_ERR_STATE",
 }
+STATUS_CODES = {v: k for k, v in STATUS.items()}
 
 
 class ChunkView(C.Structure):

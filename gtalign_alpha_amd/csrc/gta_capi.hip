@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "gta/gtalign_amd.h"
+#include "gta/gtalign_amd_testing.h"
 #include "gta_launch.h"
 
 using namespace gta;
@@ -143,6 +144,8 @@ int gta_create(int device, const gta_options* opts, int max_qrys, int max_qry_po
     gta_default_options(&o);
     if (opts) o = *opts;
     if (!check_options(&o)) return fail(nullptr, GTA_ERR_ARG, "option out of range");
+    if (o.refinement == GTA_REF_FULLA)
+        return fail(nullptr, GTA_ERR_ARG, "refinement by full search over all fragment lengths is not supported");
     if (max_qrys < 1 || max_qry_poss < 1 || max_refs < 1 || max_ref_poss < 1)
         return fail(nullptr, GTA_ERR_ARG, "invalid capacity");
     if (max_qrys > 100) return fail(nullptr, GTA_ERR_TOOMANY, "more than 100 queries per chunk");
@@ -256,15 +259,11 @@ static Geom make_geom(const gta_ctx* c) {
     return g;
 }
 
-static int run_stage(gta_ctx* c, double* seconds, double* cells) {
-    if (!c) return fail(nullptr, GTA_ERR_ARG, "null context");
-    if (!c->have_q || !c->have_r) return fail(c, GTA_ERR_STATE, "queries and references must be set first");
-    HIPCHK(c, hipSetDevice(c->device));
+// device buffers of the pair state for the resident chunks
+static int prepare_state(gta_ctx* c, DevState& S, bool& dp_in_lds, size_t& dp_lds, size_t& alnsize) {
     const int nq = c->q.nstrs, nr = c->r.nstrs;
     const size_t np = (size_t)nq * nr;
-    const int prec = c->opts.precision;
-    const int dinv = c->opts.symmetric;
-
+    hipStream_t st = c->stream;
     // pair state
     HIPCHK(c, c->s_flags.ensure(np * 4));
     HIPCHK(c, c->s_grand.ensure(np * 4));
@@ -297,13 +296,12 @@ static int run_stage(gta_ctx* c, double* seconds, double* cells) {
     HIPCHK(c, c->s_btck.ensure(tot * 4));
     HIPCHK(c, c->s_btckoff.ensure(np * 8));
     HIPCHK(c, hipMemcpyAsync(c->s_btckoff.p, off.data(), np * 8, hipMemcpyHostToDevice, c->stream));
-    const size_t dp_lds = (size_t)maxlr * 20;
-    const bool dp_in_lds = dp_lds <= std::min<size_t>(c->lds_max, 65536);
+    dp_lds = (size_t)maxlr * 20;
+    dp_in_lds = dp_lds <= std::min<size_t>(c->lds_max, 65536);
     if (!dp_in_lds) HIPCHK(c, c->s_rowbuf.ensure(alnplane * 4));
-    const size_t alnsize = gta_alignment_section_size(nq, c->q.nposs, nr, c->r.nposs);
+    alnsize = gta_alignment_section_size(nq, c->q.nposs, nr, c->r.nposs);
     HIPCHK(c, c->s_alns.ensure(alnsize));
 
-    DevState S;
     S.nq = nq; S.nr = nr;
     S.flags = c->s_flags.as<int>(); S.grand = c->s_grand.as<float>(); S.best0 = c->s_best0.as<float>();
     S.prevbest0 = c->s_prev.as<float>(); S.nalnposs = c->s_naln.as<int>();
@@ -317,6 +315,25 @@ static int run_stage(gta_ctx* c, double* seconds, double* cells) {
     S.cells = c->s_cells.as<unsigned long long>();
     S.algbytes = c->s_algb.as<unsigned long long>();
 
+    return GTA_OK;
+}
+
+static int run_stage(gta_ctx* c, double* seconds, double* cells) {
+    if (!c) return fail(nullptr, GTA_ERR_ARG, "null context");
+    if (!c->have_q || !c->have_r) return fail(c, GTA_ERR_STATE, "queries and references must be set first");
+    HIPCHK(c, hipSetDevice(c->device));
+    const int nq = c->q.nstrs, nr = c->r.nstrs;
+    const size_t np = (size_t)nq * nr;
+    const int prec = c->opts.precision;
+    const int dinv = c->opts.symmetric;
+
+    DevState S;
+    bool dp_in_lds = false;
+    size_t dp_lds = 0, alnsize = 0;
+    {
+        int rc0 = prepare_state(c, S, dp_in_lds, dp_lds, alnsize);
+        if (rc0 != GTA_OK) return rc0;
+    }
     const DevChunk Q = c->q.view(), R = c->r.view();
     const Geom G = make_geom(c);
     const gta_options& o = c->opts;
@@ -443,6 +460,35 @@ int gta_kernel_stats(const gta_ctx* c, gta_kernel_stat* out, int n) {
         out[k].alg_bytes = c->kc_bytes[k];
     }
     return KC_COUNT;
+}
+
+int gta_test_dp(gta_ctx* c, const float* tfms, int gap0, float gapopen, int usess, int dpscan,
+                int* nmatched, int* qpos, int* rpos) {
+    if (!c || !tfms || !nmatched || !qpos || !rpos) return fail(c, GTA_ERR_ARG, "null argument");
+    if (!c->have_q || !c->have_r) return fail(c, GTA_ERR_STATE, "queries and references must be set first");
+    HIPCHK(c, hipSetDevice(c->device));
+    DevState S;
+    bool dp_in_lds = false;
+    size_t dp_lds = 0, alnsize = 0;
+    int rc = prepare_state(c, S, dp_in_lds, dp_lds, alnsize);
+    if (rc != GTA_OK) return rc;
+    const DevChunk Q = c->q.view(), R = c->r.view();
+    const size_t np = (size_t)c->q.nstrs * c->r.nstrs;
+    hipStream_t st = c->stream;
+    HIPCHK(c, launch_secstr(Q, st));
+    HIPCHK(c, launch_secstr(R, st));
+    HIPCHK(c, launch_init(Q, R, S, st));
+    HIPCHK(c, hipMemcpyAsync(S.tfm_grand, tfms, np * 12 * 4, hipMemcpyHostToDevice, st));
+    DpLaunch L{DPM_GRAND_L, gap0, gapopen, usess, usess ? 0.5f : 0.0f, dpscan, dp_in_lds ? dp_lds : 0};
+    HIPCHK(c, launch_dp(Q, R, S, L, st));
+    HIPCHK(c, hipStreamSynchronize(st));
+    std::vector<float> pl(S.alnplane);
+    HIPCHK(c, hipMemcpy(nmatched, S.nalnposs, np * 4, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(pl.data(), S.aln + 7 * S.alnplane, S.alnplane * 4, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < S.alnplane; i++) qpos[i] = (int)pl[i];
+    HIPCHK(c, hipMemcpy(pl.data(), S.aln + 8 * S.alnplane, S.alnplane * 4, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < S.alnplane; i++) rpos[i] = (int)pl[i];
+    return GTA_OK;
 }
 
 int gta_process_resident(gta_ctx* c, double* seconds, double* dp_cells) {

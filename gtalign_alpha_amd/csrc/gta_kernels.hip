@@ -576,7 +576,8 @@ __global__ __launch_bounds__(GTA_WAVE) void k_dp(DevChunk Q, DevChunk R, DevStat
                 const int o = k + before;
                 aqx[o] = qx; aqy[o] = qy; aqz[o] = qz;
                 arx[o] = rx; ary[o] = ry; arz[o] = rz;
-                if (constrained) { ad2[o] = d2; aqp[o] = (float)yy; arp[o] = (float)xx; }
+                aqp[o] = (float)yy; arp[o] = (float)xx;
+                if (constrained) ad2[o] = d2;
             }
             k += __popcll(km);
         }

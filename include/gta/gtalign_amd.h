@@ -69,7 +69,10 @@ typedef struct gta_chunk_view {
 
 /* ---- options: the CLOptions the stage reads (CuBatch.cu:353-361) ------ */
 enum gta_depth { GTA_DEPTH_DEEP = 0, GTA_DEPTH_HIGH, GTA_DEPTH_MEDIUM, GTA_DEPTH_SHALLOW };
-/* internal refinement codes (CLOptions.h:122-127; the CLI stores option+1) */
+/* internal refinement codes (CLOptions.h:122-127; the CLI stores option+1).
+ * GTA_REF_FULLA (final refinement by full search over all fragment lengths,
+ * MpStageFin.h:363-376) is not implemented: gta_create rejects it with
+ * GTA_ERR_ARG. */
 enum gta_refinement { GTA_REF_LOG = 0, GTA_REF_COARSE, GTA_REF_ONE, GTA_REF_FULL, GTA_REF_FULLA };
 enum gta_precision {
     GTA_PREC_FP32 = 0,       /* plain form: FP32 throughout, as the reference */

@@ -1239,14 +1239,9 @@ void process_pair(Ctx& c) {
     // final stage (MpStageFin.h:66-90)
     dp_align_grand(c, 0.0f, true, false, false, false);
     fin_phase1(c);
-    if (!(o.depth == GTA_DEPTH_SHALLOW || o.refinement == GTA_REF_COARSE)) {
-        if (o.refinement == GTA_REF_FULLA) {
-            // Phase2_fullsearch is not part of the default path
-            fin_phase2(c);
-        } else {
-            fin_phase2(c);
-        }
-    }
+    // (refinement by full search over all fragment lengths, MpStageFin.h:363, is
+    // not restated: the library rejects that option)
+    if (!(o.depth == GTA_DEPTH_SHALLOW || o.refinement == GTA_REF_COARSE)) fin_phase2(c);
     dp_align_grand(c, 0.0f, true, false, false, true);
     production_alignment(c);
     production_phase1(c);

@@ -149,3 +149,30 @@ def d0(qlen, rlen):
 
 def fraglen(qlen, rlen, qpos, rpos, ndx):
     return lib().gta_oracle_fraglen(qlen, rlen, qpos, rpos, ndx)
+
+
+def ref_dp(qxyz, rxyz, tfm, gap0=True, gapopen=0.0):
+    """The reference's own DP (oracle/_ref) for one pair under tfm:
+    returns (btck[Lq,Lr], matched pairs in walk order)."""
+    L = ref_lib()
+    if not hasattr(L, "_dp_ready"):
+        L.gta_ref_dp.argtypes = ([C.c_void_p] * 3 + [C.c_int] + [C.c_void_p] * 3 +
+                                 [C.c_int, C.c_void_p, C.c_int, C.c_float, C.c_void_p, C.c_void_p,
+                                  C.c_void_p])
+        L.gta_ref_dp.restype = C.c_int
+        L._dp_ready = True
+    qxyz = np.ascontiguousarray(qxyz, np.float32)
+    rxyz = np.ascontiguousarray(rxyz, np.float32)
+    Lq, Lr = len(qxyz), len(rxyz)
+    qc = [np.ascontiguousarray(qxyz[:, i]) for i in range(3)]
+    rc = [np.ascontiguousarray(rxyz[:, i]) for i in range(3)]
+    bt = np.zeros((Lq, Lr), dtype=np.uint8)
+    qp = np.zeros(Lq + Lr, dtype=np.int32)
+    rp = np.zeros(Lq + Lr, dtype=np.int32)
+    t = np.ascontiguousarray(tfm, np.float32)
+    n = L.gta_ref_dp(qc[0].ctypes.data, qc[1].ctypes.data, qc[2].ctypes.data, Lq, rc[0].ctypes.data,
+                     rc[1].ctypes.data, rc[2].ctypes.data, Lr, t.ctypes.data, int(gap0), gapopen,
+                     bt.ctypes.data, qp.ctypes.data, rp.ctypes.data)
+    if n < 0:
+        raise RuntimeError("reference DP failed (%d)" % n)
+    return bt, np.stack([qp[:n], rp[:n]], axis=1)

@@ -259,4 +259,87 @@ int gta_ref_process_block(const gta_chunk_view* q, const gta_chunk_view* r, cons
     }
 }
 
+// The reference's DP fill + backtrace (MpDPHub::ExecDPwBtck128xKernel and
+// BtckToMatched128xKernel, MpDPHub.cpp:73-388,1500-1610) for one pair under a
+// given transform; bt receives the Lq x Lr backtrace codes and qpos/rpos the
+// matched positions in walk order.  Returns the number of matched pairs.
+int gta_ref_dp(const float* qx, const float* qy, const float* qz, int Lq, const float* rx,
+               const float* ry, const float* rz, int Lr, const float* tfm, int gap0, float gapopen,
+               unsigned char* bt, int* qpos, int* rpos) {
+    try {
+        std::vector<int> ql{Lq}, rl{Lr}, zq(Lq, 0), zr(Lr, 0);
+        std::vector<unsigned> d0{0};
+        std::vector<char> aq(Lq, 'A'), ar(Lr, 'A');
+        gta_chunk_view q{1, Lq, ql.data(), d0.data(), nullptr, qx, qy, qz, zq.data(), nullptr, aq.data(), nullptr};
+        gta_chunk_view r{1, Lr, rl.data(), d0.data(), nullptr, rx, ry, rz, zr.data(), nullptr, ar.data(), nullptr};
+        ChunkFields qf, rf;
+        fill(&q, qf);
+        fill(&r, rf);
+        static MpGlobalMemory* gmem = nullptr;
+        if (!gmem) gmem = new MpGlobalMemory((size_t)2 * ONEG, 1);
+        gmem->CalcMaxDbDataChunkSize(Lq);
+        const size_t dbxpad = ALIGN_UP((size_t)Lr, CUL2CLINESIZE) - Lr;
+        if (gmem->GetCurrentMaxDbPos() < Lr + dbxpad) return -4;
+        auto sec = [&](int s) { return gmem->GetHeap() + gmem->GetHeapSectionOffset(0, s); };
+        const uint maxnsteps = MpGlobalMemory::GetMaxNFragSteps();
+        const uint minfraglen = GetMinFragLengthForAln(MpGlobalMemory::GetMaxAlnLength());
+        float* tmpdpdiagbuffers = (float*)sec(MpGlobalMemory::ddsBegOfDPDiagScores);
+        float* tmpdpbotbuffer = (float*)sec(MpGlobalMemory::ddsBegOfDPBottomScores);
+        float* tmpdpalnpossbuffer = (float*)sec(MpGlobalMemory::ddsBegOfDPAlignedPoss);
+        char* btckdata = sec(MpGlobalMemory::ddsBegOfDPBackTckData);
+        float* wrkmemorytmibest = (float*)sec(MpGlobalMemory::ddsBegOfWrkMemoryTMibest);
+        float* auxwrkmemory = (float*)sec(MpGlobalMemory::ddsBegOfAuxWrkMemory);
+        float* tfmmemory = (float*)sec(MpGlobalMemory::ddsBegOfTfmMatrices);
+        float* alndatamemory = (float*)sec(MpGlobalMemory::ddsBegOfDP2AlnData);
+        MpStage1 stg1(2, maxnsteps, minfraglen, 0.f, 1, qf.beg, qf.end, rf.beg, rf.end, 1, 1, Lq, Lr,
+                      Lq, Lr, Lq, Lr, dbxpad, (float*)sec(MpGlobalMemory::ddsBegOfMtxScores),
+                      tmpdpdiagbuffers, tmpdpbotbuffer, tmpdpalnpossbuffer,
+                      (uint*)sec(MpGlobalMemory::ddsBegOfDPMaxCoords), btckdata,
+                      (float*)sec(MpGlobalMemory::ddsBegOfWrkMemory),
+                      (float*)sec(MpGlobalMemory::ddsBegOfWrkMemoryCCD),
+                      (float*)sec(MpGlobalMemory::ddsBegOfWrkMemoryTM), wrkmemorytmibest, auxwrkmemory,
+                      (float*)sec(MpGlobalMemory::ddsBegOfWrkMemory2), alndatamemory, tfmmemory,
+                      (uint*)sec(MpGlobalMemory::ddsBegOfGlobVars));
+        stg1.Preinitialize1(false);
+        for (int f = 0; f < nTTranformMatrix; f++) wrkmemorytmibest[f] = tfm[f];
+        MpDPHub dph(maxnsteps, qf.beg, qf.end, rf.beg, rf.end, 1, 1, Lq, Lr, Lq, Lr, dbxpad,
+                    tmpdpdiagbuffers, tmpdpbotbuffer, tmpdpalnpossbuffer,
+                    (uint*)sec(MpGlobalMemory::ddsBegOfDPMaxCoords), btckdata,
+                    (float*)sec(MpGlobalMemory::ddsBegOfWrkMemory),
+                    (float*)sec(MpGlobalMemory::ddsBegOfWrkMemoryCCD),
+                    (float*)sec(MpGlobalMemory::ddsBegOfWrkMemoryTM), wrkmemorytmibest, auxwrkmemory,
+                    (float*)sec(MpGlobalMemory::ddsBegOfWrkMemory2), alndatamemory, tfmmemory,
+                    (uint*)sec(MpGlobalMemory::ddsBegOfGlobVars));
+        if (gap0)
+            dph.ExecDPwBtck128xKernel<false, false, true, D02IND_SEARCH, false>(
+                gapopen, 0, qf.beg, rf.beg, wrkmemorytmibest, auxwrkmemory, tmpdpdiagbuffers,
+                tmpdpbotbuffer, btckdata);
+        else
+            dph.ExecDPwBtck128xKernel<false, false, false, D02IND_SEARCH, false>(
+                gapopen, 0, qf.beg, rf.beg, wrkmemorytmibest, auxwrkmemory, tmpdpdiagbuffers,
+                tmpdpbotbuffer, btckdata);
+        dph.BtckToMatched128xKernel<false, false>(0, qf.beg, rf.beg, btckdata, auxwrkmemory,
+                                                  tmpdpalnpossbuffer);
+        const size_t dblen = Lr + dbxpad;
+        if (bt)
+            for (int i = 0; i < Lq; i++)
+                for (int j = 0; j < Lr; j++) bt[(size_t)i * Lr + j] = (unsigned char)btckdata[(size_t)i * dblen + j];
+        const int n = (int)auxwrkmemory[tawmvNAlnPoss];
+        // matched coordinates identify the positions (the walk order is kept)
+        for (int k = 0; k < n; k++) {
+            const float x = tmpdpalnpossbuffer[k + dpapsQRYx * dblen];
+            const float y = tmpdpalnpossbuffer[k + dpapsRFNx * dblen];
+            int a = -1, b = -1;
+            for (int i = 0; i < Lq && a < 0; i++) if (qx[i] == x) a = i;
+            for (int j = 0; j < Lr && b < 0; j++) if (rx[j] == y) b = j;
+            qpos[k] = a;
+            rpos[k] = b;
+        }
+        return n;
+    } catch (...) {
+        return -1;
+    }
+}
+
 }  // extern "C"
+

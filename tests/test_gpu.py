@@ -1,0 +1,285 @@
+"""Parity of the HIP path with the oracle and the reference's golden outputs.
+
+Every test here calls the HIP library through the C ABI (GtaBatch over
+libgta_amd.so) on device 0.  The bar: alignments byte-identical, integer
+statistics identical, TM-scores within 1e-4, RMSD within 1e-3 A; the DP
+kernel alone bit-exact in its matched positions.
+"""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+from gtalign_alpha_amd import synth
+from gtalign_alpha_amd.batch import GtaBatch, GtaError
+from gtalign_alpha_amd.capi import (default_options, PREC_FP32, PREC_MIXED, STATUS_CODES,
+                                    ALN_RMSD, ALN_SCOREQ, ALN_SCORER, ALN_D0Q, ALN_D0R)
+from gtalign_alpha_amd.chunk import Chunk
+from oracle import oracle
+from tests.util import alignment_rows, matched_positions, random_rotation, tm_fp64
+
+pytestmark = pytest.mark.gpu
+
+INT_FIELDS = (2, 3, 4, 5, 6, 7, 11, 12)
+GOLDEN = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.npz")))
+
+
+def run_gpu(q, r, prec=PREC_FP32, **kw):
+    opts = default_options(precision=prec, **kw)
+    with GtaBatch(0, opts, max_qrys=max(q.nstrs, 1), max_qry_poss=max(q.nposs, 1),
+                  max_refs=max(r.nstrs, 1), max_ref_poss=max(r.nposs, 1)) as b:
+        b.TransferQueryPMDataToDevice(q)
+        b.TransferCPMDataToDevice(r)
+        return b.ProcessBlock()
+
+
+def assert_same(expect, got, tm_tol=1e-4, rmsd_tol=1e-3):
+    assert np.array_equal(expect.alns, got.alns), int((expect.alns != got.alns).sum())
+    for f in INT_FIELDS:
+        assert np.array_equal(expect.alndata[..., f], got.alndata[..., f]), f
+    assert np.abs(expect.alndata[..., 9:11] - got.alndata[..., 9:11]).max() < tm_tol
+    assert np.abs(expect.alndata[..., 8] - got.alndata[..., 8]).max() < rmsd_tol
+
+
+def _cases():
+    out = []
+    qc, rc, _ = synth.config(0, scale=0.25)
+    out.append(("c0_1x64", qc, rc))
+    qc, rc, _ = synth.config(1, scale=0.002)
+    out.append(("c1_3x100", qc.subset([0, 7, 15]), rc))
+    qc, rc, _ = synth.config(2, scale=0.0002)
+    out.append(("c2_1x100", qc, rc))
+    return out
+
+
+CASES = _cases()
+
+
+@pytest.mark.parametrize("prec", [PREC_FP32, PREC_MIXED], ids=["fp32", "mixed"])
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_stage_matches_oracle(case, prec):
+    _, q, r = case
+    o = oracle.process_block(q, r)
+    g = run_gpu(q, r, prec)
+    assert_same(o, g)
+    assert (g.alndata[..., ALN_RMSD] < 9999).sum() >= 3
+
+
+@pytest.mark.parametrize("path", GOLDEN, ids=[os.path.basename(p) for p in GOLDEN])
+def test_stage_matches_reference_golden(path):
+    """Against the reference's own CPU outputs stored as fixtures."""
+    from tests.test_golden import load
+    q, r, d = load(path)
+    g = run_gpu(q, r)
+    assert np.array_equal(g.alns, d["alns"])
+    for f in INT_FIELDS:
+        assert np.array_equal(g.alndata[..., f], d["alndata"][..., f]), f
+    assert np.abs(g.alndata[..., 9:11] - d["alndata"][..., 9:11]).max() < 1e-4
+    assert np.abs(g.alndata[..., 8] - d["alndata"][..., 8]).max() < 1e-3
+    aligned = d["alndata"][..., 8] < 9999
+    assert np.abs(g.tfms - d["tfms"])[aligned].max() < 1e-3
+
+
+@pytest.mark.parametrize("prec", [PREC_FP32, PREC_MIXED], ids=["fp32", "mixed"])
+def test_reported_tm_equals_fp64_recomputation(prec):
+    """Size-independent check: the reported TM-scores equal an FP64 recomputation
+    from the output alignment and transform (d0 of each normalising length)."""
+    qc, rc, _ = synth.config(1, scale=0.004)
+    q = qc.subset([1, 8])
+    g = run_gpu(q, rc, prec)
+    n = 0
+    for qi in range(q.nstrs):
+        for ri in range(rc.nstrs):
+            a = g.alndata[qi, ri]
+            if a[ALN_RMSD] >= 9999:
+                continue
+            rows = alignment_rows(g.alns, q, rc, qi, ri)
+            pairs = matched_positions(rows, a)
+            tq = tm_fp64(q, rc, qi, ri, pairs, g.tfms[qi, ri], int(q.len[qi]))
+            tr = tm_fp64(q, rc, qi, ri, pairs, g.tfms[qi, ri], int(rc.len[ri]))
+            assert abs(tq - a[ALN_SCOREQ]) < 1e-4, (qi, ri)
+            assert abs(tr - a[ALN_SCORER]) < 1e-4, (qi, ri)
+            n += 1
+    assert n >= 5
+
+
+def test_short_chains_and_copies():
+    """Chains of 3-8 residues (below the fragment floors) and an exact copy."""
+    rng = np.random.default_rng(3)
+    qs = [synth.chain(1, 3), synth.chain(2, 5), synth.chain(3, 30)]
+    rs = [synth.chain(4, 3), synth.chain(5, 4), synth.chain(6, 8), synth.chain(7, 25), qs[2] + 1.0]
+    q = Chunk.from_structures(qs, [synth.residues(rng, len(s)) for s in qs])
+    r = Chunk.from_structures(rs, [synth.residues(rng, len(s)) for s in rs])
+    o = oracle.process_block(q, r)
+    for prec in (PREC_FP32, PREC_MIXED):
+        assert_same(o, run_gpu(q, r, prec))
+
+
+def test_long_reference_row_buffer_path():
+    """A reference longer than the DP's LDS row staging (> 3276 residues) runs
+    the DP with its boundary row in HBM."""
+    rng = np.random.default_rng(11)
+    src = synth.chain(501, 220)
+    big = synth.chain(502, 3600)
+    big[1000:1220] = src + np.float32(0.3) * rng.standard_normal(src.shape).astype(np.float32)
+    rs = [big, synth.chain(503, 180), synth.homolog(504, src, 1.0, 20.0, 0.05, 0.05)[0]]
+    q = Chunk.from_structures([src], [synth.residues(rng, 220)])
+    r = Chunk.from_structures(rs, [synth.residues(rng, len(s)) for s in rs])
+    o = oracle.process_block(q, r)
+    g = run_gpu(q, r)
+    assert_same(o, g)
+    assert g.alndata[0, 0, ALN_SCOREQ] > 0.5
+
+
+def test_options_depth_refinement_referenced():
+    qc, rc, _ = synth.config(0, scale=0.125)
+    for kw in (dict(refinement=0), dict(refinement=3), dict(depth=0), dict(depth=1),
+               dict(referenced=1), dict(nodeletions=1), dict(prescore=0.0, scorethld=0.0)):
+        o = oracle.process_block(qc, rc, default_options(**kw))
+        assert_same(o, run_gpu(qc, rc, **kw))
+
+
+def test_context_reuse_with_smaller_chunk():
+    """One context, two reference chunks in a row, the second smaller: results
+    equal those of fresh contexts."""
+    qc, rc, _ = synth.config(1, scale=0.002)
+    q = qc.subset([2, 9])
+    r2 = rc.subset(list(range(10, 40)))
+    with GtaBatch(0, default_options(), max_qrys=2, max_qry_poss=q.nposs, max_refs=rc.nstrs,
+                  max_ref_poss=rc.nposs) as b:
+        b.TransferQueryPMDataToDevice(q)
+        b.TransferCPMDataToDevice(rc)
+        g1 = b.ProcessBlock()
+        b.TransferCPMDataToDevice(r2)
+        g2 = b.ProcessBlock()
+    assert_same(run_gpu(q, rc), g1, tm_tol=1e-7, rmsd_tol=1e-7)
+    assert_same(oracle.process_block(q, r2), g2)
+
+
+# ---- the DP kernel alone (bit-exact) ---------------------------------------------------
+
+def _dp_workload(seed, nq, nr, lo, hi):
+    rng = np.random.default_rng(seed)
+    qs = [synth.chain(seed * 100 + i, int(rng.integers(lo, hi))) for i in range(nq)]
+    rs = []
+    for j in range(nr):
+        if j % 2 == 0:
+            # a noisy copy in the query's frame with deletions and inserted points
+            src = qs[j % nq]
+            keep = rng.random(len(src)) > 0.1
+            xyz = src[keep] + rng.normal(scale=1.5, size=(int(keep.sum()), 3))
+            at = np.sort(rng.integers(0, len(xyz), 3))
+            xyz = np.insert(xyz, at, rng.normal(scale=10.0, size=(3, 3)), axis=0)
+            rs.append(xyz.astype(np.float32))
+        else:
+            rs.append(synth.chain(seed * 1000 + j, int(rng.integers(lo, hi))))
+    q = Chunk.from_structures(qs, [synth.residues(rng, len(s)) for s in qs])
+    r = Chunk.from_structures(rs, [synth.residues(rng, len(s)) for s in rs])
+    tf = np.zeros((q.nstrs, r.nstrs, 12), np.float32)
+    for i in range(q.nstrs):
+        for j in range(r.nstrs):
+            # near-identity transforms give long diagonal runs, the others gappy paths
+            R = random_rotation(rng) if (i + j) % 3 == 0 else np.eye(3)
+            tf[i, j, :9] = R.reshape(-1)
+            tf[i, j, 9:] = rng.normal(scale=2.0, size=3)
+    return q, r, tf
+
+
+@pytest.mark.parametrize("gap0,gapopen,usess,dpscan",
+                         [(True, 0.0, False, False), (False, -0.6, False, False),
+                          (False, -1.0, False, False), (False, -0.6, True, True),
+                          (True, 0.0, False, True)],
+                         ids=["gap0", "gap-0.6", "gap-1.0", "ss+dpscan", "gap0+dpscan"])
+def test_dp_kernel_bit_exact(gap0, gapopen, usess, dpscan):
+    q, r, tf = _dp_workload(7, 3, 10, 3, 400)
+    ss_q = ss_r = None
+    if usess:
+        o = oracle.process_block(q, r)
+        ss_q, ss_r = o.ss_q, o.ss_r
+    with GtaBatch(0, default_options(), max_qrys=q.nstrs, max_qry_poss=q.nposs, max_refs=r.nstrs,
+                  max_ref_poss=r.nposs) as b:
+        b.TransferQueryPMDataToDevice(q)
+        b.TransferCPMDataToDevice(r)
+        got = b.dp_under(tf, gap0, gapopen, usess, dpscan)
+    for i in range(q.nstrs):
+        for j in range(r.nstrs):
+            Lq, Lr = int(q.len[i]), int(r.len[j])
+            d = oracle.d0(Lq, Lr)
+            d02 = float(d["d02_dpscan"]) if dpscan else float(d["d0"]) ** 2
+            kw = {}
+            if usess:
+                kw = dict(qss=bytes(ss_q[q.dst[i]:q.dst[i] + Lq]).decode(),
+                          rss=bytes(ss_r[r.dst[j]:r.dst[j] + Lr]).decode(), ssw=0.5)
+            _, want = oracle.dp(q.coords(i), r.coords(j), tf[i, j], d02, gapopen, gap0, **kw)
+            assert np.array_equal(got[i, j], want), (i, j, Lq, Lr)
+
+
+def test_dp_kernel_long_reference():
+    """The DP with its boundary row in HBM (reference > 3276 residues)."""
+    q, r, tf = _dp_workload(9, 2, 2, 100, 200)
+    big = synth.chain(77, 5000)
+    r = Chunk.from_structures([big, r.coords(0)], ["A" * 5000, r.residues(0)])
+    tf = np.concatenate([tf, tf], axis=1)[:, :2]
+    with GtaBatch(0, default_options(), max_qrys=q.nstrs, max_qry_poss=q.nposs, max_refs=r.nstrs,
+                  max_ref_poss=r.nposs) as b:
+        b.TransferQueryPMDataToDevice(q)
+        b.TransferCPMDataToDevice(r)
+        got = b.dp_under(tf, False, -0.6)
+    for i in range(q.nstrs):
+        for j in range(r.nstrs):
+            d02 = float(oracle.d0(int(q.len[i]), int(r.len[j]))["d0"]) ** 2
+            _, want = oracle.dp(q.coords(i), r.coords(j), tf[i, j], d02, -0.6, False)
+            assert np.array_equal(got[i, j], want), (i, j)
+
+
+# ---- the boundary's error behaviour ----------------------------------------------------
+
+def _code(name):
+    return STATUS_CODES[name]
+
+
+def test_error_codes():
+    qc, rc, _ = synth.config(0, scale=0.05)
+    with pytest.raises(GtaError) as e:
+        GtaBatch(0, default_options(), max_qrys=101)
+    assert e.value.code == _code("GTA_ERR_TOOMANY")
+    with pytest.raises(GtaError) as e:
+        GtaBatch(0, default_options(convergence=0))
+    assert e.value.code == _code("GTA_ERR_ARG")
+    with GtaBatch(0, default_options(), max_qrys=2, max_qry_poss=1000, max_refs=rc.nstrs,
+                  max_ref_poss=rc.nposs) as b:
+        with pytest.raises(GtaError) as e:
+            b.ProcessBlock()
+        assert e.value.code == _code("GTA_ERR_STATE")
+        unsorted = Chunk.from_structures([synth.chain(1, 20), synth.chain(2, 40)], sort=False)
+        with pytest.raises(GtaError) as e:
+            b.TransferQueryPMDataToDevice(unsorted)
+        assert e.value.code == _code("GTA_ERR_ARG")
+        big = Chunk.from_structures([synth.chain(3, 1200)])
+        with pytest.raises(GtaError) as e:
+            b.TransferQueryPMDataToDevice(big)
+        assert e.value.code == _code("GTA_ERR_SIZE")
+        empty = Chunk(np.zeros(0, np.int32), np.zeros(0), np.zeros(0), np.zeros(0))
+        with pytest.raises(GtaError) as e:
+            b.TransferCPMDataToDevice(empty)
+        assert e.value.code == _code("GTA_ERR_EMPTY")
+        # a failed upload leaves the context without queries
+        b.TransferCPMDataToDevice(rc)
+        with pytest.raises(GtaError) as e:
+            b.ProcessBlock()
+        assert e.value.code == _code("GTA_ERR_STATE")
+        b.TransferQueryPMDataToDevice(qc)
+        g = b.ProcessBlock()
+    assert_same(oracle.process_block(qc, rc), g)
+
+
+def test_structure_length_limit():
+    n = 65536
+    x = np.zeros(n, np.float32)
+    c = Chunk(np.array([n], np.int32), x, x, x)
+    with GtaBatch(0, default_options(), max_qrys=1, max_qry_poss=n, max_refs=1,
+                  max_ref_poss=n) as b:
+        with pytest.raises(GtaError) as e:
+            b.TransferCPMDataToDevice(c)
+    assert e.value.code == _code("GTA_ERR_LENGTH")

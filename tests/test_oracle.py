@@ -1,0 +1,234 @@
+"""Pins of the oracle (CPU): closed forms, brute force, and the reference's own
+CPU code where it was built (oracle/_ref)."""
+import itertools
+
+import numpy as np
+import pytest
+
+from gtalign_alpha_amd import synth
+from gtalign_alpha_amd.capi import default_options
+from oracle import oracle
+from tests.util import random_rotation, kabsch_svd, tfm_apply, d0_tm
+
+
+# ---- Kabsch (transformbase.h:167-583) ---------------------------------------------
+
+def test_kabsch_recovers_rigid_motion():
+    rng = np.random.default_rng(1)
+    for n in (3, 4, 10, 200):
+        x = rng.normal(scale=10.0, size=(n, 3))
+        R = random_rotation(rng)
+        t = rng.normal(scale=30.0, size=3)
+        y = x @ R.T + t
+        tf = oracle.kabsch(x, y)
+        Rk = tf[:9].reshape(3, 3).astype(np.float64)
+        assert np.allclose(Rk, R, atol=2e-3), n
+        assert np.allclose(tf[9:], t, atol=2e-2), n
+        assert np.allclose(Rk @ Rk.T, np.eye(3), atol=1e-4)
+        assert abs(np.linalg.det(Rk) - 1.0) < 1e-4
+
+
+def test_kabsch_matches_svd_on_noisy_data():
+    rng = np.random.default_rng(2)
+    for _ in range(20):
+        n = int(rng.integers(5, 300))
+        x = rng.normal(scale=12.0, size=(n, 3))
+        y = x @ random_rotation(rng).T + rng.normal(scale=20, size=3) + rng.normal(scale=2.0, size=(n, 3))
+        tf = oracle.kabsch(x, y)
+        R, t = kabsch_svd(x, y)
+        # compare the residuals: both are least-squares optima
+        e_orc = np.sqrt((((tfm_apply(tf, x) - y) ** 2).sum(1)).mean())
+        e_svd = np.sqrt((((x @ R.T + t - y) ** 2).sum(1)).mean())
+        assert e_orc <= e_svd * (1 + 1e-3) + 1e-4
+
+
+def test_kabsch_rejects_reflection():
+    # a mirrored copy must still give a proper rotation (det = +1)
+    rng = np.random.default_rng(3)
+    x = rng.normal(scale=10.0, size=(50, 3))
+    y = x * np.array([1.0, 1.0, -1.0])
+    Rk = oracle.kabsch(x, y)[:9].reshape(3, 3).astype(np.float64)
+    assert abs(np.linalg.det(Rk) - 1.0) < 1e-4
+
+
+def test_kabsch_degenerate_inputs_finite():
+    # collinear and coincident points: the reference falls back to a valid transform
+    x = np.stack([np.arange(6.0), np.zeros(6), np.zeros(6)], 1)
+    for y in (x + 3.0, np.zeros((6, 3)), x[::-1].copy()):
+        tf = oracle.kabsch(x, y)
+        assert np.all(np.isfinite(tf))
+
+
+def test_rmsd_e0_identity_matches_direct():
+    rng = np.random.default_rng(4)
+    for _ in range(10):
+        n = int(rng.integers(10, 300))
+        x = rng.normal(scale=10.0, size=(n, 3))
+        y = x @ random_rotation(rng).T + rng.normal(scale=1.5, size=(n, 3))
+        R, t = kabsch_svd(x, y)
+        direct = np.sqrt(((x @ R.T + t - y) ** 2).sum(1).mean())
+        # E0 identity in FP32 loses precision by cancellation; 1e-2 A bounds it here
+        assert abs(oracle.rmsd(x, y) - direct) < 1e-2
+
+
+# ---- d0 / fragment geometry (stagecnsts.cuh:48-175, fragment.cuh:52-71) ------------
+
+def test_d0_textbook_values():
+    for L in (22, 50, 100, 300, 1000):
+        d = oracle.d0(L, L + 7)
+        ref = 1.24 * np.cbrt(L - 15.0) - 1.8
+        assert abs(d["d0fin"] - ref) < 1e-5
+        assert abs(d["d0"] - (ref + 0.8)) < 1e-5
+        assert abs(d["d02_dpscan"] - (ref + 2.3) ** 2) < 1e-4
+        assert abs(d["d82"] - (1.5 * L ** 0.3 + 3.5) ** 2) < 1e-3
+        assert 4.5 <= d["d0s"] <= 8.0
+    assert abs(oracle.d0(10, 10)["d0fin"] - 0.5) < 1e-7
+    assert abs(oracle.d0(10, 10)["d0"] - (0.168 + 0.8)) < 1e-6
+
+
+def test_fraglen_halving_and_floor():
+    # fragment lengths halve with the index and never go below 4 (or 0 when too short)
+    for L in (5, 8, 9, 40, 100, 257):
+        lens = [oracle.fraglen(L, L, 0, 0, k) for k in range(6)]
+        assert lens[0] == L
+        for k in range(1, 6):
+            assert lens[k] == 0 or lens[k] >= 4
+            assert lens[k] <= max(4, L >> k) or lens[k] == 0
+        assert lens[5] in (0, 4)
+    assert oracle.fraglen(3, 3, 0, 0, 0) == 3
+    assert oracle.fraglen(50, 80, 10, 0, 0) == 40
+
+
+# ---- DP (MpDPHub.cpp:73-388) ------------------------------------------------------
+
+def _paths(Lq, Lr):
+    """All monotone paths from the boundary to (Lq-1, Lr-1) as lists of moves
+    into cells; a path may start at any boundary point (free leading gaps)."""
+    out = []
+
+    def rec(i, j, acc):
+        if i < 0 or j < 0:
+            out.append(list(reversed(acc)))
+            return
+        rec(i - 1, j - 1, acc + [("D", i, j)])
+        rec(i, j - 1, acc + [("L", i, j)])
+        rec(i - 1, j, acc + [("U", i, j)])
+
+    rec(Lq - 1, Lr - 1, [])
+    return out
+
+
+def test_dp_gap0_is_the_optimum_over_paths():
+    """Without gap costs the fill is a plain max over monotone paths; brute force
+    over every path of small matrices must reach the same optimum."""
+    rng = np.random.default_rng(5)
+    I = np.eye(3).reshape(-1).tolist() + [0, 0, 0]
+    for Lq, Lr in [(3, 4), (4, 4), (5, 3), (4, 6)]:
+        for _ in range(3):
+            q = rng.normal(scale=4.0, size=(Lq, 3))
+            r = rng.normal(scale=4.0, size=(Lr, 3))
+            d02 = 4.0
+            _, pairs = oracle.dp(q, r, I, d02, 0.0, True)
+            d2 = ((q[:, None, :].astype(np.float32) - r[None, :, :].astype(np.float32)) ** 2).sum(-1)
+            s = d02 / (d02 + d2)
+            best = max(sum(s[i, j] for m, i, j in p if m == "D") for p in _paths(Lq, Lr))
+            got = sum(s[i, j] for i, j in pairs)
+            assert abs(best - got) < 1e-5, (Lq, Lr)
+
+
+def test_dp_identical_chains_align_diagonally():
+    c = synth.chain(11, 80)
+    I = np.eye(3).reshape(-1).tolist() + [0, 0, 0]
+    for gap0, g in ((True, 0.0), (False, -0.6)):
+        _, pairs = oracle.dp(c, c, I, 9.0, g, gap0)
+        assert len(pairs) == 80
+        assert np.all(pairs[:, 0] == pairs[:, 1])
+
+
+@pytest.mark.parametrize("gap0,gapopen", [(True, 0.0), (False, -0.6), (False, -1.0)])
+def test_dp_matches_reference_cell_by_cell(have_ref, gap0, gapopen):
+    """The oracle's backtrace matrix equals the reference's MpDPHub matrix,
+    for random superpositions of synthetic chains."""
+    if not have_ref:
+        pytest.skip("reference CPU build (oracle/_ref) not present")
+    rng = np.random.default_rng(6)
+    for k in range(6):
+        Lq, Lr = int(rng.integers(20, 140)), int(rng.integers(20, 140))
+        q = synth.chain(100 + k, Lq)
+        r = synth.chain(200 + k, Lr)
+        R = random_rotation(rng)
+        t = rng.normal(scale=3.0, size=3)
+        tfm = np.concatenate([R.reshape(-1), t]).astype(np.float32)
+        d = oracle.d0(Lq, Lr)
+        bo, po = oracle.dp(q, r, tfm, float(d["d0"]) ** 2, gapopen, gap0)
+        br, pr = oracle.ref_dp(q, r, tfm, gap0, gapopen)
+        assert np.array_equal(bo, br), (k, Lq, Lr)
+        assert np.array_equal(po, pr), (k, Lq, Lr)
+
+
+# ---- the whole stage ------------------------------------------------------------------
+
+def test_self_alignment_scores_one():
+    """A chain against a rigidly moved copy of itself: TM-score 1, RMSD ~0,
+    every residue aligned with itself."""
+    rng = np.random.default_rng(8)
+    from gtalign_alpha_amd.chunk import Chunk
+    c = synth.chain(31, 120)
+    R, t = random_rotation(rng), rng.normal(scale=20.0, size=3)
+    res = synth.residues(rng, 120)
+    q = Chunk.from_structures([c], [res])
+    r = Chunk.from_structures([c @ R.T + t], [res])
+    o = oracle.process_block(q, r)
+    a = o.alndata[0, 0]
+    assert abs(a[9] - 1.0) < 1e-4 and abs(a[10] - 1.0) < 1e-4
+    # the reference's FP32 RMSD (Kabsch E0 identity) cancels ~1e3 A^2 variances:
+    # an exact copy reads a few hundredths of an angstrom, not 0
+    assert a[8] < 0.1
+    assert a[4] == 120 and a[6] == 120 and a[7] == 0
+    tf = o.tfms[0, 0]
+    assert np.allclose(tfm_apply(tf, c), c @ R.T + t, atol=2e-2)
+
+
+def test_stage_matches_reference_cpu_path(have_ref):
+    """The oracle reproduces the reference's own CPU path on a config-1 sample:
+    byte-identical alignments, identical statistics, TM-scores within 1e-4,
+    RMSD within 1e-3 A."""
+    if not have_ref:
+        pytest.skip("reference CPU build (oracle/_ref) not present")
+    qc, rc, _ = synth.config(0, scale=0.25)
+    o = oracle.process_block(qc, rc)
+    r = oracle.ref_process_block(qc, rc, nthreads=1)
+    assert np.array_equal(o.alns, r.alns)
+    for f in (2, 3, 4, 5, 6, 7, 11, 12):
+        assert np.array_equal(o.alndata[..., f], r.alndata[..., f]), f
+    assert np.abs(o.alndata[..., 9:11] - r.alndata[..., 9:11]).max() < 1e-4
+    assert np.abs(o.alndata[..., 8] - r.alndata[..., 8]).max() < 1e-3
+    assert (o.alndata[..., 8] < 9999).sum() > 10
+
+
+def test_options_change_outcome():
+    qc, rc, _ = synth.config(0, scale=0.1)
+    a = oracle.process_block(qc, rc, default_options())
+    b = oracle.process_block(qc, rc, default_options(scorethld=0.0, prescore=0.0))
+    # without the score thresholds every pair gets an alignment
+    assert (b.alndata[..., 8] < 9999).sum() == qc.nstrs * rc.nstrs
+    assert (a.alndata[..., 8] < 9999).sum() < qc.nstrs * rc.nstrs
+
+
+def test_short_chains_match_reference(have_ref):
+    """Chains of 3-8 residues (below the fragment-length floors) and an exact
+    copy: the oracle equals the reference's CPU path."""
+    if not have_ref:
+        pytest.skip("reference CPU build (oracle/_ref) not present")
+    from gtalign_alpha_amd.chunk import Chunk
+    rng = np.random.default_rng(3)
+    qs = [synth.chain(1, 3), synth.chain(2, 5), synth.chain(3, 30)]
+    rs = [synth.chain(4, 3), synth.chain(5, 4), synth.chain(6, 8), synth.chain(7, 25), qs[2] + 1.0]
+    q = Chunk.from_structures(qs, [synth.residues(rng, len(s)) for s in qs])
+    r = Chunk.from_structures(rs, [synth.residues(rng, len(s)) for s in rs])
+    o = oracle.process_block(q, r)
+    ref = oracle.ref_process_block(q, r)
+    assert np.array_equal(o.alns, ref.alns)
+    assert np.abs(o.alndata - ref.alndata).max() < 1e-4
+    # the 3-residue pair aligns completely; the exact copy scores 1
+    assert o.alndata[0, 0, 4] == 30 and abs(o.alndata[0, 0, 9] - 1.0) < 1e-6
