@@ -151,6 +151,8 @@ int gta_create(int device, const gta_options* opts, int max_qrys, int max_qry_po
     if (max_qrys > 100) return fail(nullptr, GTA_ERR_TOOMANY, "more than 100 queries per chunk");
     hipError_t e = hipSetDevice(device);
     if (e != hipSuccess) return fail(nullptr, GTA_ERR_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
+    e = upload_length_tables();
+    if (e != hipSuccess) return fail(nullptr, GTA_ERR_HIP, std::string("length tables: ") + hipGetErrorString(e));
     gta_ctx* c = new gta_ctx();
     c->device = device;
     c->opts = o;

@@ -89,37 +89,6 @@ constexpr unsigned long long AB_CC = 24, AB_TM = 28, AB_EXT = 28, AB_RMSD = 24, 
 
 __device__ __forceinline__ float sqrd(float x) { return x * x; }
 
-// ---- d0 / d8 (stagecnsts.cuh:48-175) -----------------------------------------
-__device__ __forceinline__ float GetLnorm(int q, int r) { return (float)min(q, r); }
-__device__ __forceinline__ float GetD82(int q, int r) {
-    float d8 = 1.5f * powf(GetLnorm(q, r), 0.3f) + 3.5f;
-    return d8 * d8;
-}
-__device__ __forceinline__ float GetD0(int q, int r) {
-    float l = GetLnorm(q, r), d0 = 0.168f;
-    if (l > 19.f) d0 = 1.24f * cbrtf(l - 15.f) - 1.8f;
-    return d0 + 0.8f;
-}
-__device__ __forceinline__ float GetD0finP(int q, int r) {
-    float l = GetLnorm(q, r), d0 = 0.5f;
-    if (l > 21.f) d0 = 1.24f * cbrtf(l - 15.f) - 1.8f;
-    if (d0 < 0.5f) d0 = 0.5f;
-    return d0;
-}
-__device__ __forceinline__ float GetD0finNA(int q, int r) {
-    float l = GetLnorm(q, r), d0 = 0.3f;
-    if (30.f <= l) d0 = 0.6f * sqrtf(l - 0.5f) - 2.5f;
-    else if (23.f < l) d0 = 0.7f;
-    else if (19.f < l) d0 = 0.6f;
-    else if (15.f < l) d0 = 0.5f;
-    else if (11.f < l) d0 = 0.4f;
-    return fmaxf(0.3f, d0);
-}
-__device__ __forceinline__ float GetD0fin(int q, int r, int na) {
-    return na ? GetD0finNA(q, r) : GetD0finP(q, r);
-}
-__device__ __forceinline__ float GetD0s(float d0) { return fmaxf(fminf(d0, 8.f), 4.5f); }
-
 // ---- fragment geometry (fragment.cuh:19-109) --------------------------------
 __device__ __forceinline__ int GetFragLength(int qlen, int rlen, int qpos, int rpos, int ndx) {
     int aln = min(qlen - qpos, rlen - rpos);

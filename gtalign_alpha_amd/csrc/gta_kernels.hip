@@ -26,10 +26,54 @@
 // FP64 RMSD).  DP cell arithmetic is FP32 in both and rounds exactly like the
 // reference's scalar code.
 
+#include <cmath>
+#include <vector>
+
 #include "gta_device.h"
 #include "gta_kabsch.h"
 
 namespace gta {
+
+// ---- d0 / d8 (stagecnsts.cuh:48-175) -----------------------------------------
+// cbrtf(L - 15) and powf(L, 0.3) per length L are evaluated once on the host,
+// with the C library the reference's CPU path uses: the device's own
+// cbrtf/powf differ from it in the last bit for some lengths, and d0 enters
+// every DP cell score, so a last-bit difference could change an alignment.
+constexpr int MAXLEN = 65535;
+__device__ float g_cbrt_lm15[MAXLEN + 1];   // cbrtf((float)L - 15.f)
+__device__ float g_pow03[MAXLEN + 1];       // powf((float)L, 0.3f)
+
+__device__ __forceinline__ int GetLnormI(int q, int r) { return min(q, r); }
+__device__ __forceinline__ float GetD82(int q, int r) {
+    float d8 = 1.5f * g_pow03[GetLnormI(q, r)] + 3.5f;
+    return d8 * d8;
+}
+__device__ __forceinline__ float GetD0(int q, int r) {
+    const int L = GetLnormI(q, r);
+    float d0 = 0.168f;
+    if ((float)L > 19.f) d0 = 1.24f * g_cbrt_lm15[L] - 1.8f;
+    return d0 + 0.8f;
+}
+__device__ __forceinline__ float GetD0finP(int q, int r) {
+    const int L = GetLnormI(q, r);
+    float d0 = 0.5f;
+    if ((float)L > 21.f) d0 = 1.24f * g_cbrt_lm15[L] - 1.8f;
+    if (d0 < 0.5f) d0 = 0.5f;
+    return d0;
+}
+__device__ __forceinline__ float GetD0finNA(int q, int r) {
+    float l = (float)GetLnormI(q, r), d0 = 0.3f;
+    if (30.f <= l) d0 = 0.6f * sqrtf(l - 0.5f) - 2.5f;
+    else if (23.f < l) d0 = 0.7f;
+    else if (19.f < l) d0 = 0.6f;
+    else if (15.f < l) d0 = 0.5f;
+    else if (11.f < l) d0 = 0.4f;
+    return fmaxf(0.3f, d0);
+}
+__device__ __forceinline__ float GetD0fin(int q, int r, int na) {
+    return na ? GetD0finNA(q, r) : GetD0finP(q, r);
+}
+__device__ __forceinline__ float GetD0s(float d0) { return fmaxf(fminf(d0, 8.f), 4.5f); }
 
 constexpr int RWAVES = 4;                 // waves per workgroup in the refinement kernels
 constexpr int RTHREADS = RWAVES * GTA_WAVE;
@@ -952,6 +996,26 @@ hipError_t launch_refine_gapless(int prec, const DevChunk& Q, const DevChunk& R,
     else hipLaunchKernelGGL(k_refine_gapless<0>, dim3(np), dim3(RTHREADS), 0, st, Q, R, S, G, dinv);
     return hipGetLastError();
 }
+namespace {
+struct LengthTables {
+    std::vector<float> cb, pw;
+    LengthTables() : cb(MAXLEN + 1), pw(MAXLEN + 1) {
+        for (int L = 0; L <= MAXLEN; L++) {
+            volatile float l = (float)L;      // evaluated at run time by the C library
+            cb[L] = cbrtf(l - 15.f);
+            pw[L] = powf(l, 0.3f);
+        }
+    }
+};
+}  // namespace
+
+hipError_t upload_length_tables() {
+    static const LengthTables t;              // built once (thread-safe static)
+    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(g_cbrt_lm15), t.cb.data(), t.cb.size() * sizeof(float));
+    if (e != hipSuccess) return e;
+    return hipMemcpyToSymbol(HIP_SYMBOL(g_pow03), t.pw.data(), t.pw.size() * sizeof(float));
+}
+
 hipError_t launch_dp(const DevChunk& Q, const DevChunk& R, const DevState& S, const DpLaunch& L,
                      hipStream_t st) {
     const int np = S.nq * S.nr;

@@ -32,6 +32,8 @@ hipError_t launch_findfrag(int prec, const DevChunk& Q, const DevChunk& R, const
                            const Geom& G, int dinv, hipStream_t st);
 hipError_t launch_refine_gapless(int prec, const DevChunk& Q, const DevChunk& R, const DevState& S,
                                  const Geom& G, int dinv, hipStream_t st);
+// per-length d0/d8 tables on the current device (once per device, before any launch)
+hipError_t upload_length_tables();
 hipError_t launch_dp(const DevChunk& Q, const DevChunk& R, const DevState& S, const DpLaunch& L,
                      hipStream_t st);
 hipError_t launch_refine_dp(int prec, int mode, const DevChunk& Q, const DevChunk& R,
