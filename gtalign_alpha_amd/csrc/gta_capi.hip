@@ -262,7 +262,7 @@ static Geom make_geom(const gta_ctx* c) {
 }
 
 // device buffers of the pair state for the resident chunks
-static int prepare_state(gta_ctx* c, DevState& S, bool& dp_in_lds, size_t& dp_lds, size_t& alnsize) {
+static int prepare_state(gta_ctx* c, DevState& S, size_t& alnsize) {
     const int nq = c->q.nstrs, nr = c->r.nstrs;
     const size_t np = (size_t)nq * nr;
     hipStream_t st = c->stream;
@@ -286,8 +286,6 @@ static int prepare_state(gta_ctx* c, DevState& S, bool& dp_in_lds, size_t& dp_ld
     // backtrace: per pair ceil(Lq/64) strips x ceil((Lr+63)/16) blocks x 64 words
     std::vector<unsigned long long> off(np);
     unsigned long long tot = 0;
-    int maxlr = 0;
-    for (int ri = 0; ri < nr; ri++) maxlr = std::max(maxlr, c->r.len[ri]);
     for (int qi = 0; qi < nq; qi++) {
         const unsigned long long ns = (unsigned long long)((c->q.len[qi] + 63) / 64);
         for (int ri = 0; ri < nr; ri++) {
@@ -298,9 +296,7 @@ static int prepare_state(gta_ctx* c, DevState& S, bool& dp_in_lds, size_t& dp_ld
     HIPCHK(c, c->s_btck.ensure(tot * 4));
     HIPCHK(c, c->s_btckoff.ensure(np * 8));
     HIPCHK(c, hipMemcpyAsync(c->s_btckoff.p, off.data(), np * 8, hipMemcpyHostToDevice, c->stream));
-    dp_lds = (size_t)maxlr * 20;
-    dp_in_lds = dp_lds <= std::min<size_t>(c->lds_max, 65536);
-    if (!dp_in_lds) HIPCHK(c, c->s_rowbuf.ensure(alnplane * 4));
+    HIPCHK(c, c->s_rowbuf.ensure(alnplane * 4));
     alnsize = gta_alignment_section_size(nq, c->q.nposs, nr, c->r.nposs);
     HIPCHK(c, c->s_alns.ensure(alnsize));
 
@@ -313,7 +309,7 @@ static int prepare_state(gta_ctx* c, DevState& S, bool& dp_in_lds, size_t& dp_ld
     S.out = c->s_out.as<float>();
     S.aln = c->s_aln.as<float>(); S.alnplane = alnplane;
     S.btck = c->s_btck.as<unsigned>(); S.btckoff = c->s_btckoff.as<unsigned long long>();
-    S.rowbuf = dp_in_lds ? nullptr : c->s_rowbuf.as<float>();
+    S.rowbuf = c->s_rowbuf.as<float>();
     S.cells = c->s_cells.as<unsigned long long>();
     S.algbytes = c->s_algb.as<unsigned long long>();
 
@@ -330,10 +326,9 @@ static int run_stage(gta_ctx* c, double* seconds, double* cells) {
     const int dinv = c->opts.symmetric;
 
     DevState S;
-    bool dp_in_lds = false;
-    size_t dp_lds = 0, alnsize = 0;
+    size_t alnsize = 0;
     {
-        int rc0 = prepare_state(c, S, dp_in_lds, dp_lds, alnsize);
+        int rc0 = prepare_state(c, S, alnsize);
         if (rc0 != GTA_OK) return rc0;
     }
     const DevChunk Q = c->q.view(), R = c->r.view();
@@ -370,7 +365,7 @@ static int run_stage(gta_ctx* c, double* seconds, double* cells) {
     int maxaln = std::min(c->q.len.front(), c->r.len.front());
     RefineLaunch RL{dinv, 0, 0, 0, 0.f, std::min(maxaln, 1536)};
     auto dp = [&](int mode, int gap0, float gapopen, int usess, int dpscan) -> hipError_t {
-        DpLaunch L{mode, gap0, gapopen, usess, usess ? 0.5f : 0.0f, dpscan, dp_in_lds ? dp_lds : 0};
+        DpLaunch L{mode, gap0, gapopen, usess, usess ? 0.5f : 0.0f, dpscan};
         return launch_dp(Q, R, S, L, st);
     };
     auto refine = [&](int mode, int epi, int dpi, int maxdpi, float prescore) -> hipError_t {
@@ -470,9 +465,8 @@ int gta_test_dp(gta_ctx* c, const float* tfms, int gap0, float gapopen, int uses
     if (!c->have_q || !c->have_r) return fail(c, GTA_ERR_STATE, "queries and references must be set first");
     HIPCHK(c, hipSetDevice(c->device));
     DevState S;
-    bool dp_in_lds = false;
-    size_t dp_lds = 0, alnsize = 0;
-    int rc = prepare_state(c, S, dp_in_lds, dp_lds, alnsize);
+    size_t alnsize = 0;
+    int rc = prepare_state(c, S, alnsize);
     if (rc != GTA_OK) return rc;
     const DevChunk Q = c->q.view(), R = c->r.view();
     const size_t np = (size_t)c->q.nstrs * c->r.nstrs;
@@ -481,7 +475,7 @@ int gta_test_dp(gta_ctx* c, const float* tfms, int gap0, float gapopen, int uses
     HIPCHK(c, launch_secstr(R, st));
     HIPCHK(c, launch_init(Q, R, S, st));
     HIPCHK(c, hipMemcpyAsync(S.tfm_grand, tfms, np * 12 * 4, hipMemcpyHostToDevice, st));
-    DpLaunch L{DPM_GRAND_L, gap0, gapopen, usess, usess ? 0.5f : 0.0f, dpscan, dp_in_lds ? dp_lds : 0};
+    DpLaunch L{DPM_GRAND_L, gap0, gapopen, usess, usess ? 0.5f : 0.0f, dpscan};
     HIPCHK(c, launch_dp(Q, R, S, L, st));
     HIPCHK(c, hipStreamSynchronize(st));
     std::vector<float> pl(S.alnplane);

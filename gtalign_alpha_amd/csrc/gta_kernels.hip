@@ -464,9 +464,27 @@ __device__ __forceinline__ unsigned read_code(const unsigned* bt, int nblk, int 
     return ((w0 >> b) & 1u) | (((w1 >> b) & 1u) << 1);
 }
 
-template <bool LDSROW>
+// whole-wave shift by one lane (DPP wave_shr:1): lane l receives lane l-1's
+// value, lane 0 receives `fill`
+__device__ __forceinline__ float wave_shr1(float v, float fill) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, fill),
+                                                                 __builtin_bit_cast(int, v), 0x138, 0xf, 0xf,
+                                                                 false));
+}
+__device__ __forceinline__ float readlane_f(float v, int l) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+}
+
+// One wave per pair.  The matrix is swept in strips of 64 query rows; lane l
+// owns row 64s+l and at step t computes column t-l, so a step advances one
+// anti-diagonal.  The reference residues flow through the wave one lane per
+// step (DPP shifts), lane 0 taking column t from a 64-column block that the
+// whole wave loads coalesced one block ahead; the row above a strip (lane
+// 63's values of the previous strip) is stored to and read back from a
+// per-pair row in HBM in the same 64-column blocks.  No LDS, no divergent
+// branches in the step loop.
+template <bool USESS>
 __global__ __launch_bounds__(GTA_WAVE) void k_dp(DevChunk Q, DevChunk R, DevState S, DpParams prm) {
-    extern __shared__ float4 lds_dp[];
     const int p = blockIdx.x;
     const int qi = pair_q(S, p), ri = pair_r(S, p);
     const int Lq = Q.len[qi], Lr = R.len[ri];
@@ -486,88 +504,104 @@ __global__ __launch_bounds__(GTA_WAVE) void k_dp(DevChunk Q, DevChunk R, DevStat
     float d0 = GetD0(Lq, Lr);
     if (prm.dpscan) d0 += 1.5f;
     const float d02 = d0 * d0;
-    const bool gap0 = prm.gap0 != 0, usess = prm.usess != 0;
+    const bool gap0 = prm.gap0 != 0;
     const float gapopen = prm.gapopen, ssw = prm.ssw;
 
-    // reference coordinates (+ss) and the strip-boundary row
-    float4* rc;
-    float* row;
-    float4* ldsrc = lds_dp;
-    float* ldsrow = (float*)(lds_dp + Lr);
-    if (LDSROW) {
-        rc = ldsrc;
-        row = ldsrow;
-        for (int j = lane; j < Lr; j += GTA_WAVE)
-            rc[j] = make_float4(R.x[rd + j], R.y[rd + j], R.z[rd + j], (float)R.ss[rd + j]);
-        __syncthreads();
-    } else {
-        rc = nullptr;
-        row = S.rowbuf + (size_t)qi * (size_t)R.nposs + (size_t)rd;
-    }
+    const float* Rx = R.x + rd;
+    const float* Ry = R.y + rd;
+    const float* Rz = R.z + rd;
+    const char* Rs = R.ss + rd;
+    float* row = S.rowbuf + (size_t)qi * (size_t)R.nposs + (size_t)rd;
     unsigned* bt = S.btck + S.btckoff[p];
     const int nstrips = (Lq + 63) >> 6;
     const int nsteps = Lr + 63;
-    const int nblk = (nsteps + 15) >> 4;
 
     for (int s = 0; s < nstrips; s++) {
         const int i = (s << 6) + lane;
         const bool rowok = i < Lq;
         float qx = 0.f, qy = 0.f, qz = 0.f;
-        int qss = 0;
+        float qss = 0.f;
         if (rowok) {
             qx = Q.x[qd + i]; qy = Q.y[qd + i]; qz = Q.z[qd + i];
             tfm_point(tfm, qx, qy, qz);
-            qss = Q.ss[qd + i];
+            if (USESS) qss = (float)Q.ss[qd + i];
         }
-        float val = 0.f, upprev = 0.f;
+        const bool qok = rowok && qx < DEFCOORD_QRY_cmp;
+        const bool haveup = s > 0;
+        const bool putrow = s + 1 < nstrips;
+        // 64-column blocks: lane k holds column c0+k (current and next block)
+        float cbx, cby, cbz, cbs = 0.f, cbu = 0.f, nbx, nby, nbz, nbs = 0.f, nbu = 0.f;
+        {
+            const int c = lane;
+            const bool ok = c < Lr;
+            cbx = ok ? Rx[c] : 0.f; cby = ok ? Ry[c] : 0.f; cbz = ok ? Rz[c] : 0.f;
+            if (USESS) cbs = ok ? (float)Rs[c] : 0.f;
+            if (haveup) cbu = ok ? row[c] : 0.f;
+            const int c2 = 64 + lane;
+            const bool ok2 = c2 < Lr;
+            nbx = ok2 ? Rx[c2] : 0.f; nby = ok2 ? Ry[c2] : 0.f; nbz = ok2 ? Rz[c2] : 0.f;
+            if (USESS) nbs = ok2 ? (float)Rs[c2] : 0.f;
+            if (haveup) nbu = ok2 ? row[c2] : 0.f;
+        }
+        float rx = 0.f, ry = 0.f, rz = 0.f, rs = 0.f;
+        float val = 0.f, upprev = 0.f, oblk = 0.f;
         unsigned word = 0;
         for (int t = 0; t < nsteps; t++) {
-            const int j = t - lane;
-            float recv = __shfl_up(val, 1, GTA_WAVE);
-            if (lane == 0) recv = (s > 0 && t < Lr) ? row[t] : 0.f;
-            const float up = recv, diag = upprev;
-            upprev = up;
-            unsigned code = 0;
-            if (j >= 0 && j < Lr && rowok) {
-                float rx, ry, rz;
-                int rss;
-                if (LDSROW) {
-                    const float4 r4 = rc[j];
-                    rx = r4.x; ry = r4.y; rz = r4.z; rss = (int)r4.w;
-                } else {
-                    rx = R.x[rd + j]; ry = R.y[rd + j]; rz = R.z[rd + j]; rss = R.ss[rd + j];
-                }
-                float v = 0.0f;
-                if (qx < DEFCOORD_QRY_cmp && DEFCOORD_RFN_cmp < rx) v = pair_score(d02, dist2(qx, qy, qz, rx, ry, rz));
-                if (usess) v = __fadd_rn(v, (qss == rss) ? ssw : 0.0f);
-                v = __fadd_rn(v, gap0 ? diag : fabsf(diag));
-                unsigned bk = BT_DIAG;
-                float w = val;
-                if (!gap0 && w < 0.0f) w = __fsub_rn(gapopen, w);
-                if (v < w) { v = w; bk = BT_LEFT; }
-                w = up;
-                if (!gap0 && w < 0.0f) w = __fsub_rn(gapopen, w);
-                if (v < w) { v = w; bk = BT_UP; }
-                val = (gap0 || bk != BT_DIAG) ? v : -v;
-                code = bk;
-            } else if (j < 0) {
-                val = 0.f;
+            const int k = t & 63;
+            if (k == 0 && t > 0) {
+                cbx = nbx; cby = nby; cbz = nbz; cbs = nbs; cbu = nbu;
+                const int c2 = t + 64 + lane;
+                const bool ok2 = c2 < Lr;
+                nbx = ok2 ? Rx[c2] : 0.f; nby = ok2 ? Ry[c2] : 0.f; nbz = ok2 ? Rz[c2] : 0.f;
+                if (USESS) nbs = ok2 ? (float)Rs[c2] : 0.f;
+                if (haveup) nbu = ok2 ? row[c2] : 0.f;
             }
-            if (lane == 63 && j >= 0 && j < Lr) row[j] = val;
+            rx = wave_shr1(rx, readlane_f(cbx, k));
+            ry = wave_shr1(ry, readlane_f(cby, k));
+            rz = wave_shr1(rz, readlane_f(cbz, k));
+            if (USESS) rs = wave_shr1(rs, readlane_f(cbs, k));
+            const float up = wave_shr1(val, haveup ? readlane_f(cbu, k) : 0.f);
+            const float diag = upprev;
+            upprev = up;
+            const int j = t - lane;
+            const bool valid = rowok && (unsigned)j < (unsigned)Lr;
+            float sc = 0.0f;
+            if (qok && DEFCOORD_RFN_cmp < rx) sc = pair_score(d02, dist2(qx, qy, qz, rx, ry, rz));
+            if (USESS) sc = __fadd_rn(sc, (qss == rs) ? ssw : 0.0f);
+            float v = __fadd_rn(sc, gap0 ? diag : fabsf(diag));
+            unsigned bk = BT_DIAG;
+            float w = val;
+            if (!gap0 && w < 0.0f) w = __fsub_rn(gapopen, w);
+            if (v < w) { v = w; bk = BT_LEFT; }
+            w = up;
+            if (!gap0 && w < 0.0f) w = __fsub_rn(gapopen, w);
+            if (v < w) { v = w; bk = BT_UP; }
+            const float nv = (gap0 || bk != BT_DIAG) ? v : -v;
+            val = valid ? nv : (j < 0 ? 0.0f : val);
+            const unsigned code = valid ? bk : (unsigned)BT_STOP;
+            // lane 63's value of column t-63 goes to the row of the next strip
+            const int jo = t - 63;
+            if (putrow && jo >= 0) {
+                const float o = readlane_f(val, 63);
+                oblk = (lane == (jo & 63)) ? o : oblk;
+                if ((jo & 63) == 63 || jo == Lr - 1) {
+                    if (lane <= (jo & 63)) row[(jo & ~63) + lane] = oblk;
+                }
+            }
+            // 2-bit codes of 16 steps -> one 64-word block (lane 4k+{0,1}: bit 0, {2,3}: bit 1)
             const unsigned long long b0 = __ballot(code & 1u);
             const unsigned long long b1 = __ballot(code & 2u);
-            const int k = t & 15;
-            if ((lane >> 2) == k) {
-                const int sel = lane & 3;
-                const unsigned long long m = (sel < 2) ? b0 : b1;
-                word = (sel & 1) ? (unsigned)(m >> 32) : (unsigned)m;
+            if ((lane >> 2) == (t & 15)) {
+                const unsigned long long m = (lane & 2) ? b1 : b0;
+                word = (lane & 1) ? (unsigned)(m >> 32) : (unsigned)m;
             }
-            if (k == 15 || t == nsteps - 1) {
-                bt[((size_t)(s * nblk + (t >> 4)) << 6) + lane] = word;
+            if ((t & 15) == 15 || t == nsteps - 1) {
+                bt[((size_t)((s * ((nsteps + 15) >> 4)) + (t >> 4)) << 6) + lane] = word;
                 word = 0;
             }
         }
-        if (!LDSROW) __threadfence_block();
+        // the next strip reads this strip's row back
+        __threadfence_block();
     }
     if (lane == 0) {
         const unsigned long long cells = (unsigned long long)Lq * (unsigned long long)Lr;
@@ -575,6 +609,7 @@ __global__ __launch_bounds__(GTA_WAVE) void k_dp(DevChunk Q, DevChunk R, DevStat
         atomicAdd(S.algbytes + KC_DP, cells / 4 + 12ull * (unsigned long long)(Lq + Lr));
     }
     __threadfence();
+    const int nblk = (nsteps + 15) >> 4;
 
     // ---- backtrace walk --------------------------------------------------------
     if (prm.mode == DPM_GRAND && (flags & CONV_SCOREDP)) {
@@ -1020,8 +1055,8 @@ hipError_t launch_dp(const DevChunk& Q, const DevChunk& R, const DevState& S, co
                      hipStream_t st) {
     const int np = S.nq * S.nr;
     DpParams prm{L.mode, L.gap0, L.gapopen, L.usess, L.ssw, L.dpscan};
-    if (L.lds_bytes > 0)
-        hipLaunchKernelGGL(k_dp<true>, dim3(np), dim3(GTA_WAVE), L.lds_bytes, st, Q, R, S, prm);
+    if (L.usess)
+        hipLaunchKernelGGL(k_dp<true>, dim3(np), dim3(GTA_WAVE), 0, st, Q, R, S, prm);
     else
         hipLaunchKernelGGL(k_dp<false>, dim3(np), dim3(GTA_WAVE), 0, st, Q, R, S, prm);
     return hipGetLastError();

@@ -16,7 +16,6 @@ struct DpLaunch {
     int usess;
     float ssw;
     int dpscan;
-    size_t lds_bytes;  // 0 = reference coordinates and boundary rows in HBM
 };
 
 struct RefineLaunch {

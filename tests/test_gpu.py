@@ -17,7 +17,7 @@ from gtalign_alpha_amd.capi import (default_options, PREC_FP32, PREC_MIXED, STAT
                                     ALN_RMSD, ALN_SCOREQ, ALN_SCORER, ALN_D0Q, ALN_D0R)
 from gtalign_alpha_amd.chunk import Chunk
 from oracle import oracle
-from tests.util import alignment_rows, matched_positions, random_rotation, tm_fp64
+from tests.util import alignment_rows, kabsch_svd, matched_positions, random_rotation, tm_fp64
 
 pytestmark = pytest.mark.gpu
 
@@ -34,12 +34,48 @@ def run_gpu(q, r, prec=PREC_FP32, **kw):
         return b.ProcessBlock()
 
 
-def assert_same(expect, got, tm_tol=1e-4, rmsd_tol=1e-3):
+def e0_noise(q, r):
+    """Per pair: the rounding floor (A^2) of the reference's FP32 RMSD, which
+    takes the RMSD from Kabsch's E0 identity: E0 is a difference of sums of
+    squared coordinates, so near RMSD 0 its value is FP32 noise of order
+    eps * mean(|x|^2) that no other summation order reproduces."""
+    eps = float(np.finfo(np.float32).eps)
+    mq = np.array([(q.coords(i).astype(np.float64) ** 2).sum(1).mean() for i in range(q.nstrs)])
+    mr = np.array([(r.coords(i).astype(np.float64) ** 2).sum(1).mean() for i in range(r.nstrs)])
+    return 64.0 * eps * (mq[:, None] + mr[None, :])
+
+
+def assert_same(expect, got, q, r, tm_tol=1e-4, rmsd_tol=1e-3):
     assert np.array_equal(expect.alns, got.alns), int((expect.alns != got.alns).sum())
     for f in INT_FIELDS:
         assert np.array_equal(expect.alndata[..., f], got.alndata[..., f]), f
     assert np.abs(expect.alndata[..., 9:11] - got.alndata[..., 9:11]).max() < tm_tol
-    assert np.abs(expect.alndata[..., 8] - got.alndata[..., 8]).max() < rmsd_tol
+    a, b = expect.alndata[..., 8].astype(np.float64), got.alndata[..., 8].astype(np.float64)
+    # RMSD within rmsd_tol, or (near 0) within the FP32 E0 noise floor
+    ok = (np.abs(a - b) < rmsd_tol) | (np.abs(a * a - b * b) < e0_noise(q, r))
+    assert ok.all(), (np.abs(a - b).max(), np.argwhere(~ok)[:5])
+
+
+def assert_close_mixed(expect, got, q, r):
+    """The mixed form solves the 3x3 superposition in FP64, so an alignment may
+    differ from the FP32 oracle where seeds or slots are within rounding of each
+    other.  Bar: at most 2% of pairs differ (at least 1 allowed), TM-scores of
+    identically aligned pairs within 1e-4, of the rest within 0.05."""
+    ea, ga = expect.alndata[..., 8] < 9999, got.alndata[..., 8] < 9999
+    ndiff = 0
+    for qi in range(q.nstrs):
+        for ri in range(r.nstrs):
+            dtm = np.abs(expect.alndata[qi, ri, 9:11] - got.alndata[qi, ri, 9:11]).max()
+            same = ea[qi, ri] == ga[qi, ri]
+            if same and ea[qi, ri]:
+                same = (alignment_rows(expect.alns, q, r, qi, ri) ==
+                        alignment_rows(got.alns, q, r, qi, ri))
+            if same:
+                assert dtm < 1e-4, (qi, ri, dtm)
+            else:
+                ndiff += 1
+                assert dtm < 0.05, (qi, ri, dtm)
+    assert ndiff <= max(1, 0.02 * q.nstrs * r.nstrs), ndiff
 
 
 def _cases():
@@ -62,7 +98,10 @@ def test_stage_matches_oracle(case, prec):
     _, q, r = case
     o = oracle.process_block(q, r)
     g = run_gpu(q, r, prec)
-    assert_same(o, g)
+    if prec == PREC_FP32:
+        assert_same(o, g, q, r)
+    else:
+        assert_close_mixed(o, g, q, r)
     assert (g.alndata[..., ALN_RMSD] < 9999).sum() >= 3
 
 
@@ -84,7 +123,8 @@ def test_stage_matches_reference_golden(path):
 @pytest.mark.parametrize("prec", [PREC_FP32, PREC_MIXED], ids=["fp32", "mixed"])
 def test_reported_tm_equals_fp64_recomputation(prec):
     """Size-independent check: the reported TM-scores equal an FP64 recomputation
-    from the output alignment and transform (d0 of each normalising length)."""
+    from the output alignment and transform (d0 of each normalising length), and
+    the RMSD an FP64 superposition of the aligned pairs."""
     qc, rc, _ = synth.config(1, scale=0.004)
     q = qc.subset([1, 8])
     g = run_gpu(q, rc, prec)
@@ -100,6 +140,12 @@ def test_reported_tm_equals_fp64_recomputation(prec):
             tr = tm_fp64(q, rc, qi, ri, pairs, g.tfms[qi, ri], int(rc.len[ri]))
             assert abs(tq - a[ALN_SCOREQ]) < 1e-4, (qi, ri)
             assert abs(tr - a[ALN_SCORER]) < 1e-4, (qi, ri)
+            # RMSD of the alignment after its optimal superposition (FP64 SVD);
+            # the FP32 form keeps the reference's E0 cancellation (<~0.05 A)
+            x, y = q.coords(qi)[pairs[:, 0]], rc.coords(ri)[pairs[:, 1]]
+            R, t = kabsch_svd(x, y)
+            rms = float(np.sqrt(((x @ R.T + t - y) ** 2).sum(1).mean()))
+            assert abs(rms - a[ALN_RMSD]) < (1e-3 if prec == PREC_MIXED else 0.1), (qi, ri, rms)
             n += 1
     assert n >= 5
 
@@ -112,8 +158,8 @@ def test_short_chains_and_copies():
     q = Chunk.from_structures(qs, [synth.residues(rng, len(s)) for s in qs])
     r = Chunk.from_structures(rs, [synth.residues(rng, len(s)) for s in rs])
     o = oracle.process_block(q, r)
-    for prec in (PREC_FP32, PREC_MIXED):
-        assert_same(o, run_gpu(q, r, prec))
+    assert_same(o, run_gpu(q, r, PREC_FP32), q, r)
+    assert_close_mixed(o, run_gpu(q, r, PREC_MIXED), q, r)
 
 
 def test_long_reference_row_buffer_path():
@@ -128,7 +174,7 @@ def test_long_reference_row_buffer_path():
     r = Chunk.from_structures(rs, [synth.residues(rng, len(s)) for s in rs])
     o = oracle.process_block(q, r)
     g = run_gpu(q, r)
-    assert_same(o, g)
+    assert_same(o, g, q, r)
     assert g.alndata[0, 0, ALN_SCOREQ] > 0.5
 
 
@@ -137,7 +183,7 @@ def test_options_depth_refinement_referenced():
     for kw in (dict(refinement=0), dict(refinement=3), dict(depth=0), dict(depth=1),
                dict(referenced=1), dict(nodeletions=1), dict(prescore=0.0, scorethld=0.0)):
         o = oracle.process_block(qc, rc, default_options(**kw))
-        assert_same(o, run_gpu(qc, rc, **kw))
+        assert_same(o, run_gpu(qc, rc, **kw), qc, rc)
 
 
 def test_context_reuse_with_smaller_chunk():
@@ -153,8 +199,8 @@ def test_context_reuse_with_smaller_chunk():
         g1 = b.ProcessBlock()
         b.TransferCPMDataToDevice(r2)
         g2 = b.ProcessBlock()
-    assert_same(run_gpu(q, rc), g1, tm_tol=1e-7, rmsd_tol=1e-7)
-    assert_same(oracle.process_block(q, r2), g2)
+    assert_same(run_gpu(q, rc), g1, q, rc, tm_tol=1e-7, rmsd_tol=1e-7)
+    assert_same(oracle.process_block(q, r2), g2, q, r2)
 
 
 # ---- the DP kernel alone (bit-exact) ---------------------------------------------------
@@ -271,7 +317,7 @@ def test_error_codes():
         assert e.value.code == _code("GTA_ERR_STATE")
         b.TransferQueryPMDataToDevice(qc)
         g = b.ProcessBlock()
-    assert_same(oracle.process_block(qc, rc), g)
+    assert_same(oracle.process_block(qc, rc), g, qc, rc)
 
 
 def test_structure_length_limit():
