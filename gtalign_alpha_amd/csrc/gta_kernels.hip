@@ -31,6 +31,7 @@
 
 #include "gta_device.h"
 #include "gta_kabsch.h"
+#include "gta_slots.h"
 
 namespace gta {
 
@@ -179,63 +180,80 @@ __global__ void k_init(DevChunk Q, DevChunk R, DevState S) {
 }
 
 // ============================================================================
-// gapless passes over structure coordinates (stage 1)
+// gapless passes over structure coordinates (stage 1), 16-lane slot groups
 struct Coords {
     const float* x;
     const float* y;
     const float* z;
 };
 
-template <int P>
-__device__ void cc_gapless(const Coords& q, const Coords& r, int qpos, int rpos, int n,
-                           typename Prec<P>::T* cc) {
-    typename Prec<P>::Acc acc;
-    acc.clear();
-    for (int k = lane_id(); k < n; k += GTA_WAVE) {
-        int a = qpos + k, b = rpos + k;
-        acc.add(q.x[a], q.y[a], q.z[a], r.x[b], r.y[b], r.z[b]);
-    }
-    acc.reduce(cc);
+__device__ __forceinline__ void copy12(float* d, const float* s) {
+#pragma unroll
+    for (int i = 0; i < 12; i++) d[i] = s[i];
 }
-// pairs with d2 (under tprev) <= thr; returns the count
+
+// best slot of a pair: highest score, lowest slot index on ties
+struct SlotBest {
+    float best;
+    int slot;
+    float tfm[12];
+    float gbest;
+    int ndx, pos;
+    __device__ __forceinline__ void clear() { best = 0.f; slot = 0x7fffffff; gbest = 0.f; ndx = 0; pos = 0; }
+    __device__ __forceinline__ bool beats(float b, int s) const { return b > best || (b == best && s < slot); }
+};
+constexpr int NGROUPS = RWAVES * NGW;   // slot groups per workgroup
+
+// the workgroup's best over its groups (every thread returns it)
+__device__ void group_best(const SlotBest& mine, SlotBest* lds, SlotBest& out) {
+    const int g = threadIdx.x / GL;
+    if (glane() == 0) lds[g] = mine;
+    __syncthreads();
+    out = lds[0];
+    for (int i = 1; i < NGROUPS; i++)
+        if (out.beats(lds[i].best, lds[i].slot)) out = lds[i];
+}
+
+// covariance over n diagonal positions from (qp, rp); with thr, only pairs
+// within thr (squared distance under tfm); returns the group's count
 template <int P>
-__device__ int cc_gapless_ext(const Coords& q, const Coords& r, int qpos, int rpos, int n,
-                              const float* tprev, float thr, typename Prec<P>::T* cc) {
-    typename Prec<P>::Acc acc;
+__device__ __forceinline__ int gcc_gapless(const Coords& q, const Coords& r, int qp, int rp, int n,
+                                           const float* tfm, bool usethr, float thr,
+                                           typename GPrec<P>::T* cc) {
+    typename GPrec<P>::Acc acc;
     acc.clear();
     int cnt = 0;
-    for (int k = lane_id(); k < n; k += GTA_WAVE) {
-        int a = qpos + k, b = rpos + k;
-        float qx = q.x[a], qy = q.y[a], qz = q.z[a], rx = r.x[b], ry = r.y[b], rz = r.z[b];
-        float d = tfm_dist2(tprev, qx, qy, qz, rx, ry, rz);
-        if (thr < d) continue;
+    for (int k = glane(); k < n; k += GL) {
+        const int a = qp + k, b = rp + k;
+        const float qx = q.x[a], qy = q.y[a], qz = q.z[a], rx = r.x[b], ry = r.y[b], rz = r.z[b];
+        if (usethr && thr < tfm_dist2(tfm, qx, qy, qz, rx, ry, rz)) continue;
         acc.add(qx, qy, qz, rx, ry, rz);
         cnt++;
     }
     acc.reduce(cc);
-    return wave_sum_i(cnt);
+    return gsum(cnt);
 }
 
-// seed-diagonal scores (CalcScoresUnrl_Complete, MpStage1.h:700-790):
-// all pairs scored; next threshold from the 3rd smallest distinct distance
+// seed-diagonal scores (CalcScoresUnrl_Complete, MpStage1.h:700-790): all
+// pairs scored; next threshold from the 3rd smallest distinct distance
 template <int P>
-__device__ float score_gapless_seed(const Coords& q, const Coords& r, int qpos, int rpos, int n,
-                                    const float* tfm, float d0, float d02, bool calc2, bool want_thr,
-                                    float* thr) {
-    ScoreAcc<P> sc;
+__device__ __forceinline__ float gscore_seed(const Coords& q, const Coords& r, int qp, int rp, int n,
+                                             const float* tfm, float d0, float d02, bool calc2, bool want_thr,
+                                             float* thr) {
+    GScore<P> sc;
     sc.clear();
     Min3d m3;
     m3.init();
-    for (int k = lane_id(); k < n; k += GTA_WAVE) {
-        int a = qpos + k, b = rpos + k;
-        float d = tfm_dist2(tfm, q.x[a], q.y[a], q.z[a], r.x[b], r.y[b], r.z[b]);
+    for (int k = glane(); k < n; k += GL) {
+        const int a = qp + k, b = rp + k;
+        const float d = tfm_dist2(tfm, q.x[a], q.y[a], q.z[a], r.x[b], r.y[b], r.z[b]);
         sc.add(pair_score(d02, d));
-        if (want_thr) m3.put(d);
+        m3.put(d);
     }
-    float s = sc.reduce();
+    const float s = sc.reduce();
     if (want_thr) {
-        float min3 = m3.reduce_third();
-        float d0s = GetD0s(d0);
+        float min3 = gthird(m3);
+        const float d0s = GetD0s(d0);
         float d02s = d0s * d0s;
         if (calc2) d02s += 1.0f;
         if (CP_LARGEDST_cmp < min3 || min3 < d02s || n <= 3) min3 = d02s;
@@ -249,63 +267,41 @@ __device__ float score_gapless_seed(const Coords& q, const Coords& r, int qpos, 
 // pairs within d8 (when chk) scored; threshold from the multiset 3rd smallest
 // distance, rounded up to a 0.5 A step above d0s-1
 __device__ __forceinline__ float refined_threshold(float min3, float d0, int n) {
-    float d0s = GetD0s(d0) - 1.0f;
-    float d02s = d0s * d0s;
+    const float d0s = GetD0s(d0) - 1.0f;
+    const float d02s = d0s * d0s;
     if (CP_LARGEDST_cmp < min3 || min3 < d02s || n <= 3) return d02s;
-    float t = d0s + ceilf((sqrtf(min3) - d0s) * 2.0f) * 0.5f;
+    const float t = d0s + ceilf((sqrtf(min3) - d0s) * 2.0f) * 0.5f;
     return t * t;
 }
 
 template <int P>
-__device__ float score_gapless_refined(const Coords& q, const Coords& r, int qpos, int rpos, int n,
-                                       const float* tfm, float d0, float d02, float d82, bool want_thr,
-                                       float* thr) {
-    ScoreAcc<P> sc;
+__device__ __forceinline__ float gscore_gapless(const Coords& q, const Coords& r, int qp, int rp, int n,
+                                                const float* tfm, float d0, float d02, float d82,
+                                                bool want_thr, float* thr) {
+    GScore<P> sc;
     sc.clear();
     Min3 m3;
     m3.init();
-    for (int k = lane_id(); k < n; k += GTA_WAVE) {
-        int a = qpos + k, b = rpos + k;
-        float d = tfm_dist2(tfm, q.x[a], q.y[a], q.z[a], r.x[b], r.y[b], r.z[b]);
+    for (int k = glane(); k < n; k += GL) {
+        const int a = qp + k, b = rp + k;
+        const float d = tfm_dist2(tfm, q.x[a], q.y[a], q.z[a], r.x[b], r.y[b], r.z[b]);
         if (d <= d82) sc.add(pair_score(d02, d));
-        if (want_thr) m3.put(d);
+        m3.put(d);
     }
-    float s = sc.reduce();
-    if (want_thr) *thr = refined_threshold(m3.reduce_third(), d0, n);
+    const float s = sc.reduce();
+    if (want_thr) *thr = refined_threshold(gthird(m3), d0, n);
     return s;
 }
 
-__device__ __forceinline__ void copy12(float* d, const float* s) {
-#pragma unroll
-    for (int i = 0; i < 12; i++) d[i] = s[i];
-}
-
-// cross-wave argmax of the slot bests: highest score, lowest slot on ties
-struct SlotBest {
-    float best;
-    int slot;
-    float tfm[12];
-    float gbest;
-    int ndx, pos;
-};
-
-__device__ void block_best(SlotBest& mine, SlotBest* lds, SlotBest& out) {
-    const int w = threadIdx.x / GTA_WAVE;
-    if (lane_id() == 0) lds[w] = mine;
-    __syncthreads();
-    out = lds[0];
-    for (int i = 1; i < RWAVES; i++) {
-        const SlotBest& c = lds[i];
-        if (c.best > out.best || (c.best == out.best && c.slot < out.slot)) out = c;
-    }
-}
-
 // ============================================================================
-// stage 1: best gapless diagonal (seed) per pair
+// stage 1: best gapless diagonal (seed) per pair (FindGaplessAlignedFragment,
+// MpStage1.cpp:267-400).  Slots are seed diagonals; each runs up to three
+// rounds of covariance -> Kabsch -> score.
 template <int P>
 __global__ __launch_bounds__(RTHREADS) void k_findfrag(DevChunk Q, DevChunk R, DevState S, Geom G,
                                                         int dinv_on) {
-    __shared__ SlotBest lds[RWAVES];
+    __shared__ SlotBest lds[NGROUPS];
+    __shared__ int ctr;
     const int p = blockIdx.x;
     const int qi = pair_q(S, p), ri = pair_r(S, p);
     const int qlen = Q.len[qi], rlen = R.len[ri];
@@ -314,45 +310,71 @@ __global__ __launch_bounds__(RTHREADS) void k_findfrag(DevChunk Q, DevChunk R, D
         if (threadIdx.x == 0) S.best0[p] = 0.f;
         return;
     }
-    typedef typename Prec<P>::T T;
+    typedef typename GPrec<P>::T T;
     const Coords q{Q.x + qd, Q.y + qd, Q.z + qd}, r{R.x + rd, R.y + rd, R.z + rd};
     const bool dinv = dinv_on && (qlen < rlen);
     const float d0 = GetD0(qlen, rlen), d02 = d0 * d0;
     const int prv = max(min(qlen, rlen) >> 1, 5);
+    if (threadIdx.x == 0) ctr = 0;
+    __syncthreads();
+
     SlotBest mine;
-    mine.best = 0.f; mine.slot = 0x7fffffff; mine.ndx = 0; mine.pos = 0;
+    mine.clear();
     unsigned long long ab = 0;
-    for (int si = threadIdx.x / GTA_WAVE; si < G.nseeds; si += RWAVES) {
-        const int abspos = G.n1 + si * G.stepinit;
-        const int qpos = max(0, abspos), rpos = max(-abspos, 0);
-        if (qlen <= qpos || rlen <= rpos) continue;
-        if (qlen - qpos < prv || rlen - rpos < prv) continue;
-        const int n = min(qlen - qpos, rlen - rpos);
-        T cc[15];
-        float tfm[12], thr = CP_LARGEDST, best = 0.f;
-        for (int round = 0; round < 3; round++) {
-            int cnt;
-            if (round == 0) {
-                cc_gapless<P>(q, r, qpos, rpos, n, cc);
-                cnt = n;
-                ab += AB_CC * n;
-            } else {
-                float tprev[12];
-                copy12(tprev, tfm);
-                cnt = cc_gapless_ext<P>(q, r, qpos, rpos, n, tprev, thr, cc);
-                ab += AB_EXT * n;
-                if (cnt < 1 || cnt == n) break;
+    bool need = true, live = false;
+    int si = 0, qpos = 0, rpos = 0, n = 0, round = 0;
+    float tfm[12], thr = CP_LARGEDST, best = 0.f;
+    for (;;) {
+        // groups without a slot take the next valid seed diagonal
+        for (;;) {
+            const int got = fetch_slot(need, &ctr);
+            if (need) {
+                if (got >= G.nseeds) {
+                    need = false;
+                } else {
+                    si = got;
+                    const int abspos = G.n1 + si * G.stepinit;
+                    qpos = max(0, abspos);
+                    rpos = max(-abspos, 0);
+                    if (!(qlen <= qpos || rlen <= rpos) && !(qlen - qpos < prv || rlen - rpos < prv)) {
+                        n = min(qlen - qpos, rlen - rpos);
+                        round = 0;
+                        thr = CP_LARGEDST;
+                        best = 0.f;
+                        need = false;
+                        live = true;
+                    }
+                }
             }
-            ab += AB_TM * n + AB_SLOT;
-            kabsch<T>(cc, (T)cnt, dinv, tfm);
-            float sc = score_gapless_seed<P>(q, r, qpos, rpos, n, tfm, d0, d02, round > 0, round < 2, &thr);
-            if (best < sc) best = sc;
+            if (!__any(need)) break;
         }
-        if (best > mine.best) { mine.best = best; mine.slot = si; mine.ndx = qpos; mine.pos = rpos; }
+        if (!__any(live)) break;
+        if (live) {
+            T cc[15];
+            const int cnt = gcc_gapless<P>(q, r, qpos, rpos, n, tfm, round > 0, thr, cc);
+            ab += (round > 0 ? AB_EXT : AB_CC) * (unsigned long long)n;
+            bool done = false;
+            if (round > 0 && (cnt < 1 || cnt == n)) {
+                done = true;
+            } else {
+                kabsch<T>(cc, (T)cnt, dinv, tfm);
+                const float sc = gscore_seed<P>(q, r, qpos, rpos, n, tfm, d0, d02, round > 0, round < 2, &thr);
+                ab += AB_TM * (unsigned long long)n + AB_SLOT;
+                if (best < sc) best = sc;
+                if (++round == 3) done = true;
+            }
+            if (done) {
+                if (best > 0.f && mine.beats(best, si)) {
+                    mine.best = best; mine.slot = si; mine.ndx = qpos; mine.pos = rpos;
+                }
+                live = false;
+                need = true;
+            }
+        }
     }
-    if (lane_id() == 0 && ab) atomicAdd(S.algbytes + KC_FINDFRAG, ab);
+    if (glane() == 0 && ab) atomicAdd(S.algbytes + KC_FINDFRAG, ab);
     SlotBest b;
-    block_best(mine, lds, b);
+    group_best(mine, lds, b);
     if (threadIdx.x == 0) {
         S.best0[p] = b.best;
         if (b.best > 0.f) { S.qrypos[p] = b.ndx; S.rfnpos[p] = b.pos; }
@@ -361,10 +383,13 @@ __global__ __launch_bounds__(RTHREADS) void k_findfrag(DevChunk Q, DevChunk R, D
 
 // ============================================================================
 // stage 1: refinement of fragments of the best gapless alignment
+// (RefineFragInitKernel, MpStage1.cpp:470-640).  Slots are (fragment position,
+// fragment length); each iterates to convergence.
 template <int P>
 __global__ __launch_bounds__(RTHREADS) void k_refine_gapless(DevChunk Q, DevChunk R, DevState S, Geom G,
                                                               int dinv_on) {
-    __shared__ SlotBest lds[RWAVES];
+    __shared__ SlotBest lds[NGROUPS];
+    __shared__ int ctr;
     const int p = blockIdx.x;
     const int qi = pair_q(S, p), ri = pair_r(S, p);
     const int qlen = Q.len[qi], rlen = R.len[ri];
@@ -373,57 +398,88 @@ __global__ __launch_bounds__(RTHREADS) void k_refine_gapless(DevChunk Q, DevChun
         if (threadIdx.x == 0) S.best0[p] = 0.f;
         return;
     }
-    typedef typename Prec<P>::T T;
+    typedef typename GPrec<P>::T T;
     const Coords q{Q.x + qd, Q.y + qd, Q.z + qd}, r{R.x + rd, R.y + rd, R.z + rd};
     const bool dinv = dinv_on && (qlen < rlen);
     const int qpos = S.qrypos[p], rpos = S.rfnpos[p];
     const float d0 = GetD0(qlen, rlen), d02 = d0 * d0, d82 = GetD82(qlen, rlen);
     const float d0s1 = GetD0s(d0) + 1.0f, d02s1 = d0s1 * d0s1;
+    const bool anyslot = !(qlen <= qpos || rlen <= rpos);
+    const int nall = anyslot ? min(qlen - qpos, rlen - rpos) : 0;
+    if (threadIdx.x == 0) ctr = 0;
+    __syncthreads();
+
     SlotBest mine;
-    mine.best = 0.f; mine.slot = 0x7fffffff;
+    mine.clear();
     unsigned long long ab = 0;
-    if (!(qlen <= qpos || rlen <= rpos)) {
-        const int nall = min(qlen - qpos, rlen - rpos);
-        for (int si = threadIdx.x / GTA_WAVE; si < G.nlocsteps; si += RWAVES) {
-            const int sfragfct = si / NMAXSUBFRAGS, sfragndx = si - sfragfct * NMAXSUBFRAGS;
-            const int sfragpos = sfragfct * SFRAGSTEP;
-            const int fl = GetFragLength(qlen, rlen, qpos, rpos, sfragndx);
-            if (fl < 1) continue;
-            if (qlen + SFRAGSTEP <= qpos + sfragpos + fl || rlen + SFRAGSTEP <= rpos + sfragpos + fl) continue;
-            T cc[15], last[15], nlast = 0;
-            T ncc;
-            {
-                int ql = qlen, rl = rlen, qp = qpos + sfragpos, rp = rpos + sfragpos;
-                UpdateLengths(ql, rl, qp, rp, fl);
-                cc_gapless<P>(q, r, qp, rp, fl, cc);
-                ncc = (T)fl;
-                ab += AB_CC * fl;
-            }
-            float tfm[12], tfmBest[12], thr = CP_LARGEDST, best = 0.f;
-            for (int cit = 0; cit < NMAXCONVIT + 2; cit++) {
-                if (0 < cit) {
-                    float tprev[12];
-                    copy12(tprev, tfm);
-                    int cnt = cc_gapless_ext<P>(q, r, qpos, rpos, nall, tprev, (cit < 2) ? thr : d02s1, cc);
-                    ncc = (T)cnt;
-                    ab += AB_EXT * nall;
-                    if (cc_converged<T>(cc, ncc, last, nlast)) break;
+    bool need = anyslot, live = false;
+    int si = 0, cit = 0, fqp = 0, frp = 0, fl = 0;
+    float tfm[12], tfmBest[12], thr = CP_LARGEDST, best = 0.f;
+    T last[15], nlast = 0;
+    for (;;) {
+        for (;;) {
+            const int got = fetch_slot(need, &ctr);
+            if (need) {
+                if (got >= G.nlocsteps) {
+                    need = false;
+                } else {
+                    si = got;
+                    const int sfragfct = si / NMAXSUBFRAGS, sfragndx = si - sfragfct * NMAXSUBFRAGS;
+                    const int sfragpos = sfragfct * SFRAGSTEP;
+                    fl = GetFragLength(qlen, rlen, qpos, rpos, sfragndx);
+                    if (fl >= 1 && !(qlen + SFRAGSTEP <= qpos + sfragpos + fl ||
+                                     rlen + SFRAGSTEP <= rpos + sfragpos + fl)) {
+                        int ql = qlen, rl = rlen;
+                        fqp = qpos + sfragpos;
+                        frp = rpos + sfragpos;
+                        UpdateLengths(ql, rl, fqp, frp, fl);
+                        cit = 0;
+                        thr = CP_LARGEDST;
+                        best = 0.f;
+                        nlast = 0;
+                        need = false;
+                        live = true;
+                    }
                 }
-                if (ncc < (T)1) break;
+            }
+            if (!__any(need)) break;
+        }
+        if (!__any(live)) break;
+        if (live) {
+            T cc[15];
+            T ncc;
+            bool done = false;
+            if (cit == 0) {
+                ncc = (T)gcc_gapless<P>(q, r, fqp, frp, fl, tfm, false, 0.f, cc);
+                ab += AB_CC * (unsigned long long)fl;
+            } else {
+                ncc = (T)gcc_gapless<P>(q, r, qpos, rpos, nall, tfm, true, (cit < 2) ? thr : d02s1, cc);
+                ab += AB_EXT * (unsigned long long)nall;
+                if (cc_converged<T>(cc, ncc, last, nlast)) done = true;
+            }
+            if (!done && ncc < (T)1) done = true;
+            if (!done) {
 #pragma unroll
                 for (int f = 0; f < 15; f++) last[f] = cc[f];
                 nlast = ncc;
                 kabsch<T>(cc, ncc, dinv, tfm);
-                float sc = score_gapless_refined<P>(q, r, qpos, rpos, nall, tfm, d0, d02, d82, cit < 1, &thr);
-                ab += AB_TM * nall + AB_SLOT;
+                const float sc = gscore_gapless<P>(q, r, qpos, rpos, nall, tfm, d0, d02, d82, cit < 1, &thr);
+                ab += AB_TM * (unsigned long long)nall + AB_SLOT;
                 if (best < sc) { best = sc; copy12(tfmBest, tfm); }
+                if (++cit == NMAXCONVIT + 2) done = true;
             }
-            if (best > mine.best) { mine.best = best; mine.slot = si; copy12(mine.tfm, tfmBest); }
+            if (done) {
+                if (best > 0.f && mine.beats(best, si)) {
+                    mine.best = best; mine.slot = si; copy12(mine.tfm, tfmBest);
+                }
+                live = false;
+                need = true;
+            }
         }
     }
-    if (lane_id() == 0 && ab) atomicAdd(S.algbytes + KC_REFINE_GAPLESS, ab);
+    if (glane() == 0 && ab) atomicAdd(S.algbytes + KC_REFINE_GAPLESS, ab);
     SlotBest b;
-    block_best(mine, lds, b);
+    group_best(mine, lds, b);
     if (threadIdx.x == 0) {
         const bool have = b.best > 0.f;
         S.best0[p] = have ? b.best : 0.f;
@@ -691,16 +747,18 @@ struct Aln {
 };
 
 // forward alignment position f is stored at L-1-f
+// covariance of the fragment [fragpos, fragpos+fl) of the alignment (with the
+// sums of squares when sq is given, for the RMSD)
 template <int P>
-__device__ void cc_dp_frag(const Aln& a, int L, int fragpos, int fl, typename Prec<P>::T* cc,
-                           typename Prec<P>::T* sq) {
+__device__ __forceinline__ void gcc_dp_frag(const Aln& a, int L, int fragpos, int fl,
+                                            typename GPrec<P>::T* cc, typename GPrec<P>::T* sq) {
     int ql = L, rl = L, qp = fragpos, rp = fragpos;
     UpdateLengths(ql, rl, qp, rp, fl);
-    typename Prec<P>::Acc acc;
+    typedef typename GPrec<P>::T T;
+    typename GPrec<P>::Acc acc;
     acc.clear();
-    typedef typename Prec<P>::T T;
     T s2[6] = {0, 0, 0, 0, 0, 0};
-    for (int k = lane_id(); k < fl; k += GTA_WAVE) {
+    for (int k = glane(); k < fl; k += GL) {
         const int idx = L - 1 - (rp + k);
         const float qx = a.qx[idx], qy = a.qy[idx], qz = a.qz[idx], rx = a.rx[idx], ry = a.ry[idx], rz = a.rz[idx];
         acc.add(qx, qy, qz, rx, ry, rz);
@@ -711,86 +769,58 @@ __device__ void cc_dp_frag(const Aln& a, int L, int fragpos, int fl, typename Pr
     }
     acc.reduce(cc);
     if (sq) {
-        for (int i = 0; i < 6; i++) sq[i] = wave_sum(s2[i]);
+#pragma unroll
+        for (int i = 0; i < 6; i++) sq[i] = gsum(s2[i]);
     }
 }
+// covariance of the aligned pairs within thr under tprev; returns the count
 template <int P>
-__device__ int cc_dp_ext(const Aln& a, int L, const float* tprev, float thr, typename Prec<P>::T* cc) {
-    typename Prec<P>::Acc acc;
+__device__ __forceinline__ int gcc_dp_ext(const Aln& a, int L, const float* tprev, float thr,
+                                          typename GPrec<P>::T* cc) {
+    typename GPrec<P>::Acc acc;
     acc.clear();
     int cnt = 0;
-    for (int f = lane_id(); f < L; f += GTA_WAVE) {
+    for (int f = glane(); f < L; f += GL) {
         const int idx = L - 1 - f;
         const float qx = a.qx[idx], qy = a.qy[idx], qz = a.qz[idx], rx = a.rx[idx], ry = a.ry[idx], rz = a.rz[idx];
-        const float d = tfm_dist2(tprev, qx, qy, qz, rx, ry, rz);
-        if (thr < d) continue;
+        if (thr < tfm_dist2(tprev, qx, qy, qz, rx, ry, rz)) continue;
         acc.add(qx, qy, qz, rx, ry, rz);
         cnt++;
     }
     acc.reduce(cc);
-    return wave_sum_i(cnt);
+    return gsum(cnt);
 }
 template <int P>
-__device__ float score_dp(const Aln& a, int L, const float* tfm, float d0, float d02, float d82, bool chk,
-                          bool want_thr, float* thr) {
-    ScoreAcc<P> sc;
+__device__ __forceinline__ float gscore_dp(const Aln& a, int L, const float* tfm, float d0, float d02, float d82,
+                                           bool chk, bool want_thr, float* thr) {
+    GScore<P> sc;
     sc.clear();
     Min3 m3;
     m3.init();
-    for (int f = lane_id(); f < L; f += GTA_WAVE) {
+    for (int f = glane(); f < L; f += GL) {
         const int idx = L - 1 - f;
         const float d = tfm_dist2(tfm, a.qx[idx], a.qy[idx], a.qz[idx], a.rx[idx], a.ry[idx], a.rz[idx]);
         if (!chk || d <= d82) sc.add(pair_score(d02, d));
-        if (want_thr) m3.put(d);
+        m3.put(d);
     }
-    float s = sc.reduce();
-    if (want_thr) *thr = refined_threshold(m3.reduce_third(), d0, L);
+    const float s = sc.reduce();
+    if (want_thr) *thr = refined_threshold(gthird(m3), d0, L);
     return s;
 }
 
-// one slot's superposition iterations over the DP alignment
-template <int P>
-__device__ float refine_slot_dp(const Aln& a, int L, int fl, int fragpos, float d0, float d02, float d82,
-                                bool chk, float bestinit, bool dinv, float* tfmBest,
-                                const typename Prec<P>::T* ccinit, unsigned long long& ab) {
-    typedef typename Prec<P>::T T;
-    T cc[15], last[15], ncc, nlast = 0;
-    if (ccinit) {
-        for (int f = 0; f < 15; f++) cc[f] = ccinit[f];
-    } else {
-        cc_dp_frag<P>(a, L, fragpos, fl, cc, nullptr);
-    }
-    ab += AB_CC * fl;
-    ncc = (T)fl;
-    const float d0s1 = GetD0s(d0) + 1.0f, d02s1 = d0s1 * d0s1;
-    float tfm[12], thr = CP_LARGEDST, best = bestinit;
-    for (int i = 0; i < 12; i++) { tfm[i] = 0.f; tfmBest[i] = (i == 0 || i == 4 || i == 8) ? 1.f : 0.f; }
-    for (int cit = 0; cit < NMAXCONVIT + 2; cit++) {
-        if (0 < cit) {
-            float tprev[12];
-            copy12(tprev, tfm);
-            int cnt = cc_dp_ext<P>(a, L, tprev, (cit < 2) ? thr : d02s1, cc);
-            ncc = (T)cnt;
-            ab += AB_EXT * L;
-            if (cc_converged<T>(cc, ncc, last, nlast)) break;
-        }
-        if (ncc < (T)1) break;
-        for (int f = 0; f < 15; f++) last[f] = cc[f];
-        nlast = ncc;
-        kabsch<T>(cc, ncc, dinv, tfm);
-        float sc = score_dp<P>(a, L, tfm, d0, d02, d82, chk, cit < 1, &thr);
-        ab += AB_TM * L + AB_SLOT;
-        if (best < sc) { best = sc; copy12(tfmBest, tfm); }
-    }
-    return best;
-}
-
+// Superposition refinement over the DP alignment, one workgroup per pair
+// (FragmentBasedDPAlignmentRefinement, FinalFragmentBasedDPAlignmentRefinement
+// Phase1/2, ProductionFragmentBasedDPAlignmentRefinementPhase1/2:
+// MpStage1.cpp:455-510, MpStageFin.cpp:45-1470).  Slots are (fragment
+// position, fragment length) along the alignment; each iterates covariance
+// (thresholded after the first) -> Kabsch -> score to convergence.
 template <int P, int MODE>
 __global__ __launch_bounds__(RTHREADS) void k_refine_dp(DevChunk Q, DevChunk R, DevState S, Geom G,
                                                          RefineParams prm, int ldscap) {
     extern __shared__ float lds_rf[];
-    __shared__ SlotBest slds[RWAVES];
-    typedef typename Prec<P>::T T;
+    __shared__ SlotBest slds[NGROUPS];
+    __shared__ int ctr;
+    typedef typename GPrec<P>::T T;
     const int p = blockIdx.x;
     const int qi = pair_q(S, p), ri = pair_r(S, p);
     const int qlen = Q.len[qi], rlen = R.len[ri];
@@ -807,23 +837,25 @@ __global__ __launch_bounds__(RTHREADS) void k_refine_dp(DevChunk Q, DevChunk R, 
     Aln a;
     const float* g = S.aln + base;
     const size_t pl = S.alnplane;
+    if (threadIdx.x == 0) ctr = 0;
     if (!skip && L <= ldscap) {
         for (int k = threadIdx.x; k < L; k += RTHREADS) {
             lds_rf[k] = g[k]; lds_rf[ldscap + k] = g[pl + k]; lds_rf[2 * ldscap + k] = g[2 * pl + k];
             lds_rf[3 * ldscap + k] = g[3 * pl + k]; lds_rf[4 * ldscap + k] = g[4 * pl + k];
             lds_rf[5 * ldscap + k] = g[5 * pl + k];
         }
-        __syncthreads();
         a = Aln{lds_rf, lds_rf + ldscap, lds_rf + 2 * ldscap, lds_rf + 3 * ldscap, lds_rf + 4 * ldscap,
                 lds_rf + 5 * ldscap};
     } else {
         a = Aln{g, g + pl, g + 2 * pl, g + 3 * pl, g + 4 * pl, g + 5 * pl};
     }
+    __syncthreads();
 
     const bool prod = (MODE == RM_PROD1 || MODE == RM_PROD2);
     const int rna = R.na[ri];
     const float d0 = prod ? GetD0fin(qlen, rlen, rna) : GetD0(qlen, rlen);
     const float d02 = d0 * d0, d82 = GetD82(qlen, rlen);
+    const float d0s1 = GetD0s(d0) + 1.0f, d02s1 = d0s1 * d0s1;
     const int glen = max(qlen, rlen);
     const float g0 = GetD0fin(glen, glen, rna), g02 = g0 * g0;
     const bool chk = !prod;
@@ -834,48 +866,104 @@ __global__ __launch_bounds__(RTHREADS) void k_refine_dp(DevChunk Q, DevChunk R, 
     const int base_pos = phase2 ? S.sfragpos[p] : 0;
 
     SlotBest mine;
-    mine.best = 0.f; mine.slot = 0x7fffffff; mine.gbest = 0.f; mine.ndx = 0; mine.pos = 0;
+    mine.clear();
     unsigned long long ab = 0;
-    for (int si = threadIdx.x / GTA_WAVE; si < nslots; si += RWAVES) {
-        int sfragndx, sfragpos, step;
-        if (phase2) {
-            sfragndx = base_ndx;
-            if (sfragndx == 0) sfragndx++;
-            sfragpos = base_pos + (si - (G.maxnfragfcts >> 1));
-            step = 1;
-        } else {
-            const int sfragfct = si / NMAXSUBFRAGS;
-            sfragndx = si - sfragfct * NMAXSUBFRAGS;
-            sfragpos = sfragfct * SFRAGSTEP;
-            step = SFRAGSTEP;
+    bool need = nslots > 0, live = false;
+    int si = 0, cit = 0, fl = 0, sfragndx = 0, sfragpos = 0;
+    float tfm[12], tfmBest[12], thr = CP_LARGEDST, best = 0.f;
+    T last[15], nlast = 0;
+    for (;;) {
+        for (;;) {
+            const int got = fetch_slot(need, &ctr);
+            if (need) {
+                if (got >= nslots) {
+                    need = false;
+                } else {
+                    si = got;
+                    int step;
+                    if (phase2) {
+                        sfragndx = base_ndx;
+                        if (sfragndx == 0) sfragndx++;
+                        sfragpos = base_pos + (si - (G.maxnfragfcts >> 1));
+                        step = 1;
+                    } else {
+                        const int sfragfct = si / NMAXSUBFRAGS;
+                        sfragndx = si - sfragfct * NMAXSUBFRAGS;
+                        sfragpos = sfragfct * SFRAGSTEP;
+                        step = SFRAGSTEP;
+                    }
+                    fl = GetFragLength(L, L, 0, 0, sfragndx);
+                    if (fl >= 1 && !(sfragpos < 0 || L + step <= sfragpos + fl)) {
+                        cit = 0;
+                        thr = CP_LARGEDST;
+                        best = bestinit;
+#pragma unroll
+                        for (int i = 0; i < 12; i++) {
+                            tfm[i] = 0.f;
+                            tfmBest[i] = (i == 0 || i == 4 || i == 8) ? 1.f : 0.f;
+                        }
+                        nlast = 0;
+                        need = false;
+                        live = true;
+                    }
+                }
+            }
+            if (!__any(need)) break;
         }
-        const int fl = GetFragLength(L, L, 0, 0, sfragndx);
-        if (fl < 1) continue;
-        if (sfragpos < 0 || L + step <= sfragpos + fl) continue;
-        T ccinit[15];
-        const T* ccp = nullptr;
-        if (MODE == RM_PROD1 && si == 0) {
-            T sq[6];
-            cc_dp_frag<P>(a, L, sfragpos, fl, ccinit, sq);
-            const float rmsd = rmsd_e0<T>(ccinit, sq, (T)fl);
-            if (lane_id() == 0) S.out[(size_t)p * 15 + 8] = rmsd;
-            ccp = ccinit;
-        }
-        float tfmBest[12];
-        float best = refine_slot_dp<P>(a, L, fl, sfragpos, d0, d02, d82, chk, bestinit, dinv, tfmBest, ccp, ab);
-        float gbest = best;
-        if (prod && qlen != rlen) {
-            gbest = score_dp<P>(a, L, tfmBest, g0, g02, d82, false, false, nullptr);
-            ab += AB_TM * L;
-        }
-        if (best > 0.f && best > mine.best) {
-            mine.best = best; mine.slot = si; copy12(mine.tfm, tfmBest);
-            mine.gbest = gbest; mine.ndx = sfragndx; mine.pos = sfragpos;
+        if (!__any(live)) break;
+        if (live) {
+            T cc[15];
+            T ncc;
+            bool done = false;
+            if (cit == 0) {
+                if (MODE == RM_PROD1 && si == 0) {
+                    // the RMSD of the whole alignment comes with the first slot's first pass
+                    T sq[6];
+                    gcc_dp_frag<P>(a, L, sfragpos, fl, cc, sq);
+                    T ccr[15];
+#pragma unroll
+                    for (int f = 0; f < 15; f++) ccr[f] = cc[f];
+                    const float rmsd = rmsd_e0<T>(ccr, sq, (T)fl);
+                    if (glane() == 0) S.out[(size_t)p * 15 + 8] = rmsd;
+                } else {
+                    gcc_dp_frag<P>(a, L, sfragpos, fl, cc, nullptr);
+                }
+                ncc = (T)fl;
+                ab += AB_CC * (unsigned long long)fl;
+            } else {
+                ncc = (T)gcc_dp_ext<P>(a, L, tfm, (cit < 2) ? thr : d02s1, cc);
+                ab += AB_EXT * (unsigned long long)L;
+                if (cc_converged<T>(cc, ncc, last, nlast)) done = true;
+            }
+            if (!done && ncc < (T)1) done = true;
+            if (!done) {
+#pragma unroll
+                for (int f = 0; f < 15; f++) last[f] = cc[f];
+                nlast = ncc;
+                kabsch<T>(cc, ncc, dinv, tfm);
+                const float sc = gscore_dp<P>(a, L, tfm, d0, d02, d82, chk, cit < 1, &thr);
+                ab += AB_TM * (unsigned long long)L + AB_SLOT;
+                if (best < sc) { best = sc; copy12(tfmBest, tfm); }
+                if (++cit == NMAXCONVIT + 2) done = true;
+            }
+            if (done) {
+                float gbest = best;
+                if (prod && qlen != rlen) {
+                    gbest = gscore_dp<P>(a, L, tfmBest, g0, g02, d82, false, false, nullptr);
+                    ab += AB_TM * (unsigned long long)L;
+                }
+                if (best > 0.f && mine.beats(best, si)) {
+                    mine.best = best; mine.slot = si; copy12(mine.tfm, tfmBest);
+                    mine.gbest = gbest; mine.ndx = sfragndx; mine.pos = sfragpos;
+                }
+                live = false;
+                need = true;
+            }
         }
     }
-    if (lane_id() == 0 && ab) atomicAdd(S.algbytes + KC_REFINE_DP, ab);
+    if (glane() == 0 && ab) atomicAdd(S.algbytes + KC_REFINE_DP, ab);
     SlotBest b;
-    block_best(mine, slds, b);
+    group_best(mine, slds, b);
     if (threadIdx.x != 0) return;
     const bool have = b.best > 0.f;
     float* tg = S.tfm_grand + (size_t)p * 12;

@@ -1,0 +1,159 @@
+// gta_slots.h -- 16-lane slot groups for the superposition-refinement kernels.
+//
+// A refinement kernel works on many independent slots per pair (a seed
+// diagonal, or a fragment length and position); each slot iterates
+// covariance pass -> 3x3 Kabsch solve -> TM-score pass.  The solve is a long
+// serial chain, so one wave runs four slots at once, one per 16-lane DPP row:
+// the passes stride positions over the 16 lanes of the row, the row reduces
+// with four DPP steps (every lane ends with the same, bitwise identical sum),
+// and the four rows' solves run side by side in the same instructions.
+#pragma once
+
+#include "gta_device.h"
+
+namespace gta {
+
+constexpr int GL = 16;                  // lanes per slot group (one DPP row)
+constexpr int NGW = GTA_WAVE / GL;      // slot groups per wave
+
+__device__ __forceinline__ int glane() { return lane_id() & (GL - 1); }
+
+// DPP row permutations used by the group reductions
+enum { DPP_XOR1 = 0xB1, DPP_XOR2 = 0x4E, DPP_MIRROR = 0x140, DPP_HMIRROR = 0x141 };
+
+template <int CTRL> __device__ __forceinline__ int dpp_i(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xf, 0xf, false);
+}
+template <int CTRL> __device__ __forceinline__ float dpp_f(float v) {
+    return __builtin_bit_cast(float, dpp_i<CTRL>(__builtin_bit_cast(int, v)));
+}
+template <int CTRL> __device__ __forceinline__ double dpp_d(double v) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)dpp_i<CTRL>((int)(unsigned)b);
+    const unsigned hi = (unsigned)dpp_i<CTRL>((int)(unsigned)(b >> 32));
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | (unsigned long long)lo);
+}
+
+// Sum over the 16 lanes of the row.  Partners at each step hold disjoint
+// partial sums and add them commutatively, so all 16 lanes end bitwise equal.
+__device__ __forceinline__ float gsum(float v) {
+    v = v + dpp_f<DPP_XOR1>(v);
+    v = v + dpp_f<DPP_XOR2>(v);
+    v = v + dpp_f<DPP_HMIRROR>(v);
+    v = v + dpp_f<DPP_MIRROR>(v);
+    return v;
+}
+__device__ __forceinline__ double gsum(double v) {
+    v = v + dpp_d<DPP_XOR1>(v);
+    v = v + dpp_d<DPP_XOR2>(v);
+    v = v + dpp_d<DPP_HMIRROR>(v);
+    v = v + dpp_d<DPP_MIRROR>(v);
+    return v;
+}
+__device__ __forceinline__ int gsum(int v) {
+    v = v + dpp_i<DPP_XOR1>(v);
+    v = v + dpp_i<DPP_XOR2>(v);
+    v = v + dpp_i<DPP_HMIRROR>(v);
+    v = v + dpp_i<DPP_MIRROR>(v);
+    return v;
+}
+
+// three smallest of the row's values: multiset (M = Min3) or distinct (M = Min3d)
+template <int CTRL, class M> __device__ __forceinline__ void gmerge3_step(M& m) {
+    const float x = dpp_f<CTRL>(m.a), y = dpp_f<CTRL>(m.b), z = dpp_f<CTRL>(m.c);
+    m.put(x);
+    m.put(y);
+    m.put(z);
+}
+template <class M> __device__ __forceinline__ float gthird(M& m) {
+    gmerge3_step<DPP_XOR1>(m);
+    gmerge3_step<DPP_XOR2>(m);
+    gmerge3_step<DPP_HMIRROR>(m);
+    gmerge3_step<DPP_MIRROR>(m);
+    return m.c;
+}
+
+// covariance accumulators reduced over the row
+struct GAccF32 {
+    float s[15];
+    __device__ __forceinline__ void clear() {
+#pragma unroll
+        for (int i = 0; i < 15; i++) s[i] = 0.f;
+    }
+    __device__ __forceinline__ void add(float qx, float qy, float qz, float rx, float ry, float rz) {
+        s[0] += qx * rx; s[1] += qx * ry; s[2] += qx * rz;
+        s[3] += qy * rx; s[4] += qy * ry; s[5] += qy * rz;
+        s[6] += qz * rx; s[7] += qz * ry; s[8] += qz * rz;
+        s[9] += qx; s[10] += qy; s[11] += qz;
+        s[12] += rx; s[13] += ry; s[14] += rz;
+    }
+    __device__ __forceinline__ void reduce(float* out) {
+#pragma unroll
+        for (int i = 0; i < 15; i++) out[i] = gsum(s[i]);
+    }
+};
+// compensated FP32 per lane (Kahan), combined over the row in FP64
+struct GAccKahan {
+    float s[15], c[15];
+    __device__ __forceinline__ void clear() {
+#pragma unroll
+        for (int i = 0; i < 15; i++) { s[i] = 0.f; c[i] = 0.f; }
+    }
+    __device__ __forceinline__ void k(int i, float v) {
+        const float y = __fsub_rn(v, c[i]);
+        const float t = __fadd_rn(s[i], y);
+        c[i] = __fsub_rn(__fsub_rn(t, s[i]), y);
+        s[i] = t;
+    }
+    __device__ __forceinline__ void add(float qx, float qy, float qz, float rx, float ry, float rz) {
+        k(0, qx * rx); k(1, qx * ry); k(2, qx * rz);
+        k(3, qy * rx); k(4, qy * ry); k(5, qy * rz);
+        k(6, qz * rx); k(7, qz * ry); k(8, qz * rz);
+        k(9, qx); k(10, qy); k(11, qz);
+        k(12, rx); k(13, ry); k(14, rz);
+    }
+    __device__ __forceinline__ void reduce(double* out) {
+#pragma unroll
+        for (int i = 0; i < 15; i++) out[i] = gsum((double)s[i] - (double)c[i]);
+    }
+};
+
+template <int P> struct GScore;
+template <> struct GScore<0> {
+    float s;
+    __device__ __forceinline__ void clear() { s = 0.f; }
+    __device__ __forceinline__ void add(float v) { s += v; }
+    __device__ __forceinline__ float reduce() { return gsum(s); }
+};
+template <> struct GScore<1> {
+    float s, c;
+    __device__ __forceinline__ void clear() { s = 0.f; c = 0.f; }
+    __device__ __forceinline__ void add(float v) {
+        const float y = __fsub_rn(v, c), t = __fadd_rn(s, y);
+        c = __fsub_rn(__fsub_rn(t, s), y);
+        s = t;
+    }
+    __device__ __forceinline__ float reduce() { return (float)gsum((double)s - (double)c); }
+};
+
+template <int P> struct GPrec;
+template <> struct GPrec<0> { typedef float T; typedef GAccF32 Acc; };
+template <> struct GPrec<1> { typedef double T; typedef GAccKahan Acc; };
+
+// Groups of this wave that need a slot take consecutive indices from the
+// workgroup's counter in LDS.  Called by the whole wave (convergent).
+__device__ __forceinline__ int fetch_slot(bool need, int* ctr) {
+    const unsigned long long lead = __ballot(need && glane() == 0);
+    const int cnt = __popcll(lead);
+    int base = 0;
+    if (cnt) {
+        int b = 0;
+        if (lane_id() == 0) b = atomicAdd(ctr, cnt);
+        base = __shfl(b, 0, GTA_WAVE);
+    }
+    const int g0 = lane_id() & ~(GL - 1);
+    const int rank = __popcll(lead & ((1ull << g0) - 1ull));
+    return need ? base + rank : -1;
+}
+
+}  // namespace gta
