@@ -441,6 +441,20 @@ void calc_secstr(const gta_chunk_view* c, std::vector<char>& ss) {
 // ---- lane-structured reductions as in the reference CPU kernels -------------
 // A pass over n items splits them into blocks of XDIM; item (block ai, lane pi)
 // accumulates into lane pi; lanes are reduced in order.
+// Sum of XDIM lane partials in the order of the reference's
+// `#pragma omp simd reduction(+:sum)` loops over XDIM (MpStageBase.h:351,443,
+// 660,925; MpStage1.h:483-1134; MpStageFin.h:627) as g++ -O3 -fopenmp
+// vectorizes them for x86-64 (SSE2, 4 floats per vector): four strided
+// partial sums (lane i accumulates elements i, i+4, i+8, ...) added in order
+// at the end.  The order is pinned against that build (oracle/_ref).
+inline float xdim_sum(const float* v) {
+    float p0 = 0.f, p1 = 0.f, p2 = 0.f, p3 = 0.f;
+    for (int i = 0; i < XDIM; i += 4) { p0 += v[i]; p1 += v[i + 1]; p2 += v[i + 2]; p3 += v[i + 3]; }
+    float s = 0.f;
+    s += p0; s += p1; s += p2; s += p3;
+    return s;
+}
+
 struct LaneCC {
     float v[NCCX][XDIM];
     void clear() { std::memset(v, 0, sizeof(v)); }
@@ -455,7 +469,7 @@ struct LaneCC {
             v[19][pi] += rx * rx; v[20][pi] += ry * ry; v[21][pi] += rz * rz;
         }
     }
-    float sum(int f) const { float s = 0.f; for (int i = 0; i < XDIM; i++) s += v[f][i]; return s; }
+    float sum(int f) const { return xdim_sum(v[f]); }
 };
 
 struct Min3 {
@@ -687,7 +701,7 @@ float scores_dp(Ctx& c, int readcnst, float d0, float d02, float d82, bool chkds
         }
     }
     float sum = 0.f;
-    for (int i = 0; i < XDIM; i++) sum += scv[i];
+    sum = xdim_sum(scv);
     if (readcnst == READCNST_CALC && thr) {
         float min3 = m3.third_multiset();
         float d0s = GetD0s(d0) + ((readcnst == READCNST_CALC2) ? 1.0f : -1.0f);
@@ -785,7 +799,7 @@ float scores_gapless_seed(Ctx& c, int readcnst, int qpos, int rpos, int n, float
         }
     }
     float sum = 0.f;
-    for (int i = 0; i < XDIM; i++) sum += scv[i];
+    sum = xdim_sum(scv);
     if (savepos && thr) {
         float min3 = m3.third_distinct();
         float d02s = GetD02s(d0);
@@ -815,7 +829,7 @@ float scores_gapless_refined(Ctx& c, int readcnst, int qpos, int rpos, float d0,
         }
     }
     float sum = 0.f;
-    for (int i = 0; i < XDIM; i++) sum += scv[i];
+    sum = xdim_sum(scv);
     if (readcnst == READCNST_CALC && thr) {
         float min3 = m3.third_multiset();
         float d0s = GetD0s(d0) + ((readcnst == READCNST_CALC2) ? 1.0f : -1.0f);

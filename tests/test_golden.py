@@ -26,14 +26,10 @@ def load(path):
 def test_oracle_reproduces_golden(path):
     q, r, d = load(path)
     o = oracle.process_block(q, r)
-    ref = d["alndata"]
+    # bit for bit: records, transforms and alignment rows
     assert np.array_equal(o.alns, d["alns"])
-    for f in (2, 3, 4, 5, 6, 7, 11, 12):
-        assert np.array_equal(o.alndata[..., f], ref[..., f]), f
-    assert np.abs(o.alndata[..., 9:11] - ref[..., 9:11]).max() < 1e-4
-    assert np.abs(o.alndata[..., 8] - ref[..., 8]).max() < 1e-3
-    aligned = ref[..., 8] < 9999
-    assert np.abs(o.tfms - d["tfms"])[aligned].max() < 1e-3
+    assert np.array_equal(o.alndata, d["alndata"])
+    assert np.array_equal(o.tfms, d["tfms"])
 
 
 def test_golden_fixtures_present():
