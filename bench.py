@@ -31,6 +31,14 @@ from gtalign_alpha_amd.batch import GtaBatch  # noqa: E402
 from gtalign_alpha_amd.capi import default_options, PREC_FP32, PREC_MIXED  # noqa: E402
 
 HBM_PEAK = 8.0e12
+# kernel class -> the kernel (template instance) rocprofv3 lists for it in the plain FP32 form
+KERNEL_NAMES = {
+    "findfrag": "gta::k_findfrag<0>",
+    "refine_gapless": "gta::k_refine_gapless<0>",
+    "dp": "gta::k_dp<false> (29 of 30 launches; 1 is k_dp<true>)",
+    "refine_dp": "gta::k_refine_dp<0, 0> (RM_DPREFINE)",
+    "refine_final": "gta::k_refine_dp<0, 1..4> (final and production phases)",
+}
 
 
 def log(*a):
@@ -63,44 +71,132 @@ def time_form(qc, rc, prec, steps, warmup, profile):
     return wall, dev, cells, stats
 
 
-def cpu_baseline(qc, rc, gpu_tm_check=True):
-    """The reference's own CPU path (oracle/_ref) on a bounded sample."""
+def _ref_run(q, r, nthreads, tmp):
+    """The reference's CPU path on (q, r) in a child process (its thread count
+    is fixed per process); returns (alndata, seconds)."""
+    import subprocess
+    from oracle.ref_cli import save_pair
+    src, dst = os.path.join(tmp, "in_%d.npz" % nthreads), os.path.join(tmp, "out_%d.npz" % nthreads)
+    save_pair(src, q, r)
+    subprocess.run([sys.executable, "-m", "oracle.ref_cli", src, dst, str(nthreads)], cwd=ROOT, check=True)
+    d = np.load(dst, allow_pickle=False)
+    return d["alndata"].reshape(q.nstrs, r.nstrs, -1), float(d["seconds"])
+
+
+def _gpu_block(q, r, prec=PREC_FP32):
+    with GtaBatch(0, default_options(precision=prec), max_qrys=q.nstrs, max_qry_poss=q.nposs,
+                  max_refs=r.nstrs, max_ref_poss=r.nposs) as b:
+        b.TransferQueryPMDataToDevice(q)
+        b.TransferCPMDataToDevice(r)
+        return b.ProcessBlock()
+
+
+def _dp_identity(q, r, tfms):
+    """Fraction of pairs whose DP alignment under the given transforms is
+    identical between the DP kernel and the oracle's DP (GAP0 and gap-open
+    -0.6 forms)."""
+    from oracle import oracle
+    same = total = 0
+    for gap0, gapopen in ((True, 0.0), (False, -0.6)):
+        with GtaBatch(0, default_options(), max_qrys=q.nstrs, max_qry_poss=q.nposs, max_refs=r.nstrs,
+                      max_ref_poss=r.nposs) as b:
+            b.TransferQueryPMDataToDevice(q)
+            b.TransferCPMDataToDevice(r)
+            got = b.dp_under(tfms, gap0, gapopen)
+        for qi in range(q.nstrs):
+            for ri in range(r.nstrs):
+                d02 = float(oracle.d0(int(q.len[qi]), int(r.len[ri]))["d0"]) ** 2
+                _, want = oracle.dp(q.coords(qi), r.coords(ri), tfms[qi, ri], d02, gapopen, gap0)
+                same += int(np.array_equal(got[qi, ri], want))
+                total += 1
+    return same / total, total
+
+
+def _fp64_summary(res, q, r):
+    from oracle.fp64 import fp64_errors
+    dtm, drm = fp64_errors(res, q, r)
+    if not len(dtm):
+        return None
+    return {"aligned_pairs": int(len(dtm)), "tm_err_max": float(dtm.max()), "tm_err_mean": float(dtm.mean()),
+            "rmsd_err_max": float(drm.max()), "rmsd_err_mean": float(drm.mean())}
+
+
+def _tm_agreement(ref, g):
+    aligned = ref[..., 8] < 9999.0
+    dtm = np.abs(g.alndata[..., 9:11] - ref[..., 9:11]).max(-1)
+    return {"pairs": int(dtm.size), "aligned_pairs": int(aligned.sum()),
+            "same_aligned_set": bool(np.array_equal(aligned, g.alndata[..., 8] < 9999.0)),
+            "tm_err_max": float(dtm.max()), "pairs_tm_err_gt_1e-4": int((dtm > 1e-4).sum())}
+
+
+def cpu_baseline(qc, rc, src):
+    """The reference's own CPU path (oracle/_ref) on a bounded sample of the
+    workload: timed with the host's threads; the GPU's agreement is checked
+    against a single-thread run (the reference's result for a few pairs
+    depends on its thread count)."""
+    import tempfile
     from oracle import oracle
     rng = np.random.default_rng(7)
     qsel = [0, qc.nstrs // 3, (2 * qc.nstrs) // 3, qc.nstrs - 1]
-    nref = min(rc.nstrs, 2000)
+    nref = min(rc.nstrs, 8000)
     rsel = np.sort(rng.choice(rc.nstrs, nref, replace=False))
     q = qc.subset(qsel)
     r = rc.subset(rsel)
-    if oracle.ref_available():
-        threads = max(1, min(16, os.cpu_count() or 1))
-        t0 = time.perf_counter()
-        res = oracle.ref_process_block(q, r, nthreads=threads)
-        dt = time.perf_counter() - t0
-        kind = "reference"
-    else:
+    npairs = q.nstrs * r.nstrs
+    if not oracle.ref_available():
         q = q.subset([0, 1])
         r = r.subset(list(range(min(400, r.nstrs))))
-        threads = 1
         t0 = time.perf_counter()
         res = oracle.process_block(q, r)
         dt = time.perf_counter() - t0
-        kind = "port"
-    npairs = q.nstrs * r.nstrs
-    out = {"value": npairs / dt, "unit": "pairs/s", "cores": threads, "kind": kind,
-           "sample": "%d queries x %d references of the same workload (%d pairs), %.1f s" %
-                     (q.nstrs, r.nstrs, npairs, dt)}
-    if gpu_tm_check:
-        with GtaBatch(0, default_options(precision=PREC_FP32), max_qrys=q.nstrs,
-                      max_qry_poss=q.nposs, max_refs=r.nstrs, max_ref_poss=r.nposs) as b:
-            b.TransferQueryPMDataToDevice(q)
-            b.TransferCPMDataToDevice(r)
-            g = b.ProcessBlock()
-        aligned = res.alndata[..., 8] < 9999.0
-        dtm = np.abs(g.alndata[..., 9:11] - res.alndata[..., 9:11])
-        out["tm_err_max"] = float(dtm.max())
-        out["aligned_pairs"] = int(aligned.sum())
-        out["same_aligned_set"] = bool(np.array_equal(aligned, g.alndata[..., 8] < 9999.0))
+        out = {"value": q.nstrs * r.nstrs / dt, "unit": "pairs/s", "cores": 1, "kind": "port",
+               "sample": "%d queries x %d references of the same workload, %.1f s" % (q.nstrs, r.nstrs, dt)}
+        out["gpu_vs_port"] = _tm_agreement(res.alndata, _gpu_block(q, r))
+        return out
+    threads = max(1, min(16, os.cpu_count() or 1))
+    with tempfile.TemporaryDirectory() as tmp:
+        ref_t, dt = _ref_run(q, r, threads, tmp)
+        # single-thread parity sample: 2 queries x (their homologs + as many others)
+        qs = q.subset([0, 3])
+        rsrc = src[rsel]
+        hom = np.flatnonzero((rsrc == qsel[0]) | (rsrc == qsel[3]))[:200]
+        oth = np.setdiff1d(np.arange(r.nstrs), hom)
+        oth = oth[np.linspace(0, len(oth) - 1, len(hom) + 50).astype(int)]
+        rs = r.subset(np.union1d(hom, oth))
+        ref_1, _ = _ref_run(qs, rs, 1, tmp)
+    out = {"value": npairs / dt, "unit": "pairs/s", "cores": threads, "kind": "reference",
+           "sample": "%d queries x %d references of the same workload (%d pairs), %.1f s with %d threads" %
+                     (q.nstrs, r.nstrs, npairs, dt, threads)}
+    g32, gmx = _gpu_block(qs, rs, PREC_FP32), _gpu_block(qs, rs, PREC_MIXED)
+    out["gpu_vs_reference_1thread"] = {"sample": "%d x %d pairs" % (qs.nstrs, rs.nstrs),
+                                       "plain": _tm_agreement(ref_1, g32), "mixed": _tm_agreement(ref_1, gmx)}
+    out["gpu_vs_reference_%dthreads" % threads] = _tm_agreement(ref_t, _gpu_block(q, r))
+    # FP64 recomputation of every reported TM-score (under the reported
+    # transform) and RMSD (SVD superposition of the aligned pairs)
+    out["vs_fp64"] = {"plain": _fp64_summary(g32, qs, rs), "mixed": _fp64_summary(gmx, qs, rs)}
+    frac, n = _dp_identity(qs, rs, g32.tfms)
+    out["dp_bit_identical"] = {"fraction": frac, "fills": n}
+    return out
+
+
+def precision_stress():
+    """Near-collinear/near-planar chains against their noisy rigid copies
+    (20 queries x 2,000 references, 2,000 copy pairs): max/mean error of the
+    reported TM-score and RMSD against FP64 recomputations, plain and mixed."""
+    from oracle.fp64 import fp64_errors
+    qc, rc, src = synth.config(3)
+    out = {"workload": "20 segment chains (40%% straight 3-8 residue segments, rest helical) x %d "
+                       "noisy rigid copies (sigma 0.01-0.1 A)" % rc.nstrs}
+    for name, prec in (("plain", PREC_FP32), ("mixed", PREC_MIXED)):
+        g = _gpu_block(qc, rc, prec)
+        # the copy pairs only
+        sel = np.zeros((qc.nstrs, rc.nstrs), bool)
+        sel[src, np.arange(rc.nstrs)] = True
+        g.alndata = np.where(sel[..., None], g.alndata, np.float32(9999.9))
+        dtm, drm = fp64_errors(g, qc, rc)
+        out[name] = {"copy_pairs_aligned": int(len(dtm)), "tm_err_max": float(dtm.max()),
+                     "tm_err_mean": float(dtm.mean()), "rmsd_err_max": float(drm.max()),
+                     "rmsd_err_mean": float(drm.mean())}
     return out
 
 
@@ -122,7 +218,7 @@ def main():
         sys.exit(2)
 
     t0 = time.perf_counter()
-    qc, rc, _ = synth.config(1, scale=args.scale)
+    qc, rc, src = synth.config(1, scale=args.scale)
     npairs = qc.nstrs * rc.nstrs
     log("workload: %d queries x %d references = %d pairs (%d + %d residues), generated in %.1f s" %
         (qc.nstrs, rc.nstrs, npairs, qc.nposs, rc.nposs, time.perf_counter() - t0))
@@ -134,8 +230,11 @@ def main():
     mixed = None
     if not args.no_mixed:
         log("mixed-precision form")
-        wall_m, dev_m, _, _ = time_form(qc, rc, PREC_MIXED, args.steps, args.warmup, False)
+        wall_m, dev_m, _, stats_m = time_form(qc, rc, PREC_MIXED, args.steps, args.warmup, True)
         mixed = npairs * args.steps / wall_m
+        for st in stats_m:
+            if st["launches"]:
+                log("  %-16s launches %4d  %8.3f s" % (st["name"], st["launches"], st["seconds"]))
 
     for st in stats:
         if st["launches"]:
@@ -153,14 +252,18 @@ def main():
     achieved = dom["alg_bytes"] / dom["seconds"]
     roofline = {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
-                "kernel": dom["name"],
+                "kernel": KERNEL_NAMES.get(dom["name"], dom["name"]),
+                "launches": dom["launches"],
                 "launch_ms": round(1e3 * dom["seconds"] / dom["launches"], 3),
                 "alg_bytes_per_launch": dom["alg_bytes"] / dom["launches"]}
+
+    log("precision stress")
+    stress = precision_stress()
 
     cpu = None
     if not args.no_cpu_baseline:
         log("CPU baseline")
-        cpu = cpu_baseline(qc, rc)
+        cpu = cpu_baseline(qc, rc, src)
 
     out = {
         "metric": "aligned pairs/sec + DP GCUPS on 1 MI355X, batched vs plain, TM err<=1e-4",
@@ -182,6 +285,7 @@ def main():
         "mixed_precision_pairs_per_s": round(mixed, 1) if mixed else None,
         "kernels": kern,
         "roofline": roofline,
+        "precision_stress": stress,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -407,14 +407,14 @@ static int run_stage(gta_ctx* c, double* seconds, double* cells) {
     if (0.0f < o.scorethld) GTA_TIMED(KC_MISC, launch_setlow(Q, R, S, o.scorethld, st));
     // final stage (MpStageFin.h:66-90)
     GTA_TIMED(KC_DP, dp(DPM_GRAND_L, 1, 0.0f, 0, 0));
-    GTA_TIMED(KC_REFINE_DP, refine(RM_FIN1_L, 0, 0, 0, 0.f));
+    GTA_TIMED(KC_REFINE_FIN, refine(RM_FIN1_L, 0, 0, 0, 0.f));
     if (!(o.depth == GTA_DEPTH_SHALLOW || o.refinement == GTA_REF_COARSE))
-        GTA_TIMED(KC_REFINE_DP, refine(RM_FIN2_L, 0, 0, 0, 0.f));
+        GTA_TIMED(KC_REFINE_FIN, refine(RM_FIN2_L, 0, 0, 0, 0.f));
     GTA_TIMED(KC_DP, dp(DPM_CONSTRAINED_L, 1, 0.0f, 0, 0));
     GTA_TIMED(KC_PROD_ALN, launch_prod_aln(Q, R, S, c->s_alns.as<char>(), o.nodeletions, st));
-    GTA_TIMED(KC_REFINE_DP, refine(RM_PROD1_L, 0, 0, 0, 0.f));
+    GTA_TIMED(KC_REFINE_FIN, refine(RM_PROD1_L, 0, 0, 0, 0.f));
     if (o.refinement == GTA_REF_ONE || o.refinement == GTA_REF_COARSE)
-        GTA_TIMED(KC_REFINE_DP, refine(RM_PROD2_L, 0, 0, 0, 0.f));
+        GTA_TIMED(KC_REFINE_FIN, refine(RM_PROD2_L, 0, 0, 0, 0.f));
     if (o.referenced) GTA_TIMED(KC_MISC, launch_revert(S, st));
 #undef GTA_TIMED
     HIPCHK(c, hipEventRecord(c->ev1, st));
@@ -448,7 +448,7 @@ int gta_set_profiling(gta_ctx* c, int on) {
 
 int gta_kernel_stats(const gta_ctx* c, gta_kernel_stat* out, int n) {
     static const char* names[KC_COUNT] = {"secstr", "init", "findfrag", "refine_gapless", "dp",
-                                          "refine_dp", "prod_aln", "misc"};
+                                          "refine_dp", "refine_final", "prod_aln", "misc"};
     if (!c) return GTA_ERR_ARG;
     for (int k = 0; k < KC_COUNT && k < n; k++) {
         out[k].name = names[k];

@@ -79,8 +79,8 @@ struct DevState {
 };
 
 // kernel classes for the measurement counters
-enum { KC_SECSTR = 0, KC_INIT, KC_FINDFRAG, KC_REFINE_GAPLESS, KC_DP, KC_REFINE_DP, KC_PROD_ALN,
-       KC_MISC, KC_COUNT };
+enum { KC_SECSTR = 0, KC_INIT, KC_FINDFRAG, KC_REFINE_GAPLESS, KC_DP, KC_REFINE_DP, KC_REFINE_FIN,
+       KC_PROD_ALN, KC_MISC, KC_COUNT };
 
 // algorithmic bytes per aligned pair and pass, and per slot iteration
 // (covariance pass 24 B, TM-score pass and thresholded covariance pass 28 B,
@@ -131,16 +131,6 @@ __device__ __forceinline__ float pair_score(float d02, float d2) {
 // ---- wave64 collectives -------------------------------------------------------
 __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
 
-__device__ __forceinline__ float wave_sum(float v) {
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, GTA_WAVE);
-    return v;
-}
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, GTA_WAVE);
-    return v;
-}
 __device__ __forceinline__ int wave_sum_i(int v) {
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, GTA_WAVE);
@@ -156,14 +146,6 @@ struct Min3 {
         else if (d < b) { c = b; b = d; }
         else if (d < c) c = d;
     }
-    __device__ float reduce_third() {
-#pragma unroll
-        for (int o = 32; o >= 1; o >>= 1) {
-            float x = __shfl_xor(a, o, GTA_WAVE), y = __shfl_xor(b, o, GTA_WAVE), z = __shfl_xor(c, o, GTA_WAVE);
-            put(x); put(y); put(z);
-        }
-        return c;
-    }
 };
 // three smallest distinct values
 struct Min3d {
@@ -174,60 +156,6 @@ struct Min3d {
         if (d < a) { c = b; b = a; a = d; }
         else if (d < b) { c = b; b = d; }
         else if (d < c) c = d;
-    }
-    __device__ float reduce_third() {
-#pragma unroll
-        for (int o = 32; o >= 1; o >>= 1) {
-            float x = __shfl_xor(a, o, GTA_WAVE), y = __shfl_xor(b, o, GTA_WAVE), z = __shfl_xor(c, o, GTA_WAVE);
-            put(x); put(y); put(z);
-        }
-        return c;
-    }
-};
-
-// ---- covariance accumulators ----------------------------------------------------
-// plain FP32 (the reference's arithmetic)
-struct AccF32 {
-    float s[15];
-    __device__ void clear() {
-#pragma unroll
-        for (int i = 0; i < 15; i++) s[i] = 0.f;
-    }
-    __device__ void add(float qx, float qy, float qz, float rx, float ry, float rz) {
-        s[0] += qx * rx; s[1] += qx * ry; s[2] += qx * rz;
-        s[3] += qy * rx; s[4] += qy * ry; s[5] += qy * rz;
-        s[6] += qz * rx; s[7] += qz * ry; s[8] += qz * rz;
-        s[9] += qx; s[10] += qy; s[11] += qz;
-        s[12] += rx; s[13] += ry; s[14] += rz;
-    }
-    __device__ void reduce(float* out) {
-#pragma unroll
-        for (int i = 0; i < 15; i++) out[i] = wave_sum(s[i]);
-    }
-};
-// compensated FP32 per lane (Kahan), combined across lanes in FP64
-struct AccKahan {
-    float s[15], c[15];
-    __device__ void clear() {
-#pragma unroll
-        for (int i = 0; i < 15; i++) { s[i] = 0.f; c[i] = 0.f; }
-    }
-    __device__ __forceinline__ void k(int i, float v) {
-        float y = __fsub_rn(v, c[i]);
-        float t = __fadd_rn(s[i], y);
-        c[i] = __fsub_rn(__fsub_rn(t, s[i]), y);
-        s[i] = t;
-    }
-    __device__ void add(float qx, float qy, float qz, float rx, float ry, float rz) {
-        k(0, qx * rx); k(1, qx * ry); k(2, qx * rz);
-        k(3, qy * rx); k(4, qy * ry); k(5, qy * rz);
-        k(6, qz * rx); k(7, qz * ry); k(8, qz * rz);
-        k(9, qx); k(10, qy); k(11, qz);
-        k(12, rx); k(13, ry); k(14, rz);
-    }
-    __device__ void reduce(double* out) {
-#pragma unroll
-        for (int i = 0; i < 15; i++) out[i] = wave_sum((double)s[i] - (double)c[i]);
     }
 };
 

@@ -97,30 +97,48 @@ __device__ __forceinline__ void CalcPartialA(int col, T d, const T* rr, T* a) {
     n = (n > eps) ? (T)1 / t_sqrt(n) : (T)0;
     a[col] = v0 * n; a[col + 3] = v1 * n; a[col + 6] = v2 * n;
 }
+// Completion of a unit vector x orthogonal to y when x has degenerated
+// (transformbase.h:400-420): the row r of y with the smallest magnitude
+// (last one on ties, starting from 1) gets x_r = 0, and the other two rows
+// k = r+1, l = r+2 (mod 3) get x_k = -y_l/p, x_l = y_k/p, p = |(y_k, y_l)|.
+// Written with selects so that no array is indexed at run time.
+template <typename T>
+__device__ __forceinline__ bool complete_orthogonal(T y0, T y1, T y2, T& x0, T& x1, T& x2) {
+    T p = 1;
+    int r = 2;
+    if (t_abs(y0) <= p) { p = t_abs(y0); r = 0; }
+    if (t_abs(y1) <= p) { p = t_abs(y1); r = 1; }
+    if (t_abs(y2) <= p) { p = t_abs(y2); r = 2; }
+    const T yk = (r == 0) ? y1 : ((r == 1) ? y2 : y0);
+    const T yl = (r == 0) ? y2 : ((r == 1) ? y0 : y1);
+    p = t_sqrt(yk * yk + yl * yl);
+    if (p <= (T)TFMTOL) return false;
+    const T xk = -yl / p, xl = yk / p;
+    x0 = (r == 0) ? (T)0 : ((r == 1) ? xl : xk);
+    x1 = (r == 1) ? (T)0 : ((r == 2) ? xl : xk);
+    x2 = (r == 2) ? (T)0 : ((r == 0) ? xl : xk);
+    return true;
+}
+
 template <typename T>
 __device__ __forceinline__ bool CalcCompleteA(T e0, T e1, T e2, T* a) {
     T d = a[0] * a[2] + a[3] * a[5] + a[6] * a[8];
-    int m1 = 0, m = 2;
-    if (e0 - e1 > e1 - e2) { m1 = 2; m = 0; }
+    // x: the column to orthogonalise (m1), y: the other eigenvector column (m)
+    const bool sw = e0 - e1 > e1 - e2;   // m1 = 2, m = 0; otherwise m1 = 0, m = 2
+    const T y0 = sw ? a[0] : a[2], y1 = sw ? a[3] : a[5], y2 = sw ? a[6] : a[8];
+    T x0 = sw ? a[2] : a[0], x1 = sw ? a[5] : a[3], x2 = sw ? a[8] : a[6];
     T p = 0;
-    a[m1] -= d * a[m]; p += a[m1] * a[m1];
-    a[m1 + 3] -= d * a[m + 3]; p += a[m1 + 3] * a[m1 + 3];
-    a[m1 + 6] -= d * a[m + 6]; p += a[m1 + 6] * a[m1 + 6];
+    x0 -= d * y0; p += x0 * x0;
+    x1 -= d * y1; p += x1 * x1;
+    x2 -= d * y2; p += x2 * x2;
     if (p > (T)TFMTOL) {
         p = (T)1 / t_sqrt(p);
-        a[m1] *= p; a[m1 + 3] *= p; a[m1 + 6] *= p;
-    } else {
-        p = 1;
-        int j = 6, k = 0, l = 3;
-        if (t_abs(a[m]) <= p) { p = t_abs(a[m]); j = 0; k = 3; l = 6; }
-        if (t_abs(a[m + 3]) <= p) { p = t_abs(a[m + 3]); j = 3; k = 6; l = 0; }
-        if (t_abs(a[m + 6]) <= p) { p = t_abs(a[m + 6]); j = 6; k = 0; l = 3; }
-        p = t_sqrt(a[k + m] * a[k + m] + a[l + m] * a[l + m]);
-        if (p <= (T)TFMTOL) return false;
-        a[j + m1] = 0;
-        a[k + m1] = -a[l + m] / p;
-        a[l + m1] = a[k + m] / p;
+        x0 *= p; x1 *= p; x2 *= p;
+    } else if (!complete_orthogonal(y0, y1, y2, x0, x1, x2)) {
+        return false;
     }
+    if (sw) { a[2] = x0; a[5] = x1; a[8] = x2; }
+    else { a[0] = x0; a[3] = x1; a[6] = x2; }
     a[1] = a[5] * a[6] - a[3] * a[8];
     a[4] = a[8] * a[0] - a[6] * a[2];
     a[7] = a[2] * a[3] - a[0] * a[5];
@@ -151,16 +169,9 @@ __device__ __forceinline__ bool CalcRotMtx(const T* a, T* b) {
         p = (T)1 / t_sqrt(p);
         b[1] *= p; b[4] *= p; b[7] *= p;
     } else {
-        p = 1;
-        int j = 6, k = 0, l = 3;
-        if (t_abs(b[0]) <= p) { p = t_abs(b[0]); j = 0; k = 3; l = 6; }
-        if (t_abs(b[3]) <= p) { p = t_abs(b[3]); j = 3; k = 6; l = 0; }
-        if (t_abs(b[6]) <= p) { p = t_abs(b[6]); j = 6; k = 0; l = 3; }
-        p = t_sqrt(b[k] * b[k] + b[l] * b[l]);
-        if (p <= (T)TFMTOL) return false;
-        b[j + 1] = 0;
-        b[k + 1] = -b[l] / p;
-        b[l + 1] = b[k] / p;
+        T x0, x1, x2;
+        if (!complete_orthogonal(b[0], b[3], b[6], x0, x1, x2)) return false;
+        b[1] = x0; b[4] = x1; b[7] = x2;
     }
     b[2] = b[3] * b[7] - b[4] * b[6];
     b[5] = b[6] * b[1] - b[7] * b[0];

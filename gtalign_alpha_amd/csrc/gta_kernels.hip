@@ -82,29 +82,6 @@ constexpr int RTHREADS = RWAVES * GTA_WAVE;
 __device__ __forceinline__ int pair_q(const DevState& S, int p) { return p / S.nr; }
 __device__ __forceinline__ int pair_r(const DevState& S, int p) { return p % S.nr; }
 
-template <int P> struct Prec;
-template <> struct Prec<0> { typedef float T; typedef AccF32 Acc; };
-template <> struct Prec<1> { typedef double T; typedef AccKahan Acc; };
-
-// sum of per-lane FP32 partial sums; P=1 keeps a Kahan compensation per lane
-template <int P> struct ScoreAcc;
-template <> struct ScoreAcc<0> {
-    float s;
-    __device__ void clear() { s = 0.f; }
-    __device__ void add(float v) { s += v; }
-    __device__ float reduce() { return wave_sum(s); }
-};
-template <> struct ScoreAcc<1> {
-    float s, c;
-    __device__ void clear() { s = 0.f; c = 0.f; }
-    __device__ void add(float v) {
-        float y = __fsub_rn(v, c), t = __fadd_rn(s, y);
-        c = __fsub_rn(__fsub_rn(t, s), y);
-        s = t;
-    }
-    __device__ float reduce() { return (float)wave_sum((double)s - (double)c); }
-};
-
 // convergence of the covariance record (CheckConvergenceRefined_Complete,
 // MpStageBase.h:262-277): all 16 fields changed by less than 1e-5 relative
 template <typename T>
@@ -208,6 +185,13 @@ constexpr int NGROUPS = RWAVES * NGW;   // slot groups per workgroup
 __device__ void group_best(const SlotBest& mine, SlotBest* lds, SlotBest& out) {
     const int g = threadIdx.x / GL;
     if (glane() == 0) lds[g] = mine;
+    __syncthreads();
+    out = lds[0];
+    for (int i = 1; i < NGROUPS; i++)
+        if (out.beats(lds[i].best, lds[i].slot)) out = lds[i];
+}
+// the same when each group already keeps its record in lds[group]
+__device__ void group_best_lds(const SlotBest* lds, SlotBest& out) {
     __syncthreads();
     out = lds[0];
     for (int i = 1; i < NGROUPS; i++)
@@ -389,6 +373,8 @@ template <int P>
 __global__ __launch_bounds__(RTHREADS) void k_refine_gapless(DevChunk Q, DevChunk R, DevState S, Geom G,
                                                               int dinv_on) {
     __shared__ SlotBest lds[NGROUPS];
+    __shared__ float tbest[NGROUPS][12];
+    __shared__ typename GPrec<P>::T tlast[NGROUPS][16];
     __shared__ int ctr;
     const int p = blockIdx.x;
     const int qi = pair_q(S, p), ri = pair_r(S, p);
@@ -409,13 +395,16 @@ __global__ __launch_bounds__(RTHREADS) void k_refine_gapless(DevChunk Q, DevChun
     if (threadIdx.x == 0) ctr = 0;
     __syncthreads();
 
-    SlotBest mine;
-    mine.clear();
+    // per group: its best slot and the current slot's best transform, in LDS
+    const int grp = threadIdx.x / GL;
+    SlotBest& mine = lds[grp];
+    float* tfmBest = tbest[grp];
+    if (glane() == 0) mine.clear();
     unsigned long long ab = 0;
     bool need = anyslot, live = false;
     int si = 0, cit = 0, fqp = 0, frp = 0, fl = 0;
-    float tfm[12], tfmBest[12], thr = CP_LARGEDST, best = 0.f;
-    T last[15], nlast = 0;
+    float tfm[12], thr = CP_LARGEDST, best = 0.f;
+    T* last = tlast[grp];       // previous covariance record (15 sums, count)
     for (;;) {
         for (;;) {
             const int got = fetch_slot(need, &ctr);
@@ -436,7 +425,6 @@ __global__ __launch_bounds__(RTHREADS) void k_refine_gapless(DevChunk Q, DevChun
                         cit = 0;
                         thr = CP_LARGEDST;
                         best = 0.f;
-                        nlast = 0;
                         need = false;
                         live = true;
                     }
@@ -455,21 +443,25 @@ __global__ __launch_bounds__(RTHREADS) void k_refine_gapless(DevChunk Q, DevChun
             } else {
                 ncc = (T)gcc_gapless<P>(q, r, qpos, rpos, nall, tfm, true, (cit < 2) ? thr : d02s1, cc);
                 ab += AB_EXT * (unsigned long long)nall;
-                if (cc_converged<T>(cc, ncc, last, nlast)) done = true;
+                if (cc_converged<T>(cc, ncc, last, last[15])) done = true;
             }
             if (!done && ncc < (T)1) done = true;
             if (!done) {
-#pragma unroll
-                for (int f = 0; f < 15; f++) last[f] = cc[f];
-                nlast = ncc;
+                if (glane() == 0) {
+                    for (int f = 0; f < 15; f++) last[f] = cc[f];
+                    last[15] = ncc;
+                }
                 kabsch<T>(cc, ncc, dinv, tfm);
                 const float sc = gscore_gapless<P>(q, r, qpos, rpos, nall, tfm, d0, d02, d82, cit < 1, &thr);
                 ab += AB_TM * (unsigned long long)nall + AB_SLOT;
-                if (best < sc) { best = sc; copy12(tfmBest, tfm); }
+                if (best < sc) {
+                    best = sc;
+                    if (glane() == 0) copy12(tfmBest, tfm);
+                }
                 if (++cit == NMAXCONVIT + 2) done = true;
             }
             if (done) {
-                if (best > 0.f && mine.beats(best, si)) {
+                if (best > 0.f && mine.beats(best, si) && glane() == 0) {
                     mine.best = best; mine.slot = si; copy12(mine.tfm, tfmBest);
                 }
                 live = false;
@@ -479,7 +471,7 @@ __global__ __launch_bounds__(RTHREADS) void k_refine_gapless(DevChunk Q, DevChun
     }
     if (glane() == 0 && ab) atomicAdd(S.algbytes + KC_REFINE_GAPLESS, ab);
     SlotBest b;
-    group_best(mine, lds, b);
+    group_best_lds(lds, b);
     if (threadIdx.x == 0) {
         const bool have = b.best > 0.f;
         S.best0[p] = have ? b.best : 0.f;
@@ -763,8 +755,9 @@ __device__ __forceinline__ void gcc_dp_frag(const Aln& a, int L, int fragpos, in
         const float qx = a.qx[idx], qy = a.qy[idx], qz = a.qz[idx], rx = a.rx[idx], ry = a.ry[idx], rz = a.rz[idx];
         acc.add(qx, qy, qz, rx, ry, rz);
         if (sq) {
-            s2[0] += (T)(qx * qx); s2[1] += (T)(qy * qy); s2[2] += (T)(qz * qz);
-            s2[3] += (T)(rx * rx); s2[4] += (T)(ry * ry); s2[5] += (T)(rz * rz);
+            // squares formed in T: exact for T = double
+            s2[0] += (T)qx * (T)qx; s2[1] += (T)qy * (T)qy; s2[2] += (T)qz * (T)qz;
+            s2[3] += (T)rx * (T)rx; s2[4] += (T)ry * (T)ry; s2[5] += (T)rz * (T)rz;
         }
     }
     acc.reduce(cc);
@@ -819,6 +812,8 @@ __global__ __launch_bounds__(RTHREADS) void k_refine_dp(DevChunk Q, DevChunk R, 
                                                          RefineParams prm, int ldscap) {
     extern __shared__ float lds_rf[];
     __shared__ SlotBest slds[NGROUPS];
+    __shared__ float tbest[NGROUPS][12];
+    __shared__ typename GPrec<P>::T tlast[NGROUPS][16];
     __shared__ int ctr;
     typedef typename GPrec<P>::T T;
     const int p = blockIdx.x;
@@ -865,13 +860,16 @@ __global__ __launch_bounds__(RTHREADS) void k_refine_dp(DevChunk Q, DevChunk R, 
     const int base_ndx = phase2 ? S.sfragndx[p] : 0;
     const int base_pos = phase2 ? S.sfragpos[p] : 0;
 
-    SlotBest mine;
-    mine.clear();
+    // per group: its best slot and the current slot's best transform, in LDS
+    const int grp = threadIdx.x / GL;
+    SlotBest& mine = slds[grp];
+    float* tfmBest = tbest[grp];
+    if (glane() == 0) mine.clear();
     unsigned long long ab = 0;
     bool need = nslots > 0, live = false;
     int si = 0, cit = 0, fl = 0, sfragndx = 0, sfragpos = 0;
-    float tfm[12], tfmBest[12], thr = CP_LARGEDST, best = 0.f;
-    T last[15], nlast = 0;
+    float tfm[12], thr = CP_LARGEDST, best = 0.f;
+    T* last = tlast[grp];       // previous covariance record (15 sums, count)
     for (;;) {
         for (;;) {
             const int got = fetch_slot(need, &ctr);
@@ -898,11 +896,9 @@ __global__ __launch_bounds__(RTHREADS) void k_refine_dp(DevChunk Q, DevChunk R, 
                         thr = CP_LARGEDST;
                         best = bestinit;
 #pragma unroll
-                        for (int i = 0; i < 12; i++) {
-                            tfm[i] = 0.f;
-                            tfmBest[i] = (i == 0 || i == 4 || i == 8) ? 1.f : 0.f;
-                        }
-                        nlast = 0;
+                        for (int i = 0; i < 12; i++) tfm[i] = 0.f;
+                        if (glane() == 0)
+                            for (int i = 0; i < 12; i++) tfmBest[i] = (i == 0 || i == 4 || i == 8) ? 1.f : 0.f;
                         need = false;
                         live = true;
                     }
@@ -933,26 +929,32 @@ __global__ __launch_bounds__(RTHREADS) void k_refine_dp(DevChunk Q, DevChunk R, 
             } else {
                 ncc = (T)gcc_dp_ext<P>(a, L, tfm, (cit < 2) ? thr : d02s1, cc);
                 ab += AB_EXT * (unsigned long long)L;
-                if (cc_converged<T>(cc, ncc, last, nlast)) done = true;
+                if (cc_converged<T>(cc, ncc, last, last[15])) done = true;
             }
             if (!done && ncc < (T)1) done = true;
             if (!done) {
-#pragma unroll
-                for (int f = 0; f < 15; f++) last[f] = cc[f];
-                nlast = ncc;
+                if (glane() == 0) {
+                    for (int f = 0; f < 15; f++) last[f] = cc[f];
+                    last[15] = ncc;
+                }
                 kabsch<T>(cc, ncc, dinv, tfm);
                 const float sc = gscore_dp<P>(a, L, tfm, d0, d02, d82, chk, cit < 1, &thr);
                 ab += AB_TM * (unsigned long long)L + AB_SLOT;
-                if (best < sc) { best = sc; copy12(tfmBest, tfm); }
+                if (best < sc) {
+                    best = sc;
+                    if (glane() == 0) copy12(tfmBest, tfm);
+                }
                 if (++cit == NMAXCONVIT + 2) done = true;
             }
             if (done) {
                 float gbest = best;
                 if (prod && qlen != rlen) {
-                    gbest = gscore_dp<P>(a, L, tfmBest, g0, g02, d82, false, false, nullptr);
+                    float tb[12];
+                    copy12(tb, tfmBest);
+                    gbest = gscore_dp<P>(a, L, tb, g0, g02, d82, false, false, nullptr);
                     ab += AB_TM * (unsigned long long)L;
                 }
-                if (best > 0.f && mine.beats(best, si)) {
+                if (best > 0.f && mine.beats(best, si) && glane() == 0) {
                     mine.best = best; mine.slot = si; copy12(mine.tfm, tfmBest);
                     mine.gbest = gbest; mine.ndx = sfragndx; mine.pos = sfragpos;
                 }
@@ -961,9 +963,9 @@ __global__ __launch_bounds__(RTHREADS) void k_refine_dp(DevChunk Q, DevChunk R, 
             }
         }
     }
-    if (glane() == 0 && ab) atomicAdd(S.algbytes + KC_REFINE_DP, ab);
+    if (glane() == 0 && ab) atomicAdd(S.algbytes + (MODE == RM_DPREFINE ? KC_REFINE_DP : KC_REFINE_FIN), ab);
     SlotBest b;
-    group_best(mine, slds, b);
+    group_best_lds(slds, b);
     if (threadIdx.x != 0) return;
     const bool have = b.best > 0.f;
     float* tg = S.tfm_grand + (size_t)p * 12;

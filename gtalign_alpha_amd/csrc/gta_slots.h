@@ -92,32 +92,30 @@ struct GAccF32 {
         for (int i = 0; i < 15; i++) out[i] = gsum(s[i]);
     }
 };
-// compensated FP32 per lane (Kahan), combined over the row in FP64
-struct GAccKahan {
-    float s[15], c[15];
+// FP64 per lane: products of FP32 coordinates are exact in FP64 (24+24 < 53
+// significand bits) and the sums carry FP64 rounding, so the covariance and
+// the sums of squares that the RMSD's E0 identity differences are accurate to
+// ~1e-16 relative.  (Kahan-compensated FP32 sums still round every product to
+// FP32 first: on near-exact copies that leaves RMSD errors of several 1e-3 A.)
+struct GAccF64 {
+    double s[15];
     __device__ __forceinline__ void clear() {
 #pragma unroll
-        for (int i = 0; i < 15; i++) { s[i] = 0.f; c[i] = 0.f; }
-    }
-    __device__ __forceinline__ void k(int i, float v) {
-        const float y = __fsub_rn(v, c[i]);
-        const float t = __fadd_rn(s[i], y);
-        c[i] = __fsub_rn(__fsub_rn(t, s[i]), y);
-        s[i] = t;
+        for (int i = 0; i < 15; i++) s[i] = 0.0;
     }
     __device__ __forceinline__ void add(float qx, float qy, float qz, float rx, float ry, float rz) {
-        k(0, qx * rx); k(1, qx * ry); k(2, qx * rz);
-        k(3, qy * rx); k(4, qy * ry); k(5, qy * rz);
-        k(6, qz * rx); k(7, qz * ry); k(8, qz * rz);
-        k(9, qx); k(10, qy); k(11, qz);
-        k(12, rx); k(13, ry); k(14, rz);
+        const double x = qx, y = qy, z = qz, a = rx, b = ry, c = rz;
+        s[0] += x * a; s[1] += x * b; s[2] += x * c;
+        s[3] += y * a; s[4] += y * b; s[5] += y * c;
+        s[6] += z * a; s[7] += z * b; s[8] += z * c;
+        s[9] += x; s[10] += y; s[11] += z;
+        s[12] += a; s[13] += b; s[14] += c;
     }
     __device__ __forceinline__ void reduce(double* out) {
 #pragma unroll
-        for (int i = 0; i < 15; i++) out[i] = gsum((double)s[i] - (double)c[i]);
+        for (int i = 0; i < 15; i++) out[i] = gsum(s[i]);
     }
 };
-
 template <int P> struct GScore;
 template <> struct GScore<0> {
     float s;
@@ -138,7 +136,7 @@ template <> struct GScore<1> {
 
 template <int P> struct GPrec;
 template <> struct GPrec<0> { typedef float T; typedef GAccF32 Acc; };
-template <> struct GPrec<1> { typedef double T; typedef GAccKahan Acc; };
+template <> struct GPrec<1> { typedef double T; typedef GAccF64 Acc; };
 
 // Groups of this wave that need a slot take consecutive indices from the
 // workgroup's counter in LDS.  Called by the whole wave (convergent).

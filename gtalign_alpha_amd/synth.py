@@ -89,15 +89,67 @@ def make_set(seed, n, lens, queries, qres, hom_frac, sigma_lo, sigma_hi, del_hi,
     return structs, res, src
 
 
+def stress_chain(rng, L, straight_frac=0.4):
+    """A chain of 3-8 residue segments, a fraction straight_frac of them
+    straight (collinear CA atoms 3.8 A apart) and the rest ideal helix (rise
+    1.5 A, 100 deg per residue, radius 2.3 A), joined at random angles: most
+    superposition fragments of such a chain are near-collinear or near-planar."""
+    out = np.zeros((L, 3))
+    pos = np.zeros(3)
+    d = rng.normal(size=3)
+    d /= np.linalg.norm(d)
+    i = 0
+    while i < L:
+        n = min(int(rng.integers(3, 9)), L - i)
+        # a new direction at a random angle (30-110 deg) from the previous one
+        v = rng.normal(size=3)
+        v -= v.dot(d) * d
+        v /= np.linalg.norm(v)
+        ang = np.deg2rad(rng.uniform(30, 110))
+        d = np.cos(ang) * d + np.sin(ang) * v
+        d /= np.linalg.norm(d)
+        if rng.random() < straight_frac:
+            for k in range(n):
+                pos = pos + 3.8 * d
+                out[i + k] = pos
+        else:
+            u = np.cross(d, v)
+            u /= np.linalg.norm(u)
+            c0 = pos - 2.3 * v
+            for k in range(n):
+                t = np.deg2rad(100.0) * (k + 1)
+                out[i + k] = c0 + 2.3 * (np.cos(t) * v + np.sin(t) * u) + 1.5 * (k + 1) * d
+            pos = out[i + n - 1]
+        i += n
+    return (out - out.mean(0)).astype(np.float32)
+
+
+def rigid_copy(rng, x, sigma, maxtrans=50.0):
+    """x under a random rotation and translation with Gaussian noise sigma."""
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    a, b, c, e = q
+    R = np.array([[a * a + b * b - c * c - e * e, 2 * (b * c - a * e), 2 * (b * e + a * c)],
+                  [2 * (b * c + a * e), a * a - b * b + c * c - e * e, 2 * (c * e - a * b)],
+                  [2 * (b * e - a * c), 2 * (c * e + a * b), a * a - b * b - c * c + e * e]])
+    t = rng.uniform(-maxtrans, maxtrans, 3)
+    y = np.asarray(x, np.float64) @ R.T + t + rng.normal(scale=sigma, size=x.shape)
+    return y.astype(np.float32)
+
+
 def config(index, scale=1.0, seed=None):
     """Query and reference chunks of one benchmark configuration.
 
     index 0: 1 query of 150 vs 256 refs U(50,300), 30% homologs (sigma 1, 0-10% deletions)
     index 1: 16 queries x 50,000 refs, lognormal(250, 0.5) in [30,1000], 20% homologs
     index 2: 1 query of 400 x 500,000 refs, lognormal(220, 0.7) in [20,4000], 10% homologs
+    index 3: precision stress, 20 query chains of straight and helical 3-8 residue
+             segments (lengths 60-300) x 2,000 references; reference j is a copy of
+             query j mod 20 under a random rigid motion with noise sigma 0.01-0.1 A
     index 4: all-vs-all family set, lognormal(200, 0.5) in [40,800] (chunked by the caller)
     `scale` shrinks the number of references (tests and bounded CPU samples).
-    Returns (queries, references, homolog-source index per reference).
+    Returns (queries, references, src): src[j] is the query-chunk position of
+    the query that reference-chunk position j is a homolog (copy) of, or -1.
     """
     seed = BASE_SEED + index if seed is None else seed
     rng = np.random.default_rng(seed)
@@ -116,11 +168,29 @@ def config(index, scale=1.0, seed=None):
         qlens = np.array([400])
         rlens = lognormal_lengths(rng, nr, 220, 0.7, 20, 4000)
         hom, slo, shi, dhi, ihi = 0.1, 0.5, 3.0, 0.075, 0.075
+    elif index == 3:
+        nq, nr = 20, max(1, int(round(2000 * scale)))
+        qs = [stress_chain(rng, int(L)) for L in rng.integers(60, 301, nq)]
+        qr = [residues(rng, len(x)) for x in qs]
+        rs, rr, src = [], [], np.zeros(nr, dtype=np.int64)
+        for j in range(nr):
+            src[j] = j % nq
+            rs.append(rigid_copy(rng, qs[src[j]], rng.uniform(0.01, 0.1)))
+            rr.append(qr[src[j]])
+        return _chunks(qs, qr, rs, rr, src)
     else:
         raise ValueError("unsupported configuration %d" % index)
     qs = [chain(int(rng.integers(0, 2**62)), int(L)) for L in qlens]
     qr = [residues(rng, int(L)) for L in qlens]
     rs, rr, src = make_set(int(rng.integers(0, 2**62)), nr, rlens, qs, qr, hom, slo, shi, dhi, ihi)
+    return _chunks(qs, qr, rs, rr, src)
+
+
+def _chunks(qs, qr, rs, rr, src):
+    """Chunks sorted by length, with src mapped to chunk positions."""
     qc = Chunk.from_structures(qs, qr)
     rc = Chunk.from_structures(rs, rr)
-    return qc, rc, src[rc.order]
+    qpos = np.full(len(qs) + 1, -1, dtype=np.int64)
+    qpos[qc.order] = np.arange(len(qs))
+    s = np.asarray(src)[rc.order]
+    return qc, rc, np.where(s >= 0, qpos[s], -1)

@@ -1,4 +1,6 @@
-"""Shared helpers of the tests (FP64 restatements used as checks)."""
+"""FP64 restatements used as checks (TEST INFRASTRUCTURE ONLY): rigid motions,
+SVD superposition, TM-score from an alignment and a transform, and readers of
+the alignment section.  Used by tests/ and bench.py's CPU-baseline leg."""
 import numpy as np
 
 from gtalign_alpha_amd.capi import ALN_BEGCOORDS, ALN_ALNLENGTH
@@ -75,3 +77,25 @@ def tm_fp64(q, r, qi, ri, pairs, tfm, L):
     d2 = ((tfm_apply(tfm, qx) - rx) ** 2).sum(1)
     d0 = d0_tm(L)
     return float((d0 * d0 / (d0 * d0 + d2)).sum() / L)
+
+
+def fp64_errors(res, q, r):
+    """Per aligned pair of a result: |reported TM - FP64 TM| (both
+    normalisations, under the reported transform) and |reported RMSD - FP64
+    RMSD| (SVD superposition of the aligned pairs).  Returns two arrays."""
+    from gtalign_alpha_amd.capi import ALN_RMSD, ALN_SCOREQ, ALN_SCORER
+    dtm, drm = [], []
+    for qi in range(q.nstrs):
+        for ri in range(r.nstrs):
+            a = res.alndata[qi, ri]
+            if a[ALN_RMSD] >= 9999:
+                continue
+            pairs = matched_positions(alignment_rows(res.alns, q, r, qi, ri), a)
+            tq = tm_fp64(q, r, qi, ri, pairs, res.tfms[qi, ri], int(q.len[qi]))
+            tr = tm_fp64(q, r, qi, ri, pairs, res.tfms[qi, ri], int(r.len[ri]))
+            dtm.append(max(abs(tq - a[ALN_SCOREQ]), abs(tr - a[ALN_SCORER])))
+            x, y = q.coords(qi)[pairs[:, 0]], r.coords(ri)[pairs[:, 1]]
+            R, t = kabsch_svd(x, y)
+            rms = float(np.sqrt(((x @ R.T + t - y) ** 2).sum(1).mean()))
+            drm.append(abs(rms - a[ALN_RMSD]))
+    return np.array(dtm), np.array(drm)

@@ -42,6 +42,6 @@ if __name__ == "__main__":
     qc, rc, src = synth.config(1, scale=0.01)
     # two queries; 40 references, 20 of them homologs of the chosen queries
     qsel = [2, 9]
-    hom = [i for i in range(rc.nstrs) if src[i] in qc.order[qsel]][:20]
+    hom = [i for i in range(rc.nstrs) if src[i] in qsel][:20]
     other = [i for i in range(rc.nstrs) if src[i] < 0][:40 - len(hom)]
     write("config2_2x40", qc.subset(qsel), rc.subset(sorted(hom + other, key=lambda i: -rc.len[i])))

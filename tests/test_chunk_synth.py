@@ -4,7 +4,7 @@ import pytest
 
 from gtalign_alpha_amd import synth
 from gtalign_alpha_amd.chunk import Chunk, MAX_STRUCTURE_LENGTH
-from tests.util import kabsch_svd
+from oracle.fp64 import kabsch_svd
 
 
 def test_chunk_sorted_and_prefix_offsets():

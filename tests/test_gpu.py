@@ -17,7 +17,7 @@ from gtalign_alpha_amd.capi import (default_options, PREC_FP32, PREC_MIXED, STAT
                                     ALN_RMSD, ALN_SCOREQ, ALN_SCORER, ALN_D0Q, ALN_D0R)
 from gtalign_alpha_amd.chunk import Chunk
 from oracle import oracle
-from tests.util import alignment_rows, kabsch_svd, matched_positions, random_rotation, tm_fp64
+from oracle.fp64 import alignment_rows, kabsch_svd, matched_positions, random_rotation, tm_fp64
 
 pytestmark = pytest.mark.gpu
 
@@ -120,34 +120,50 @@ def test_stage_matches_reference_golden(path):
     assert np.abs(g.tfms - d["tfms"])[aligned].max() < 1e-3
 
 
-@pytest.mark.parametrize("prec", [PREC_FP32, PREC_MIXED], ids=["fp32", "mixed"])
-def test_reported_tm_equals_fp64_recomputation(prec):
-    """Size-independent check: the reported TM-scores equal an FP64 recomputation
-    from the output alignment and transform (d0 of each normalising length), and
-    the RMSD an FP64 superposition of the aligned pairs."""
-    qc, rc, _ = synth.config(1, scale=0.004)
-    q = qc.subset([1, 8])
-    g = run_gpu(q, rc, prec)
+def check_fp64(g, q, r, prec):
+    """Every reported TM-score equals an FP64 recomputation from the output
+    alignment and transform (d0 of each normalising length), and the RMSD an
+    FP64 superposition of the aligned pairs (the FP32 form keeps the
+    reference's E0 cancellation, a few hundredths of an angstrom near 0).
+    Returns the number of aligned pairs checked."""
     n = 0
     for qi in range(q.nstrs):
-        for ri in range(rc.nstrs):
+        for ri in range(r.nstrs):
             a = g.alndata[qi, ri]
             if a[ALN_RMSD] >= 9999:
                 continue
-            rows = alignment_rows(g.alns, q, rc, qi, ri)
+            rows = alignment_rows(g.alns, q, r, qi, ri)
             pairs = matched_positions(rows, a)
-            tq = tm_fp64(q, rc, qi, ri, pairs, g.tfms[qi, ri], int(q.len[qi]))
-            tr = tm_fp64(q, rc, qi, ri, pairs, g.tfms[qi, ri], int(rc.len[ri]))
-            assert abs(tq - a[ALN_SCOREQ]) < 1e-4, (qi, ri)
-            assert abs(tr - a[ALN_SCORER]) < 1e-4, (qi, ri)
-            # RMSD of the alignment after its optimal superposition (FP64 SVD);
-            # the FP32 form keeps the reference's E0 cancellation (<~0.05 A)
-            x, y = q.coords(qi)[pairs[:, 0]], rc.coords(ri)[pairs[:, 1]]
+            tq = tm_fp64(q, r, qi, ri, pairs, g.tfms[qi, ri], int(q.len[qi]))
+            tr = tm_fp64(q, r, qi, ri, pairs, g.tfms[qi, ri], int(r.len[ri]))
+            assert abs(tq - a[ALN_SCOREQ]) < 1e-4, (qi, ri, tq, a[ALN_SCOREQ])
+            assert abs(tr - a[ALN_SCORER]) < 1e-4, (qi, ri, tr, a[ALN_SCORER])
+            x, y = q.coords(qi)[pairs[:, 0]], r.coords(ri)[pairs[:, 1]]
             R, t = kabsch_svd(x, y)
             rms = float(np.sqrt(((x @ R.T + t - y) ** 2).sum(1).mean()))
             assert abs(rms - a[ALN_RMSD]) < (1e-3 if prec == PREC_MIXED else 0.1), (qi, ri, rms)
             n += 1
-    assert n >= 5
+    return n
+
+
+@pytest.mark.parametrize("prec", [PREC_FP32, PREC_MIXED], ids=["fp32", "mixed"])
+def test_reported_tm_equals_fp64_recomputation(prec):
+    """Size-independent check against FP64 recomputations."""
+    qc, rc, _ = synth.config(1, scale=0.004)
+    q = qc.subset([1, 8])
+    assert check_fp64(run_gpu(q, rc, prec), q, rc, prec) >= 5
+
+
+def test_precision_stress_fp64():
+    """Chains of straight and helical 3-8 residue segments (near-collinear and
+    near-planar fragments) against their rigidly moved copies with 0.01-0.1 A
+    noise: the mixed form's TM-scores and RMSDs equal FP64 recomputations
+    (1e-4, 1e-3 A); every copy is found."""
+    qc, rc, src = synth.config(3, scale=0.05)
+    g = run_gpu(qc, rc, PREC_MIXED)
+    for ri in range(rc.nstrs):
+        assert g.alndata[src[ri], ri, ALN_SCOREQ] > 0.99, ri
+    assert check_fp64(g, qc, rc, PREC_MIXED) >= rc.nstrs
 
 
 def test_short_chains_and_copies():
@@ -159,7 +175,11 @@ def test_short_chains_and_copies():
     r = Chunk.from_structures(rs, [synth.residues(rng, len(s)) for s in rs])
     o = oracle.process_block(q, r)
     assert_same(o, run_gpu(q, r, PREC_FP32), q, r)
-    assert_close_mixed(o, run_gpu(q, r, PREC_MIXED), q, r)
+    # 3-5 residue alignments are where the FP32 solve is least accurate: the
+    # mixed form is held to the FP64 recomputation and to the same aligned set
+    g = run_gpu(q, r, PREC_MIXED)
+    assert np.array_equal(g.alndata[..., ALN_RMSD] < 9999, o.alndata[..., ALN_RMSD] < 9999)
+    assert check_fp64(g, q, r, PREC_MIXED) >= 8
 
 
 def test_long_reference_row_buffer_path():

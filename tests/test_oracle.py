@@ -8,7 +8,7 @@ import pytest
 from gtalign_alpha_amd import synth
 from gtalign_alpha_amd.capi import default_options
 from oracle import oracle
-from tests.util import random_rotation, kabsch_svd, tfm_apply, d0_tm
+from oracle.fp64 import random_rotation, kabsch_svd, tfm_apply, d0_tm
 
 
 # ---- Kabsch (transformbase.h:167-583) ---------------------------------------------
