@@ -45,8 +45,8 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def time_form(qc, rc, prec, steps, warmup, profile):
-    opts = default_options(precision=prec)
+def time_form(qc, rc, prec, steps, warmup, profile, batched=0):
+    opts = default_options(precision=prec, batched=batched)
     with GtaBatch(0, opts, max_qrys=qc.nstrs, max_qry_poss=qc.nposs, max_refs=rc.nstrs,
                   max_ref_poss=rc.nposs) as b:
         b.TransferQueryPMDataToDevice(qc)
@@ -179,7 +179,10 @@ def cpu_baseline(qc, rc, src):
     return out
 
 
-def precision_stress():
+BASE_SEED_ASM = 20261206
+
+
+def precision_stress(assemblies=True):
     """Near-collinear/near-planar chains against their noisy rigid copies
     (20 queries x 2,000 references, 2,000 copy pairs): max/mean error of the
     reported TM-score and RMSD against FP64 recomputations, plain and mixed."""
@@ -197,6 +200,67 @@ def precision_stress():
         out[name] = {"copy_pairs_aligned": int(len(dtm)), "tm_err_max": float(dtm.max()),
                      "tm_err_mean": float(dtm.mean()), "rmsd_err_max": float(drm.max()),
                      "rmsd_err_mean": float(drm.mean())}
+    if assemblies:
+        # two assemblies of 150 domains (37,860 residues), the second a rigid copy
+        # of the first with 0.05 A noise: one pair, filled by 16-wave DP, its
+        # searches split over workgroups
+        from gtalign_alpha_amd.chunk import Chunk
+        rng = np.random.default_rng(BASE_SEED_ASM)
+        a = synth.assembly(rng)
+        b = synth.rigid_copy(rng, a, 0.05)
+        q = Chunk.from_structures([a], ["A" * len(a)])
+        r = Chunk.from_structures([b], ["A" * len(b)])
+        asm = {"residues": int(len(a))}
+        for name, prec in (("plain", PREC_FP32), ("mixed", PREC_MIXED)):
+            t0 = time.perf_counter()
+            g = _gpu_block(q, r, prec)
+            dt = time.perf_counter() - t0
+            dtm, drm = fp64_errors(g, q, r)
+            asm[name] = {"seconds": round(dt, 2), "tm_q": float(g.alndata[0, 0, 9]),
+                         "rmsd": float(g.alndata[0, 0, 8]),
+                         "tm_err": float(dtm.max()) if len(dtm) else None,
+                         "rmsd_err": float(drm.max()) if len(drm) else None}
+            log("  assembly pair, %s: %.1f s, %s" % (name, dt, asm[name]))
+        out["assemblies"] = asm
+    return out
+
+
+def copy_calibration(mib):
+    """One device-to-device copy of mib MiB (the runtime's copy kernel): its
+    FETCH_SIZE / WRITE_SIZE counters give the scale of those counters."""
+    hip = C.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(n)) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0      # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_bytes": n, "copies": 3}), flush=True)
+
+
+def batched_vs_plain(steps, warmup):
+    """1 query of 400 x 500,000 references (lognormal(220, 0.7) in [20, 4000],
+    10% homologs) in one chunk: pairs/s of the plain FP32 form against the
+    batched form (device-built live-pair lists, persistent grids, the launch
+    sequence as one HIP graph) in FP32 and mixed precision."""
+    t0 = time.perf_counter()
+    qc, rc, _ = synth.config(2)
+    npairs = qc.nstrs * rc.nstrs
+    log("batched vs plain: %d x %d pairs (%d + %d residues), generated in %.1f s" %
+        (qc.nstrs, rc.nstrs, qc.nposs, rc.nposs, time.perf_counter() - t0))
+    out = {"workload": "1 query of 400 x %d references, lognormal(220,0.7) in [20,4000], 10%% homologs"
+                       % rc.nstrs, "pairs": npairs}
+    for name, prec, batched in (("plain_fp32", PREC_FP32, 0), ("batched_fp32", PREC_FP32, 1),
+                                ("batched_mixed", PREC_MIXED, 1)):
+        log(" " + name)
+        wall, dev, cells, _ = time_form(qc, rc, prec, steps, warmup, False, batched)
+        out[name] = {"pairs_per_s": round(npairs * steps / wall, 1), "ms_per_step": round(1e3 * wall / steps, 2),
+                     "dp_gcups": round(cells / dev / 1e9, 2)}
+    out["batched_mixed_vs_plain"] = round(out["batched_mixed"]["pairs_per_s"] / out["plain_fp32"]["pairs_per_s"], 3)
+    out["batched_fp32_vs_plain"] = round(out["batched_fp32"]["pairs_per_s"] / out["plain_fp32"]["pairs_per_s"], 3)
     return out
 
 
@@ -209,7 +273,17 @@ def main():
                     help="fraction of the 50,000 reference chains (default: the full workload)")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-stress", action="store_true", help="skip the precision-stress section")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the plain-vs-batched comparison on 1 x 500,000 pairs")
+    ap.add_argument("--no-assemblies", action="store_true",
+                    help="skip the 37,860-residue assembly pair of the precision-stress section")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU: the stage's pairs shard across GPUs with no exchange "
@@ -257,8 +331,14 @@ def main():
                 "launch_ms": round(1e3 * dom["seconds"] / dom["launches"], 3),
                 "alg_bytes_per_launch": dom["alg_bytes"] / dom["launches"]}
 
-    log("precision stress")
-    stress = precision_stress()
+    batched = None
+    if not args.no_batched:
+        batched = batched_vs_plain(max(1, min(args.steps, 2)), 1)
+
+    stress = None
+    if not args.no_stress:
+        log("precision stress")
+        stress = precision_stress(not args.no_assemblies)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -285,6 +365,7 @@ def main():
         "mixed_precision_pairs_per_s": round(mixed, 1) if mixed else None,
         "kernels": kern,
         "roofline": roofline,
+        "batched_vs_plain": batched,
         "precision_stress": stress,
         "cpu_baseline": cpu,
     }

@@ -62,6 +62,10 @@ def lib():
         L.gta_test_dp.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_float, C.c_int, C.c_int,
                                   C.c_void_p, C.c_void_p, C.c_void_p]
         L.gta_test_dp.restype = C.c_int
+        L.gta_test_set_dp_buckets.argtypes = [C.c_void_p, C.c_longlong, C.c_int]
+        L.gta_test_set_dp_buckets.restype = C.c_int
+        L.gta_test_set_split.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
+        L.gta_test_set_split.restype = C.c_int
         _lib = L
     return _lib
 
@@ -145,6 +149,17 @@ class GtaBatch:
                 o = qi * self._r.nposs + int(self._r.dst[ri])
                 out[qi, ri] = np.stack([qp[o:o + k], rp[o:o + k]], axis=1)
         return out
+
+    def set_dp_buckets(self, min_cells, min_strips):
+        """Thresholds of the multi-wave DP bucket (testing hook,
+        include/gta/gtalign_amd_testing.h)."""
+        self._check(lib().gta_test_set_dp_buckets(self._h, int(min_cells), int(min_strips)))
+
+    def set_split(self, min_len, seeds_per_part=256, slots_per_part=32):
+        """Workgroup split of the superposition searches for long pairs
+        (testing hook, include/gta/gtalign_amd_testing.h)."""
+        self._check(lib().gta_test_set_split(self._h, int(min_len), int(seeds_per_part),
+                                             int(slots_per_part)))
 
     def close(self):
         if getattr(self, "_h", None):

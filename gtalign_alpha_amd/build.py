@@ -43,7 +43,7 @@ def build_hip(out=LIB):
 def build_synth(out=SYNTH):
     cxx = shutil.which("g++") or "g++"
     tmp = out + ".tmp"
-    _run([cxx, "-std=c++17", "-O2", "-fPIC", "-shared", os.path.join(CSRC, "synth.cpp"), "-o", tmp])
+    _run([cxx, "-std=c++17", "-O2", "-fPIC", "-shared", "-fopenmp", os.path.join(CSRC, "synth.cpp"), "-o", tmp])
     os.replace(tmp, out)
 
 

@@ -67,6 +67,9 @@ int GetMaxNFragSteps(int maxalnlen, int step, int fraglen) {
 
 }  // namespace
 
+// work-queue counters of the batched form (one per persistent launch)
+#define GTA_MAX_QUEUES 128
+
 struct gta_ctx {
     int device = 0;
     gta_options opts;
@@ -79,6 +82,25 @@ struct gta_ctx {
     // pair state
     DevBuf s_flags, s_grand, s_best0, s_prev, s_naln, s_qpos, s_rpos, s_fndx, s_fpos;
     DevBuf s_tibest, s_tgrand, s_out, s_aln, s_btck, s_btckoff, s_rowbuf, s_cells, s_alns, s_algb;
+    // DP size buckets: pairs filled by W waves (and their ring rows)
+    DevBuf s_dpsmall, s_dpbig, s_ring, s_ringoff, s_fail;
+    DpBuckets dpb{};
+    // a pair is filled by several waves when Lq*Lr >= mw_min_cells and it
+    // has at least mw_min_strips 64-row strips (and Lr >= 256)
+    long long mw_min_cells = 1ll << 22;
+    int mw_min_strips = 16;
+    // refinement split: pairs whose shorter structure has at least
+    // split_min_len residues spread their slots over several workgroups
+    DevBuf s_rsmall, s_rbig, s_parts;
+    RefSplit rsp{};
+    int split_min_len = 2048;
+    int split_seeds = 256, split_slots = 32;
+    // batched form: live-pair work lists, queue counters, the captured graph
+    DevBuf s_bigmask, s_lista, s_listb, s_lcnt, s_ltot, s_heads;
+    const unsigned char* bigmask = nullptr;
+    int ncu = 256;
+    hipGraphExec_t gexec = nullptr;
+    bool gvalid = false;
     // measurement
     int profiling = 0;
     std::vector<hipEvent_t> evpool;
@@ -116,6 +138,7 @@ void gta_default_options(gta_options* o) {
     o->nodeletions = 0;
     o->referenced = 0;
     o->precision = GTA_PREC_FP32;
+    o->batched = 0;
 }
 
 size_t gta_alignment_section_size(int nq, int nqposs, int nr, int nrposs) {
@@ -133,6 +156,7 @@ static int check_options(const gta_options* o) {
     if (o->depth < 0 || o->depth > GTA_DEPTH_SHALLOW) return 0;
     if (o->refinement < 0 || o->refinement > GTA_REF_FULLA) return 0;
     if (o->precision != GTA_PREC_FP32 && o->precision != GTA_PREC_MIXED) return 0;
+    if (o->batched != 0 && o->batched != 1) return 0;
     return 1;
 }
 
@@ -161,6 +185,9 @@ int gta_create(int device, const gta_options* opts, int max_qrys, int max_qry_po
     int lds = 0;
     if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
         c->lds_max = (size_t)lds;
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu > 0)
+        c->ncu = ncu;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
         delete c;
@@ -178,7 +205,10 @@ void gta_destroy(gta_ctx* c) {
     DevBuf* bufs[] = {&c->s_flags, &c->s_grand, &c->s_best0, &c->s_prev, &c->s_naln, &c->s_qpos,
                       &c->s_rpos, &c->s_fndx, &c->s_fpos, &c->s_tibest, &c->s_tgrand, &c->s_out,
                       &c->s_aln, &c->s_btck, &c->s_btckoff, &c->s_rowbuf, &c->s_cells, &c->s_alns,
-                      &c->s_algb};
+                      &c->s_algb, &c->s_dpsmall, &c->s_dpbig, &c->s_ring, &c->s_ringoff, &c->s_fail,
+                      &c->s_rsmall, &c->s_rbig, &c->s_parts, &c->s_bigmask, &c->s_lista, &c->s_listb,
+                      &c->s_lcnt, &c->s_ltot, &c->s_heads};
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     for (DevBuf* b : bufs) b->release();
     for (hipEvent_t e : c->evpool) (void)hipEventDestroy(e);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -233,6 +263,7 @@ int gta_set_queries(gta_ctx* c, const gta_chunk_view* q) {
     if (q && q->nstrs > 100) return fail(c, GTA_ERR_TOOMANY, "more than 100 queries per chunk");
     int rc = upload_chunk(c, c->q, q, c->max_qrys, c->max_qposs);
     c->have_q = (rc == GTA_OK);
+    c->gvalid = false;
     return rc;
 }
 
@@ -240,6 +271,7 @@ int gta_set_references(gta_ctx* c, const gta_chunk_view* r) {
     if (!c) return fail(nullptr, GTA_ERR_ARG, "null context");
     int rc = upload_chunk(c, c->r, r, c->max_refs, c->max_rposs);
     c->have_r = (rc == GTA_OK);
+    c->gvalid = false;
     return rc;
 }
 
@@ -297,8 +329,110 @@ static int prepare_state(gta_ctx* c, DevState& S, size_t& alnsize) {
     HIPCHK(c, c->s_btckoff.ensure(np * 8));
     HIPCHK(c, hipMemcpyAsync(c->s_btckoff.p, off.data(), np * 8, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, c->s_rowbuf.ensure(alnplane * 4));
+    // DP size buckets
+    {
+        std::vector<int> small, big;
+        std::vector<unsigned long long> roff;
+        unsigned long long ringtot = 0;
+        int minstrips = 1 << 30;
+        for (int qi = 0; qi < nq; qi++) {
+            const int Lq = c->q.len[qi], ns = (Lq + 63) / 64;
+            for (int ri = 0; ri < nr; ri++) {
+                const int Lr = c->r.len[ri];
+                const int pidx = qi * nr + ri;
+                if ((long long)Lq * Lr >= c->mw_min_cells && ns >= c->mw_min_strips && Lr >= 256) {
+                    big.push_back(pidx);
+                    minstrips = std::min(minstrips, ns);
+                } else {
+                    small.push_back(pidx);
+                }
+            }
+        }
+        DpBuckets B{};
+        if (!big.empty()) {
+            B.waves = minstrips >= 32 ? 16 : (minstrips >= 16 ? 8 : 4);
+            for (int pidx : big) {
+                roff.push_back(ringtot);
+                ringtot += (unsigned long long)B.waves * (unsigned long long)c->r.len[pidx % nr];
+            }
+            HIPCHK(c, c->s_dpsmall.ensure(std::max<size_t>(small.size(), 1) * 4));
+            HIPCHK(c, c->s_dpbig.ensure(big.size() * 4));
+            HIPCHK(c, c->s_ringoff.ensure(big.size() * 8));
+            HIPCHK(c, c->s_ring.ensure(ringtot * 4));
+            if (!small.empty())
+                HIPCHK(c, hipMemcpy(c->s_dpsmall.p, small.data(), small.size() * 4, hipMemcpyHostToDevice));
+            HIPCHK(c, hipMemcpy(c->s_dpbig.p, big.data(), big.size() * 4, hipMemcpyHostToDevice));
+            HIPCHK(c, hipMemcpy(c->s_ringoff.p, roff.data(), roff.size() * 8, hipMemcpyHostToDevice));
+            B.small = c->s_dpsmall.as<int>();
+            B.nsmall = (int)small.size();
+            B.big = c->s_dpbig.as<int>();
+            B.nbig = (int)big.size();
+            B.ring = c->s_ring.as<float>();
+            B.ringoff = c->s_ringoff.as<unsigned long long>();
+        }
+        HIPCHK(c, c->s_fail.ensure(4));
+        HIPCHK(c, hipMemset(c->s_fail.p, 0, 4));
+        B.failflag = c->s_fail.as<int>();
+        c->dpb = B;
+    }
+    // refinement split
+    {
+        std::vector<int> small, big;
+        for (int qi = 0; qi < nq; qi++)
+            for (int ri = 0; ri < nr; ri++) {
+                const int pidx = qi * nr + ri;
+                if (std::min(c->q.len[qi], c->r.len[ri]) >= c->split_min_len) big.push_back(pidx);
+                else small.push_back(pidx);
+            }
+        RefSplit Sp{};
+        Sp.seeds_per_part = c->split_seeds;
+        Sp.slots_per_part = c->split_slots;
+        if (!big.empty()) {
+            const Geom G = make_geom(c);
+            const int maxparts = std::max({(G.nseeds + c->split_seeds - 1) / c->split_seeds,
+                                           (G.nlocsteps + c->split_slots - 1) / c->split_slots,
+                                           (G.maxnfragfcts + c->split_slots - 1) / c->split_slots, 1});
+            HIPCHK(c, c->s_rsmall.ensure(std::max<size_t>(small.size(), 1) * 4));
+            HIPCHK(c, c->s_rbig.ensure(big.size() * 4));
+            HIPCHK(c, c->s_parts.ensure(big.size() * (size_t)maxparts * part_record_bytes()));
+            if (!small.empty())
+                HIPCHK(c, hipMemcpy(c->s_rsmall.p, small.data(), small.size() * 4, hipMemcpyHostToDevice));
+            HIPCHK(c, hipMemcpy(c->s_rbig.p, big.data(), big.size() * 4, hipMemcpyHostToDevice));
+            Sp.small = c->s_rsmall.as<int>();
+            Sp.nsmall = (int)small.size();
+            Sp.big = c->s_rbig.as<int>();
+            Sp.nbig = (int)big.size();
+            Sp.partbuf = c->s_parts.p;
+        }
+        c->rsp = Sp;
+    }
+    // batched form: per-pair bucket marks and the live-pair lists
+    if (c->opts.batched) {
+        c->bigmask = nullptr;
+        if (c->dpb.nbig > 0 || c->rsp.nbig > 0) {
+            std::vector<unsigned char> m(np, 0);
+            if (c->dpb.nbig > 0) {
+                std::vector<int> big(c->dpb.nbig);
+                HIPCHK(c, hipMemcpy(big.data(), c->dpb.big, big.size() * 4, hipMemcpyDeviceToHost));
+                for (int pidx : big) m[pidx] |= 1;
+            }
+            if (c->rsp.nbig > 0) {
+                std::vector<int> big(c->rsp.nbig);
+                HIPCHK(c, hipMemcpy(big.data(), c->rsp.big, big.size() * 4, hipMemcpyDeviceToHost));
+                for (int pidx : big) m[pidx] |= 2;
+            }
+            HIPCHK(c, c->s_bigmask.ensure(np));
+            HIPCHK(c, hipMemcpy(c->s_bigmask.p, m.data(), np, hipMemcpyHostToDevice));
+            c->bigmask = c->s_bigmask.as<unsigned char>();
+        }
+        HIPCHK(c, c->s_lista.ensure(np * 4));
+        HIPCHK(c, c->s_listb.ensure(np * 4));
+        HIPCHK(c, c->s_lcnt.ensure(((np + 1023) / 1024) * 8));
+        HIPCHK(c, c->s_ltot.ensure(8));
+        HIPCHK(c, c->s_heads.ensure(GTA_MAX_QUEUES * 4));
+    }
     alnsize = gta_alignment_section_size(nq, c->q.nposs, nr, c->r.nposs);
-    HIPCHK(c, c->s_alns.ensure(alnsize));
+    HIPCHK(c, c->s_alns.ensure((alnsize + 3) & ~(size_t)3));
 
     S.nq = nq; S.nr = nr;
     S.flags = c->s_flags.as<int>(); S.grand = c->s_grand.as<float>(); S.best0 = c->s_best0.as<float>();
@@ -364,14 +498,32 @@ static int run_stage(gta_ctx* c, double* seconds, double* cells) {
     } while (0)
     int maxaln = std::min(c->q.len.front(), c->r.len.front());
     RefineLaunch RL{dinv, 0, 0, 0, 0.f, std::min(maxaln, 1536)};
+    // batched form: after each filter the live pairs are listed on the device
+    // and the launches that follow are persistent grids draining those lists
+    const bool batched = o.batched != 0;
+    bool listed = false;
+    int nqueue = 0;
+    int* lista = batched ? c->s_lista.as<int>() : nullptr;
+    int* listb = batched ? c->s_listb.as<int>() : nullptr;
+    int* ltot = batched ? c->s_ltot.as<int>() : nullptr;
+    int* heads = batched ? c->s_heads.as<int>() : nullptr;
+    const int pgrid_dp = c->ncu * 32, pgrid_rf = c->ncu * 4;
+    auto queue = [&](const int* items, const int* count) -> WorkQ {
+        if (!listed || nqueue >= GTA_MAX_QUEUES) return WorkQ{};
+        return WorkQ{items, count, heads + nqueue++};
+    };
+    auto relist = [&]() -> hipError_t {
+        listed = true;
+        return launch_live_lists(S, c->bigmask, c->s_lcnt.as<int>(), ltot, lista, listb, st);
+    };
     auto dp = [&](int mode, int gap0, float gapopen, int usess, int dpscan) -> hipError_t {
         DpLaunch L{mode, gap0, gapopen, usess, usess ? 0.5f : 0.0f, dpscan};
-        return launch_dp(Q, R, S, L, st);
+        return launch_dp(Q, R, S, L, c->dpb, queue(lista, ltot), pgrid_dp, st);
     };
     auto refine = [&](int mode, int epi, int dpi, int maxdpi, float prescore) -> hipError_t {
         RefineLaunch L = RL;
         L.epi = epi; L.dpi = dpi; L.maxndpiters = maxdpi; L.prescore = prescore;
-        return launch_refine_dp(prec, mode, Q, R, S, G, L, st);
+        return launch_refine_dp(prec, mode, Q, R, S, G, L, c->rsp, queue(listb, ltot + 1), pgrid_rf, st);
     };
     // DPRefine<GAP0,PRESCREEN> (MpStage1.cpp:455-510)
     auto dprefine = [&](int gap0, float prescore, int maxdpi) -> int {
@@ -382,41 +534,87 @@ static int run_stage(gta_ctx* c, double* seconds, double* cells) {
         return GTA_OK;
     };
 
-    HIPCHK(c, hipMemsetAsync(c->s_cells.p, 0, 8, st));
-    HIPCHK(c, hipMemsetAsync(c->s_algb.p, 0, KC_COUNT * 8, st));
-    HIPCHK(c, hipMemsetAsync(c->s_alns.p, 0, alnsize, st));
-    HIPCHK(c, hipEventRecord(c->ev0, st));
-    GTA_TIMED(KC_SECSTR, launch_secstr(Q, st));
-    GTA_TIMED(KC_SECSTR, launch_secstr(R, st));
-    GTA_TIMED(KC_INIT, launch_init(Q, R, S, st));
-    // stage 1 (MpStage1.h:78-85)
-    GTA_TIMED(KC_FINDFRAG, launch_findfrag(prec, Q, R, S, G, dinv, st));
-    GTA_TIMED(KC_REFINE_GAPLESS, launch_refine_gapless(prec, Q, R, S, G, dinv, st));
-    int rc;
-    if ((rc = dprefine(1, o.prescore, 2)) != GTA_OK) return rc;
-    // stage 2, pass 1: SS-weighted DP with gap cost (MpStage2.h:60-80)
-    GTA_TIMED(KC_DP, dp(DPM_GRAND_L, 0, -1.0f, 1, 1));
-    GTA_TIMED(KC_REFINE_DP, refine(RM_DPREFINE_L, 0, 0, 0, 0.f));
-    if ((rc = dprefine(0, 0.f, 2)) != GTA_OK) return rc;
-    if ((rc = dprefine(1, 0.f, 2)) != GTA_OK) return rc;
-    // stage 2, pass 2: no gap cost, convergence iterations
-    GTA_TIMED(KC_DP, dp(DPM_GRAND_L, 1, 0.0f, 0, 0));
-    GTA_TIMED(KC_REFINE_DP, refine(RM_DPREFINE_L, 0, 0, 0, 0.f));
-    if ((rc = dprefine(0, 0.f, 2)) != GTA_OK) return rc;
-    if ((rc = dprefine(1, 0.f, o.convergence)) != GTA_OK) return rc;
-    if (0.0f < o.scorethld) GTA_TIMED(KC_MISC, launch_setlow(Q, R, S, o.scorethld, st));
-    // final stage (MpStageFin.h:66-90)
-    GTA_TIMED(KC_DP, dp(DPM_GRAND_L, 1, 0.0f, 0, 0));
-    GTA_TIMED(KC_REFINE_FIN, refine(RM_FIN1_L, 0, 0, 0, 0.f));
-    if (!(o.depth == GTA_DEPTH_SHALLOW || o.refinement == GTA_REF_COARSE))
-        GTA_TIMED(KC_REFINE_FIN, refine(RM_FIN2_L, 0, 0, 0, 0.f));
-    GTA_TIMED(KC_DP, dp(DPM_CONSTRAINED_L, 1, 0.0f, 0, 0));
-    GTA_TIMED(KC_PROD_ALN, launch_prod_aln(Q, R, S, c->s_alns.as<char>(), o.nodeletions, st));
-    GTA_TIMED(KC_REFINE_FIN, refine(RM_PROD1_L, 0, 0, 0, 0.f));
-    if (o.refinement == GTA_REF_ONE || o.refinement == GTA_REF_COARSE)
-        GTA_TIMED(KC_REFINE_FIN, refine(RM_PROD2_L, 0, 0, 0, 0.f));
-    if (o.referenced) GTA_TIMED(KC_MISC, launch_revert(S, st));
+    // the batched form replays the sequence below as a HIP graph once captured
+    // (not while profiling: the per-launch events are host-timed)
+    const bool graph = batched && !c->profiling;
+    auto enqueue = [&]() -> int {
+        // cleared by kernels (no memset nodes in a captured graph)
+        HIPCHK(c, launch_fill_i32((int*)c->s_cells.p, 0, 2, st));
+        HIPCHK(c, launch_fill_i32((int*)c->s_algb.p, 0, KC_COUNT * 2, st));
+        HIPCHK(c, launch_fill_i32((int*)c->s_alns.p, 0, (alnsize + 3) / 4, st));
+        HIPCHK(c, launch_fill_i32(c->dpb.failflag, 0, 1, st));
+        if (batched) HIPCHK(c, launch_fill_i32(heads, 0, GTA_MAX_QUEUES, st));
+        GTA_TIMED(KC_SECSTR, launch_secstr(Q, st));
+        GTA_TIMED(KC_SECSTR, launch_secstr(R, st));
+        GTA_TIMED(KC_INIT, launch_init(Q, R, S, st));
+        // stage 1 (MpStage1.h:78-85)
+        GTA_TIMED(KC_FINDFRAG, launch_findfrag(prec, Q, R, S, G, dinv, c->rsp, st));
+        GTA_TIMED(KC_REFINE_GAPLESS, launch_refine_gapless(prec, Q, R, S, G, dinv, c->rsp, st));
+        int rc;
+        if ((rc = dprefine(1, o.prescore, 2)) != GTA_OK) return rc;
+        if (batched) HIPCHK(c, relist());
+        // stage 2, pass 1: SS-weighted DP with gap cost (MpStage2.h:60-80)
+        GTA_TIMED(KC_DP, dp(DPM_GRAND_L, 0, -1.0f, 1, 1));
+        GTA_TIMED(KC_REFINE_DP, refine(RM_DPREFINE_L, 0, 0, 0, 0.f));
+        if ((rc = dprefine(0, 0.f, 2)) != GTA_OK) return rc;
+        if ((rc = dprefine(1, 0.f, 2)) != GTA_OK) return rc;
+        // stage 2, pass 2: no gap cost, convergence iterations
+        GTA_TIMED(KC_DP, dp(DPM_GRAND_L, 1, 0.0f, 0, 0));
+        GTA_TIMED(KC_REFINE_DP, refine(RM_DPREFINE_L, 0, 0, 0, 0.f));
+        if ((rc = dprefine(0, 0.f, 2)) != GTA_OK) return rc;
+        if ((rc = dprefine(1, 0.f, o.convergence)) != GTA_OK) return rc;
+        if (0.0f < o.scorethld) {
+            GTA_TIMED(KC_MISC, launch_setlow(Q, R, S, o.scorethld, st));
+            if (batched) HIPCHK(c, relist());
+        }
+        // final stage (MpStageFin.h:66-90)
+        GTA_TIMED(KC_DP, dp(DPM_GRAND_L, 1, 0.0f, 0, 0));
+        GTA_TIMED(KC_REFINE_FIN, refine(RM_FIN1_L, 0, 0, 0, 0.f));
+        if (!(o.depth == GTA_DEPTH_SHALLOW || o.refinement == GTA_REF_COARSE))
+            GTA_TIMED(KC_REFINE_FIN, refine(RM_FIN2_L, 0, 0, 0, 0.f));
+        GTA_TIMED(KC_DP, dp(DPM_CONSTRAINED_L, 1, 0.0f, 0, 0));
+        GTA_TIMED(KC_PROD_ALN, launch_prod_aln(Q, R, S, c->s_alns.as<char>(), o.nodeletions, queue(lista, ltot),
+                                               pgrid_dp, st));
+        GTA_TIMED(KC_REFINE_FIN, refine(RM_PROD1_L, 0, 0, 0, 0.f));
+        if (o.refinement == GTA_REF_ONE || o.refinement == GTA_REF_COARSE)
+            GTA_TIMED(KC_REFINE_FIN, refine(RM_PROD2_L, 0, 0, 0, 0.f));
+        if (o.referenced) GTA_TIMED(KC_MISC, launch_revert(S, st));
+        return GTA_OK;
+    };
 #undef GTA_TIMED
+    HIPCHK(c, hipEventRecord(c->ev0, st));
+    if (graph && c->gvalid) {
+        HIPCHK(c, hipGraphLaunch(c->gexec, st));
+    } else {
+        // ends a capture that an error return leaves open
+        struct CaptureGuard {
+            hipStream_t st;
+            bool on = false;
+            ~CaptureGuard() {
+                if (on) {
+                    hipGraph_t g = nullptr;
+                    if (hipStreamEndCapture(st, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
+                }
+            }
+        } cap{st};
+        if (graph) {
+            HIPCHK(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            cap.on = true;
+        }
+        const int erc = enqueue();
+        if (erc != GTA_OK) return erc;
+        if (graph) {
+            hipGraph_t g = nullptr;
+            cap.on = false;
+            HIPCHK(c, hipStreamEndCapture(st, &g));
+            if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+            hipError_t ge = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            HIPCHK(c, ge);
+            c->gvalid = true;
+            HIPCHK(c, hipGraphLaunch(c->gexec, st));
+        }
+    }
     HIPCHK(c, hipEventRecord(c->ev1, st));
     HIPCHK(c, hipEventSynchronize(c->ev1));
     float ms = 0.f;
@@ -434,6 +632,9 @@ static int run_stage(gta_ctx* c, double* seconds, double* cells) {
             c->kc_seconds[e.first] += t * 1e-3;
         }
     }
+    int dpfail = 0;
+    HIPCHK(c, hipMemcpy(&dpfail, c->s_fail.p, 4, hipMemcpyDeviceToHost));
+    if (dpfail) return fail(c, GTA_ERR_HIP, "multi-wave DP: a row wait exceeded its spin limit");
     unsigned long long ab[KC_COUNT];
     HIPCHK(c, hipMemcpy(ab, c->s_algb.p, sizeof(ab), hipMemcpyDeviceToHost));
     for (int k = 0; k < KC_COUNT; k++) c->kc_bytes[k] = (double)ab[k];
@@ -476,7 +677,7 @@ int gta_test_dp(gta_ctx* c, const float* tfms, int gap0, float gapopen, int uses
     HIPCHK(c, launch_init(Q, R, S, st));
     HIPCHK(c, hipMemcpyAsync(S.tfm_grand, tfms, np * 12 * 4, hipMemcpyHostToDevice, st));
     DpLaunch L{DPM_GRAND_L, gap0, gapopen, usess, usess ? 0.5f : 0.0f, dpscan};
-    HIPCHK(c, launch_dp(Q, R, S, L, st));
+    HIPCHK(c, launch_dp(Q, R, S, L, c->dpb, WorkQ{}, 0, st));
     HIPCHK(c, hipStreamSynchronize(st));
     std::vector<float> pl(S.alnplane);
     HIPCHK(c, hipMemcpy(nmatched, S.nalnposs, np * 4, hipMemcpyDeviceToHost));
@@ -484,6 +685,22 @@ int gta_test_dp(gta_ctx* c, const float* tfms, int gap0, float gapopen, int uses
     for (size_t i = 0; i < S.alnplane; i++) qpos[i] = (int)pl[i];
     HIPCHK(c, hipMemcpy(pl.data(), S.aln + 8 * S.alnplane, S.alnplane * 4, hipMemcpyDeviceToHost));
     for (size_t i = 0; i < S.alnplane; i++) rpos[i] = (int)pl[i];
+    return GTA_OK;
+}
+
+int gta_test_set_dp_buckets(gta_ctx* c, long long min_cells, int min_strips) {
+    if (!c || min_cells < 0 || min_strips < 1) return fail(c, GTA_ERR_ARG, "invalid DP bucket thresholds");
+    c->mw_min_cells = min_cells;
+    c->mw_min_strips = min_strips;
+    return GTA_OK;
+}
+
+int gta_test_set_split(gta_ctx* c, int min_len, int seeds_per_part, int slots_per_part) {
+    if (!c || min_len < 1 || seeds_per_part < 1 || slots_per_part < 1)
+        return fail(c, GTA_ERR_ARG, "invalid split parameters");
+    c->split_min_len = min_len;
+    c->split_seeds = seeds_per_part;
+    c->split_slots = slots_per_part;
     return GTA_OK;
 }
 

@@ -53,6 +53,15 @@ struct Geom {
     int maxnfragfcts;           // final-stage positions (MpStageFin.cpp:308-309)
 };
 
+// A device work queue: items[0 .. *count) taken in order by the waves or
+// workgroups of a persistent launch through the counter *head (zeroed
+// before the launch).  items == nullptr: no queue (one block per pair).
+struct WorkQ {
+    const int* items;
+    const int* count;
+    int* head;
+};
+
 struct DevState {
     int nq, nr;
     int* flags;

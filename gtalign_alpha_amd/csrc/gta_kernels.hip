@@ -82,6 +82,22 @@ constexpr int RTHREADS = RWAVES * GTA_WAVE;
 __device__ __forceinline__ int pair_q(const DevState& S, int p) { return p / S.nr; }
 __device__ __forceinline__ int pair_r(const DevState& S, int p) { return p % S.nr; }
 
+// Next item of a device work queue for the calling wave (lane 0 takes it
+// from the queue's counter; every lane returns it).
+__device__ __forceinline__ int wq_next_wave(const WorkQ& wq) {
+    int it = 0;
+    if (lane_id() == 0) it = atomicAdd(wq.head, 1);
+    return __shfl(it, 0, GTA_WAVE);
+}
+// The same for a whole workgroup (thread 0 takes it, LDS broadcast).
+__device__ __forceinline__ int wq_next_block(const WorkQ& wq, int* slot) {
+    if (threadIdx.x == 0) *slot = atomicAdd(wq.head, 1);
+    __syncthreads();
+    const int it = *slot;
+    __syncthreads();
+    return it;
+}
+
 // convergence of the covariance record (CheckConvergenceRefined_Complete,
 // MpStageBase.h:262-277): all 16 fields changed by less than 1e-5 relative
 template <typename T>
@@ -281,17 +297,29 @@ __device__ __forceinline__ float gscore_gapless(const Coords& q, const Coords& r
 // stage 1: best gapless diagonal (seed) per pair (FindGaplessAlignedFragment,
 // MpStage1.cpp:267-400).  Slots are seed diagonals; each runs up to three
 // rounds of covariance -> Kabsch -> score.
+__device__ __forceinline__ void findfrag_epilogue(const DevState& S, int p, const SlotBest& b) {
+    S.best0[p] = b.best;
+    if (b.best > 0.f) { S.qrypos[p] = b.ndx; S.rfnpos[p] = b.pos; }
+}
+
+// Slots [lo, hi) of part `part` of nparts of the range [0, n).
+__device__ __forceinline__ void part_range(int n0, int n1, int part, int nparts, int& lo, int& hi) {
+    const int n = max(n1 - n0, 0), per = (n + nparts - 1) / nparts;
+    lo = min(n0 + part * per, n1);
+    hi = min(lo + per, n1);
+}
+
 template <int P>
 __global__ __launch_bounds__(RTHREADS) void k_findfrag(DevChunk Q, DevChunk R, DevState S, Geom G,
-                                                        int dinv_on) {
+                                                        int dinv_on, const int* plist, SlotBest* parts) {
     __shared__ SlotBest lds[NGROUPS];
     __shared__ int ctr;
-    const int p = blockIdx.x;
+    const int p = plist ? plist[blockIdx.x] : (int)blockIdx.x;
     const int qi = pair_q(S, p), ri = pair_r(S, p);
     const int qlen = Q.len[qi], rlen = R.len[ri];
     const int qd = Q.dst[qi], rd = R.dst[ri];
     if (S.flags[p] & CONV_LOWTMSC) {
-        if (threadIdx.x == 0) S.best0[p] = 0.f;
+        if (threadIdx.x == 0 && !parts) S.best0[p] = 0.f;
         return;
     }
     typedef typename GPrec<P>::T T;
@@ -299,7 +327,13 @@ __global__ __launch_bounds__(RTHREADS) void k_findfrag(DevChunk Q, DevChunk R, D
     const bool dinv = dinv_on && (qlen < rlen);
     const float d0 = GetD0(qlen, rlen), d02 = d0 * d0;
     const int prv = max(min(qlen, rlen) >> 1, 5);
-    if (threadIdx.x == 0) ctr = 0;
+    // the pair's seeds: qlen-qpos >= prv and rlen-rpos >= prv, i.e.
+    // abspos = n1 + si*stepinit in [prv-rlen, qlen-prv]
+    const int slo = max(0, (prv - rlen - G.n1 + G.stepinit - 1) / G.stepinit);
+    const int shi = min(G.nseeds, (qlen - prv - G.n1) / G.stepinit + 1);
+    int lo, hi;
+    part_range(slo, shi, blockIdx.y, gridDim.y, lo, hi);
+    if (threadIdx.x == 0) ctr = lo;
     __syncthreads();
 
     SlotBest mine;
@@ -313,7 +347,7 @@ __global__ __launch_bounds__(RTHREADS) void k_findfrag(DevChunk Q, DevChunk R, D
         for (;;) {
             const int got = fetch_slot(need, &ctr);
             if (need) {
-                if (got >= G.nseeds) {
+                if (got >= hi) {
                     need = false;
                 } else {
                     si = got;
@@ -360,8 +394,8 @@ __global__ __launch_bounds__(RTHREADS) void k_findfrag(DevChunk Q, DevChunk R, D
     SlotBest b;
     group_best(mine, lds, b);
     if (threadIdx.x == 0) {
-        S.best0[p] = b.best;
-        if (b.best > 0.f) { S.qrypos[p] = b.ndx; S.rfnpos[p] = b.pos; }
+        if (parts) parts[(size_t)blockIdx.x * gridDim.y + blockIdx.y] = b;
+        else findfrag_epilogue(S, p, b);
     }
 }
 
@@ -369,19 +403,28 @@ __global__ __launch_bounds__(RTHREADS) void k_findfrag(DevChunk Q, DevChunk R, D
 // stage 1: refinement of fragments of the best gapless alignment
 // (RefineFragInitKernel, MpStage1.cpp:470-640).  Slots are (fragment position,
 // fragment length); each iterates to convergence.
+__device__ __forceinline__ void gapless_epilogue(const DevState& S, int p, const SlotBest& b) {
+    const bool have = b.best > 0.f;
+    S.best0[p] = have ? b.best : 0.f;
+    if (have) {
+        copy12(S.tfm_ibest + p * 12, b.tfm);
+        if (S.grand[p] < b.best) { S.grand[p] = b.best; copy12(S.tfm_grand + p * 12, b.tfm); }
+    }
+}
+
 template <int P>
 __global__ __launch_bounds__(RTHREADS) void k_refine_gapless(DevChunk Q, DevChunk R, DevState S, Geom G,
-                                                              int dinv_on) {
+                                                              int dinv_on, const int* plist, SlotBest* parts) {
     __shared__ SlotBest lds[NGROUPS];
     __shared__ float tbest[NGROUPS][12];
     __shared__ typename GPrec<P>::T tlast[NGROUPS][16];
     __shared__ int ctr;
-    const int p = blockIdx.x;
+    const int p = plist ? plist[blockIdx.x] : (int)blockIdx.x;
     const int qi = pair_q(S, p), ri = pair_r(S, p);
     const int qlen = Q.len[qi], rlen = R.len[ri];
     const int qd = Q.dst[qi], rd = R.dst[ri];
     if (S.flags[p] & CONV_LOWTMSC) {
-        if (threadIdx.x == 0) S.best0[p] = 0.f;
+        if (threadIdx.x == 0 && !parts) S.best0[p] = 0.f;
         return;
     }
     typedef typename GPrec<P>::T T;
@@ -392,7 +435,9 @@ __global__ __launch_bounds__(RTHREADS) void k_refine_gapless(DevChunk Q, DevChun
     const float d0s1 = GetD0s(d0) + 1.0f, d02s1 = d0s1 * d0s1;
     const bool anyslot = !(qlen <= qpos || rlen <= rpos);
     const int nall = anyslot ? min(qlen - qpos, rlen - rpos) : 0;
-    if (threadIdx.x == 0) ctr = 0;
+    int lo, hi;
+    part_range(0, G.nlocsteps, blockIdx.y, gridDim.y, lo, hi);
+    if (threadIdx.x == 0) ctr = lo;
     __syncthreads();
 
     // per group: its best slot and the current slot's best transform, in LDS
@@ -409,7 +454,7 @@ __global__ __launch_bounds__(RTHREADS) void k_refine_gapless(DevChunk Q, DevChun
         for (;;) {
             const int got = fetch_slot(need, &ctr);
             if (need) {
-                if (got >= G.nlocsteps) {
+                if (got >= hi) {
                     need = false;
                 } else {
                     si = got;
@@ -473,12 +518,8 @@ __global__ __launch_bounds__(RTHREADS) void k_refine_gapless(DevChunk Q, DevChun
     SlotBest b;
     group_best_lds(lds, b);
     if (threadIdx.x == 0) {
-        const bool have = b.best > 0.f;
-        S.best0[p] = have ? b.best : 0.f;
-        if (have) {
-            copy12(S.tfm_ibest + p * 12, b.tfm);
-            if (S.grand[p] < b.best) { S.grand[p] = b.best; copy12(S.tfm_grand + p * 12, b.tfm); }
-        }
+        if (parts) parts[(size_t)blockIdx.x * gridDim.y + blockIdx.y] = b;
+        else gapless_epilogue(S, p, b);
     }
 }
 
@@ -523,148 +564,177 @@ __device__ __forceinline__ float readlane_f(float v, int l) {
     return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
 }
 
-// One wave per pair.  The matrix is swept in strips of 64 query rows; lane l
-// owns row 64s+l and at step t computes column t-l, so a step advances one
-// anti-diagonal.  The reference residues flow through the wave one lane per
-// step (DPP shifts), lane 0 taking column t from a 64-column block that the
-// whole wave loads coalesced one block ahead; the row above a strip (lane
-// 63's values of the previous strip) is stored to and read back from a
-// per-pair row in HBM in the same 64-column blocks.  No LDS, no divergent
-// branches in the step loop.
-template <bool USESS>
-__global__ __launch_bounds__(GTA_WAVE) void k_dp(DevChunk Q, DevChunk R, DevState S, DpParams prm) {
-    const int p = blockIdx.x;
-    const int qi = pair_q(S, p), ri = pair_r(S, p);
-    const int Lq = Q.len[qi], Lr = R.len[ri];
-    const int qd = Q.dst[qi], rd = R.dst[ri];
-    const int lane = lane_id();
-    const int flags = S.flags[p];
-    if (prm.mode == DPM_IBEST) {
-        if (flags & (CONV_LOWTMSC | CONV_SCOREDP)) { if (lane == 0) S.nalnposs[p] = 0; return; }
-    } else if (flags & CONV_LOWTMSC) {
-        if (prm.mode == DPM_GRAND && lane == 0) S.nalnposs[p] = 0;
-        return;
-    }
+// Per-pair parameters of a DP fill.
+struct DpPair {
+    int p, qi, ri, Lq, Lr, qd, rd, flags;
     float tfm[12];
+    float d02;
+};
+
+// false when the pair takes no DP in this mode (its matched count is set)
+__device__ __forceinline__ bool dp_setup(const DevChunk& Q, const DevChunk& R, const DevState& S,
+                                         const DpParams& prm, int p, DpPair& P) {
+    P.p = p;
+    P.qi = pair_q(S, p);
+    P.ri = pair_r(S, p);
+    P.Lq = Q.len[P.qi];
+    P.Lr = R.len[P.ri];
+    P.qd = Q.dst[P.qi];
+    P.rd = R.dst[P.ri];
+    P.flags = S.flags[p];
+    if (prm.mode == DPM_IBEST) {
+        if (P.flags & (CONV_LOWTMSC | CONV_SCOREDP)) {
+            if (threadIdx.x == 0) S.nalnposs[p] = 0;
+            return false;
+        }
+    } else if (P.flags & CONV_LOWTMSC) {
+        if (prm.mode == DPM_GRAND && threadIdx.x == 0) S.nalnposs[p] = 0;
+        return false;
+    }
     const float* tsrc = (prm.mode == DPM_IBEST ? S.tfm_ibest : S.tfm_grand) + (size_t)p * 12;
 #pragma unroll
-    for (int i = 0; i < 12; i++) tfm[i] = tsrc[i];
-    float d0 = GetD0(Lq, Lr);
+    for (int i = 0; i < 12; i++) P.tfm[i] = tsrc[i];
+    float d0 = GetD0(P.Lq, P.Lr);
     if (prm.dpscan) d0 += 1.5f;
-    const float d02 = d0 * d0;
+    P.d02 = d0 * d0;
+    return true;
+}
+
+// Row handoff between the strips of one wave: nothing to wait for.
+struct NoSync {
+    __device__ __forceinline__ bool wait(int, int) { return true; }
+    __device__ __forceinline__ void publish(int, int) {}
+};
+
+// One strip of 64 query rows of the DP (k_dp's strip sweep).  Lane l owns row
+// 64s+l and at step t computes column t-l, so a step advances one
+// anti-diagonal.  The reference residues flow through the wave one lane per
+// step (DPP shifts), lane 0 taking column t from a 64-column block that the
+// whole wave loads coalesced one block ahead; the row above the strip
+// (rowin, written by the previous strip) and this strip's last row (rowout)
+// are exchanged in the same 64-column blocks.  No LDS, no divergent branches
+// in the step loop.  Returns false when a row wait gave up (Sync::wait).
+template <bool USESS, class Sync>
+__device__ __forceinline__ bool dp_strip(const DevChunk& Q, const DevChunk& R, const DevState& S,
+                                         const DpParams& prm, const DpPair& P, int s, const float* rowin,
+                                         float* rowout, unsigned* bt, Sync& sync) {
+    const int lane = lane_id();
+    const int Lq = P.Lq, Lr = P.Lr;
+    const float* Rx = R.x + P.rd;
+    const float* Ry = R.y + P.rd;
+    const float* Rz = R.z + P.rd;
+    const char* Rs = R.ss + P.rd;
     const bool gap0 = prm.gap0 != 0;
-    const float gapopen = prm.gapopen, ssw = prm.ssw;
-
-    const float* Rx = R.x + rd;
-    const float* Ry = R.y + rd;
-    const float* Rz = R.z + rd;
-    const char* Rs = R.ss + rd;
-    float* row = S.rowbuf + (size_t)qi * (size_t)R.nposs + (size_t)rd;
-    unsigned* bt = S.btck + S.btckoff[p];
-    const int nstrips = (Lq + 63) >> 6;
+    const float gapopen = prm.gapopen, ssw = prm.ssw, d02 = P.d02;
     const int nsteps = Lr + 63;
-
-    for (int s = 0; s < nstrips; s++) {
-        const int i = (s << 6) + lane;
-        const bool rowok = i < Lq;
-        float qx = 0.f, qy = 0.f, qz = 0.f;
-        float qss = 0.f;
-        if (rowok) {
-            qx = Q.x[qd + i]; qy = Q.y[qd + i]; qz = Q.z[qd + i];
-            tfm_point(tfm, qx, qy, qz);
-            if (USESS) qss = (float)Q.ss[qd + i];
+    const int nblkw = (nsteps + 15) >> 4;
+    const int i = (s << 6) + lane;
+    const bool rowok = i < Lq;
+    float qx = 0.f, qy = 0.f, qz = 0.f;
+    float qss = 0.f;
+    if (rowok) {
+        qx = Q.x[P.qd + i]; qy = Q.y[P.qd + i]; qz = Q.z[P.qd + i];
+        tfm_point(P.tfm, qx, qy, qz);
+        if (USESS) qss = (float)Q.ss[P.qd + i];
+    }
+    const bool qok = rowok && qx < DEFCOORD_QRY_cmp;
+    const bool haveup = rowin != nullptr;
+    const bool putrow = rowout != nullptr;
+    // 64-column blocks: lane k holds column c0+k (current and next block)
+    float cbx, cby, cbz, cbs = 0.f, cbu = 0.f, nbx, nby, nbz, nbs = 0.f, nbu = 0.f;
+    {
+        const int c = lane;
+        const bool ok = c < Lr;
+        cbx = ok ? Rx[c] : 0.f; cby = ok ? Ry[c] : 0.f; cbz = ok ? Rz[c] : 0.f;
+        if (USESS) cbs = ok ? (float)Rs[c] : 0.f;
+        const int c2 = 64 + lane;
+        const bool ok2 = c2 < Lr;
+        nbx = ok2 ? Rx[c2] : 0.f; nby = ok2 ? Ry[c2] : 0.f; nbz = ok2 ? Rz[c2] : 0.f;
+        if (USESS) nbs = ok2 ? (float)Rs[c2] : 0.f;
+        if (haveup) {
+            if (!sync.wait(s - 1, min(128, Lr))) return false;
+            cbu = ok ? rowin[c] : 0.f;
+            nbu = ok2 ? rowin[c2] : 0.f;
         }
-        const bool qok = rowok && qx < DEFCOORD_QRY_cmp;
-        const bool haveup = s > 0;
-        const bool putrow = s + 1 < nstrips;
-        // 64-column blocks: lane k holds column c0+k (current and next block)
-        float cbx, cby, cbz, cbs = 0.f, cbu = 0.f, nbx, nby, nbz, nbs = 0.f, nbu = 0.f;
-        {
-            const int c = lane;
-            const bool ok = c < Lr;
-            cbx = ok ? Rx[c] : 0.f; cby = ok ? Ry[c] : 0.f; cbz = ok ? Rz[c] : 0.f;
-            if (USESS) cbs = ok ? (float)Rs[c] : 0.f;
-            if (haveup) cbu = ok ? row[c] : 0.f;
-            const int c2 = 64 + lane;
+    }
+    float rx = 0.f, ry = 0.f, rz = 0.f, rs = 0.f;
+    float val = 0.f, upprev = 0.f, oblk = 0.f;
+    unsigned word = 0;
+    for (int t = 0; t < nsteps; t++) {
+        const int k = t & 63;
+        if (k == 0 && t > 0) {
+            cbx = nbx; cby = nby; cbz = nbz; cbs = nbs; cbu = nbu;
+            const int c2 = t + 64 + lane;
             const bool ok2 = c2 < Lr;
             nbx = ok2 ? Rx[c2] : 0.f; nby = ok2 ? Ry[c2] : 0.f; nbz = ok2 ? Rz[c2] : 0.f;
             if (USESS) nbs = ok2 ? (float)Rs[c2] : 0.f;
-            if (haveup) nbu = ok2 ? row[c2] : 0.f;
-        }
-        float rx = 0.f, ry = 0.f, rz = 0.f, rs = 0.f;
-        float val = 0.f, upprev = 0.f, oblk = 0.f;
-        unsigned word = 0;
-        for (int t = 0; t < nsteps; t++) {
-            const int k = t & 63;
-            if (k == 0 && t > 0) {
-                cbx = nbx; cby = nby; cbz = nbz; cbs = nbs; cbu = nbu;
-                const int c2 = t + 64 + lane;
-                const bool ok2 = c2 < Lr;
-                nbx = ok2 ? Rx[c2] : 0.f; nby = ok2 ? Ry[c2] : 0.f; nbz = ok2 ? Rz[c2] : 0.f;
-                if (USESS) nbs = ok2 ? (float)Rs[c2] : 0.f;
-                if (haveup) nbu = ok2 ? row[c2] : 0.f;
-            }
-            rx = wave_shr1(rx, readlane_f(cbx, k));
-            ry = wave_shr1(ry, readlane_f(cby, k));
-            rz = wave_shr1(rz, readlane_f(cbz, k));
-            if (USESS) rs = wave_shr1(rs, readlane_f(cbs, k));
-            const float up = wave_shr1(val, haveup ? readlane_f(cbu, k) : 0.f);
-            const float diag = upprev;
-            upprev = up;
-            const int j = t - lane;
-            const bool valid = rowok && (unsigned)j < (unsigned)Lr;
-            float sc = 0.0f;
-            if (qok && DEFCOORD_RFN_cmp < rx) sc = pair_score(d02, dist2(qx, qy, qz, rx, ry, rz));
-            if (USESS) sc = __fadd_rn(sc, (qss == rs) ? ssw : 0.0f);
-            float v = __fadd_rn(sc, gap0 ? diag : fabsf(diag));
-            unsigned bk = BT_DIAG;
-            float w = val;
-            if (!gap0 && w < 0.0f) w = __fsub_rn(gapopen, w);
-            if (v < w) { v = w; bk = BT_LEFT; }
-            w = up;
-            if (!gap0 && w < 0.0f) w = __fsub_rn(gapopen, w);
-            if (v < w) { v = w; bk = BT_UP; }
-            const float nv = (gap0 || bk != BT_DIAG) ? v : -v;
-            val = valid ? nv : (j < 0 ? 0.0f : val);
-            const unsigned code = valid ? bk : (unsigned)BT_STOP;
-            // lane 63's value of column t-63 goes to the row of the next strip
-            const int jo = t - 63;
-            if (putrow && jo >= 0) {
-                const float o = readlane_f(val, 63);
-                oblk = (lane == (jo & 63)) ? o : oblk;
-                if ((jo & 63) == 63 || jo == Lr - 1) {
-                    if (lane <= (jo & 63)) row[(jo & ~63) + lane] = oblk;
-                }
-            }
-            // 2-bit codes of 16 steps -> one 64-word block (lane 4k+{0,1}: bit 0, {2,3}: bit 1)
-            const unsigned long long b0 = __ballot(code & 1u);
-            const unsigned long long b1 = __ballot(code & 2u);
-            if ((lane >> 2) == (t & 15)) {
-                const unsigned long long m = (lane & 2) ? b1 : b0;
-                word = (lane & 1) ? (unsigned)(m >> 32) : (unsigned)m;
-            }
-            if ((t & 15) == 15 || t == nsteps - 1) {
-                bt[((size_t)((s * ((nsteps + 15) >> 4)) + (t >> 4)) << 6) + lane] = word;
-                word = 0;
+            if (haveup && t + 64 < Lr) {
+                if (!sync.wait(s - 1, min(t + 128, Lr))) return false;
+                nbu = ok2 ? rowin[c2] : 0.f;
             }
         }
-        // the next strip reads this strip's row back
-        __threadfence_block();
+        rx = wave_shr1(rx, readlane_f(cbx, k));
+        ry = wave_shr1(ry, readlane_f(cby, k));
+        rz = wave_shr1(rz, readlane_f(cbz, k));
+        if (USESS) rs = wave_shr1(rs, readlane_f(cbs, k));
+        const float up = wave_shr1(val, haveup ? readlane_f(cbu, k) : 0.f);
+        const float diag = upprev;
+        upprev = up;
+        const int j = t - lane;
+        const bool valid = rowok && (unsigned)j < (unsigned)Lr;
+        float sc = 0.0f;
+        if (qok && DEFCOORD_RFN_cmp < rx) sc = pair_score(d02, dist2(qx, qy, qz, rx, ry, rz));
+        if (USESS) sc = __fadd_rn(sc, (qss == rs) ? ssw : 0.0f);
+        float v = __fadd_rn(sc, gap0 ? diag : fabsf(diag));
+        unsigned bk = BT_DIAG;
+        float w = val;
+        if (!gap0 && w < 0.0f) w = __fsub_rn(gapopen, w);
+        if (v < w) { v = w; bk = BT_LEFT; }
+        w = up;
+        if (!gap0 && w < 0.0f) w = __fsub_rn(gapopen, w);
+        if (v < w) { v = w; bk = BT_UP; }
+        const float nv = (gap0 || bk != BT_DIAG) ? v : -v;
+        val = valid ? nv : (j < 0 ? 0.0f : val);
+        const unsigned code = valid ? bk : (unsigned)BT_STOP;
+        // lane 63's value of column t-63 goes to the row of the next strip
+        const int jo = t - 63;
+        if (putrow && jo >= 0) {
+            const float o = readlane_f(val, 63);
+            oblk = (lane == (jo & 63)) ? o : oblk;
+            if ((jo & 63) == 63 || jo == Lr - 1) {
+                if (lane <= (jo & 63)) rowout[(jo & ~63) + lane] = oblk;
+                sync.publish(s, jo + 1);
+            }
+        }
+        // 2-bit codes of 16 steps -> one 64-word block (lane 4k+{0,1}: bit 0, {2,3}: bit 1)
+        const unsigned long long b0 = __ballot(code & 1u);
+        const unsigned long long b1 = __ballot(code & 2u);
+        if ((lane >> 2) == (t & 15)) {
+            const unsigned long long m = (lane & 2) ? b1 : b0;
+            word = (lane & 1) ? (unsigned)(m >> 32) : (unsigned)m;
+        }
+        if ((t & 15) == 15 || t == nsteps - 1) {
+            bt[((size_t)(s * nblkw + (t >> 4)) << 6) + lane] = word;
+            word = 0;
+        }
     }
-    if (lane == 0) {
-        const unsigned long long cells = (unsigned long long)Lq * (unsigned long long)Lr;
-        atomicAdd(S.cells, cells);
-        atomicAdd(S.algbytes + KC_DP, cells / 4 + 12ull * (unsigned long long)(Lq + Lr));
-    }
-    __threadfence();
-    const int nblk = (nsteps + 15) >> 4;
+    return true;
+}
 
-    // ---- backtrace walk --------------------------------------------------------
-    if (prm.mode == DPM_GRAND && (flags & CONV_SCOREDP)) {
+// Backtrace walk from the bottom-right cell (BtckToMatched, btck2match.cuh:46-52;
+// ConstrainedBtckToMatched keeps pairs within d8 and their d2).  Vectorised:
+// the 64 lanes read 64 successive cells of the current direction and a ballot
+// finds the run length.  Matched pairs are stored in walk (reverse) order.
+__device__ void dp_walk(const DevChunk& Q, const DevChunk& R, const DevState& S, const DpParams& prm,
+                        const DpPair& P, const unsigned* bt) {
+    const int lane = lane_id();
+    const int p = P.p, Lq = P.Lq, Lr = P.Lr, qd = P.qd, rd = P.rd;
+    if (prm.mode == DPM_GRAND && (P.flags & CONV_SCOREDP)) {
         if (lane == 0) S.nalnposs[p] = 0;
         return;
     }
-    const size_t base = (size_t)qi * (size_t)R.nposs + (size_t)rd;
+    const int nblk = (Lr + 63 + 15) >> 4;
+    const size_t base = (size_t)P.qi * (size_t)R.nposs + (size_t)rd;
     float* aqx = S.aln + base;
     float* aqy = aqx + S.alnplane;
     float* aqz = aqy + S.alnplane;
@@ -693,7 +763,7 @@ __global__ __launch_bounds__(GTA_WAVE) void k_dp(DevChunk Q, DevChunk R, DevStat
                 qx = Q.x[qd + yy]; qy = Q.y[qd + yy]; qz = Q.z[qd + yy];
                 rx = R.x[rd + xx]; ry = R.y[rd + xx]; rz = R.z[rd + xx];
                 if (constrained) {
-                    d2 = tfm_dist2(tfm, qx, qy, qz, rx, ry, rz);
+                    d2 = tfm_dist2(P.tfm, qx, qy, qz, rx, ry, rz);
                     keep = d2 <= d82;
                 }
             }
@@ -718,6 +788,122 @@ __global__ __launch_bounds__(GTA_WAVE) void k_dp(DevChunk Q, DevChunk R, DevStat
         }
     }
     if (lane == 0) S.nalnposs[p] = k;
+}
+
+__device__ __forceinline__ void dp_count(const DevState& S, const DpPair& P) {
+    if (threadIdx.x == 0) {
+        const unsigned long long cells = (unsigned long long)P.Lq * (unsigned long long)P.Lr;
+        atomicAdd(S.cells, cells);
+        atomicAdd(S.algbytes + KC_DP, cells / 4 + 12ull * (unsigned long long)(P.Lq + P.Lr));
+    }
+}
+
+// DP fill with backtrace and the walk, one wave per pair (plist, when given,
+// maps blocks to pairs).  The strips run in order; each reads the row the
+// previous one wrote to the pair's row in HBM.
+template <bool USESS>
+__device__ void dp_pair(const DevChunk& Q, const DevChunk& R, const DevState& S, const DpParams& prm, int p) {
+    DpPair P;
+    if (!dp_setup(Q, R, S, prm, p, P)) return;
+    float* row = S.rowbuf + (size_t)P.qi * (size_t)R.nposs + (size_t)P.rd;
+    unsigned* bt = S.btck + S.btckoff[P.p];
+    const int nstrips = (P.Lq + 63) >> 6;
+    NoSync ns;
+    for (int s = 0; s < nstrips; s++) {
+        dp_strip<USESS>(Q, R, S, prm, P, s, s > 0 ? row : nullptr, s + 1 < nstrips ? row : nullptr, bt, ns);
+        // the next strip reads this strip's row back
+        __threadfence_block();
+    }
+    dp_count(S, P);
+    __threadfence();
+    dp_walk(Q, R, S, prm, P, bt);
+}
+
+// One wave per pair: block b takes pair plist[b] (b without a list), or, with
+// a work queue, the waves of a persistent grid take queued pairs until the
+// queue is drained.
+template <bool USESS>
+__global__ __launch_bounds__(GTA_WAVE) void k_dp(DevChunk Q, DevChunk R, DevState S, DpParams prm,
+                                                  const int* plist, WorkQ wq) {
+    if (wq.items) {
+        const int n = *wq.count;
+        for (int it = wq_next_wave(wq); it < n; it = wq_next_wave(wq)) dp_pair<USESS>(Q, R, S, prm, wq.items[it]);
+    } else {
+        dp_pair<USESS>(Q, R, S, prm, plist ? plist[blockIdx.x] : (int)blockIdx.x);
+    }
+}
+
+// Row handoff between the waves of one workgroup: wave w sweeps strips
+// w, w+W, ...; strip s writes its last row into ring row s mod W of the pair
+// and publishes, in LDS, how many of its columns are stored.  The wave of
+// strip s+1 waits (bounded spin) until the columns it loads next are there.
+// The rows go through HBM with workgroup-scope release/acquire fences
+// around the LDS progress counters.
+template <int W>
+struct LdsSync {
+    int* strip;     // [W] strip each wave is on
+    int* done;      // [W] columns of that strip's row stored
+    int* fail;      // set when a wait gave up
+    __device__ __forceinline__ bool wait(int sp, int cols) {
+        const int pw = sp % W;
+        for (long long it = 0;; it++) {
+            const int cs = __hip_atomic_load(strip + pw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            const int cd = __hip_atomic_load(done + pw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (cs > sp || (cs == sp && cd >= cols)) break;
+            if (it > (1ll << 24) || __hip_atomic_load(fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) {
+                __hip_atomic_store(fail, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                return false;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        return true;
+    }
+    __device__ __forceinline__ void publish(int s, int cols) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        if (lane_id() == 0) __hip_atomic_store(done + (s % W), cols, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    __device__ __forceinline__ void begin(int s) {
+        if (lane_id() == 0) {
+            __hip_atomic_store(done + (s % W), 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_store(strip + (s % W), s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+    }
+};
+
+// DP of one large pair per workgroup of W waves (pairs from plist; ring rows
+// of W x Lr floats per pair at ringoff[item]).  Same cells, codes and walk as
+// k_dp; only the strips run concurrently, pipelined down the query.
+template <bool USESS, int W>
+__global__ __launch_bounds__(W * GTA_WAVE) void k_dp_mw(DevChunk Q, DevChunk R, DevState S, DpParams prm,
+                                                         const int* plist, float* ring,
+                                                         const unsigned long long* ringoff, int* failflag) {
+    __shared__ int strip[W], done[W], fail;
+    DpPair P;
+    if (!dp_setup(Q, R, S, prm, plist[blockIdx.x], P)) return;
+    const int w = threadIdx.x / GTA_WAVE;
+    if (threadIdx.x < W) { strip[threadIdx.x] = -1; done[threadIdx.x] = 0; }
+    if (threadIdx.x == 0) fail = 0;
+    __syncthreads();
+    float* rows = ring + ringoff[blockIdx.x];
+    unsigned* bt = S.btck + S.btckoff[P.p];
+    const int nstrips = (P.Lq + 63) >> 6;
+    LdsSync<W> sy{strip, done, &fail};
+    for (int s = w; s < nstrips; s += W) {
+        sy.begin(s);
+        const float* rin = s > 0 ? rows + (size_t)((s - 1) % W) * P.Lr : nullptr;
+        float* rout = s + 1 < nstrips ? rows + (size_t)(s % W) * P.Lr : nullptr;
+        if (!dp_strip<USESS>(Q, R, S, prm, P, s, rin, rout, bt, sy)) break;
+        // the strip's row is complete
+        sy.publish(s, P.Lr);
+    }
+    __syncthreads();
+    if (fail) {
+        if (threadIdx.x == 0) { atomicOr(failflag, 1); S.nalnposs[P.p] = 0; }
+        return;
+    }
+    dp_count(S, P);
+    if (w == 0) dp_walk(Q, R, S, prm, P, bt);
 }
 
 // ============================================================================
@@ -807,16 +993,66 @@ __device__ __forceinline__ float gscore_dp(const Aln& a, int L, const float* tfm
 // MpStage1.cpp:455-510, MpStageFin.cpp:45-1470).  Slots are (fragment
 // position, fragment length) along the alignment; each iterates covariance
 // (thresholded after the first) -> Kabsch -> score to convergence.
+template <int MODE>
+__device__ void refine_dp_epilogue(const DevChunk& Q, const DevChunk& R, const DevState& S,
+                                   const RefineParams& prm, int p, const SlotBest& b) {
+    const int qi = pair_q(S, p), ri = pair_r(S, p);
+    const int qlen = Q.len[qi], rlen = R.len[ri];
+    const bool prod = (MODE == RM_PROD1 || MODE == RM_PROD2);
+    const bool have = b.best > 0.f;
+    float* tg = S.tfm_grand + (size_t)p * 12;
+    if (!prod) {
+        S.best0[p] = have ? b.best : 0.f;
+        if (have) copy12(S.tfm_ibest + (size_t)p * 12, b.tfm);
+        if (MODE == RM_FIN1) { S.sfragndx[p] = have ? b.ndx : 0; S.sfragpos[p] = have ? b.pos : 0; }
+        if (have && S.grand[p] < b.best) { S.grand[p] = b.best; copy12(tg, b.tfm); }
+    } else {
+        const float grand = (MODE == RM_PROD2) ? S.grand[p] : 0.f;
+        if (have && grand < b.best) {
+            S.grand[p] = b.best;
+            if (MODE == RM_PROD1) { S.sfragndx[p] = b.ndx; S.sfragpos[p] = b.pos; }
+            float sq = b.best, sr = b.gbest;
+            if (rlen < qlen) { float t = sq; sq = sr; sr = t; }
+            float* o = S.out + (size_t)p * 15;
+            o[9] = sq / (float)qlen;
+            o[10] = sr / (float)rlen;
+            o[11] = GetD0fin(qlen, qlen, Q.na[qi]);
+            o[12] = GetD0fin(rlen, rlen, R.na[ri]);
+            copy12(tg, b.tfm);
+        }
+    }
+    if (MODE == RM_DPREFINE && prm.epi) {
+        int fl = S.flags[p];
+        if (0 < prm.dpi && !(fl & (CONV_SCOREDP | CONV_LOWTMSC)) &&
+            fabsf(S.best0[p] - S.prevbest0[p]) < SCORE_CONVEPSILON)
+            fl |= CONV_SCOREDP;
+        if (prm.dpi + 1 < prm.maxndpiters && !(fl & (CONV_SCOREDP | CONV_LOWTMSC))) S.prevbest0[p] = S.best0[p];
+        if (prm.maxndpiters <= prm.dpi + 1) {
+            if (0.0f < prm.prescore && S.grand[p] < prm.prescore * (float)min(qlen, rlen)) fl |= CONV_LOWTMSC;
+            fl &= ~CONV_SCOREDP;
+        }
+        S.flags[p] = fl;
+    }
+}
+
+// LDS of a refinement workgroup
+template <int P>
+struct RefineLds {
+    SlotBest slds[NGROUPS];
+    float tbest[NGROUPS][12];
+    typename GPrec<P>::T tlast[NGROUPS][16];
+    int ctr;
+};
+
 template <int P, int MODE>
-__global__ __launch_bounds__(RTHREADS) void k_refine_dp(DevChunk Q, DevChunk R, DevState S, Geom G,
-                                                         RefineParams prm, int ldscap) {
-    extern __shared__ float lds_rf[];
-    __shared__ SlotBest slds[NGROUPS];
-    __shared__ float tbest[NGROUPS][12];
-    __shared__ typename GPrec<P>::T tlast[NGROUPS][16];
-    __shared__ int ctr;
+__device__ void refine_pair(const DevChunk& Q, const DevChunk& R, const DevState& S, const Geom& G,
+                            const RefineParams& prm, int ldscap, int p, SlotBest* parts, float* lds_rf,
+                            RefineLds<P>& rl) {
+    SlotBest* slds = rl.slds;
+    float (*tbest)[12] = rl.tbest;
+    typename GPrec<P>::T (*tlast)[16] = rl.tlast;
+    int& ctr = rl.ctr;
     typedef typename GPrec<P>::T T;
-    const int p = blockIdx.x;
     const int qi = pair_q(S, p), ri = pair_r(S, p);
     const int qlen = Q.len[qi], rlen = R.len[ri];
     const int flags = S.flags[p];
@@ -832,7 +1068,6 @@ __global__ __launch_bounds__(RTHREADS) void k_refine_dp(DevChunk Q, DevChunk R, 
     Aln a;
     const float* g = S.aln + base;
     const size_t pl = S.alnplane;
-    if (threadIdx.x == 0) ctr = 0;
     if (!skip && L <= ldscap) {
         for (int k = threadIdx.x; k < L; k += RTHREADS) {
             lds_rf[k] = g[k]; lds_rf[ldscap + k] = g[pl + k]; lds_rf[2 * ldscap + k] = g[2 * pl + k];
@@ -859,6 +1094,10 @@ __global__ __launch_bounds__(RTHREADS) void k_refine_dp(DevChunk Q, DevChunk R, 
     const int nslots = skip ? 0 : (phase2 ? G.maxnfragfcts : G.nlocsteps);
     const int base_ndx = phase2 ? S.sfragndx[p] : 0;
     const int base_pos = phase2 ? S.sfragpos[p] : 0;
+    int slo, shi;
+    part_range(0, nslots, blockIdx.y, gridDim.y, slo, shi);
+    if (threadIdx.x == 0) ctr = slo;
+    __syncthreads();
 
     // per group: its best slot and the current slot's best transform, in LDS
     const int grp = threadIdx.x / GL;
@@ -866,7 +1105,7 @@ __global__ __launch_bounds__(RTHREADS) void k_refine_dp(DevChunk Q, DevChunk R, 
     float* tfmBest = tbest[grp];
     if (glane() == 0) mine.clear();
     unsigned long long ab = 0;
-    bool need = nslots > 0, live = false;
+    bool need = slo < shi, live = false;
     int si = 0, cit = 0, fl = 0, sfragndx = 0, sfragpos = 0;
     float tfm[12], thr = CP_LARGEDST, best = 0.f;
     T* last = tlast[grp];       // previous covariance record (15 sums, count)
@@ -874,7 +1113,7 @@ __global__ __launch_bounds__(RTHREADS) void k_refine_dp(DevChunk Q, DevChunk R, 
         for (;;) {
             const int got = fetch_slot(need, &ctr);
             if (need) {
-                if (got >= nslots) {
+                if (got >= shi) {
                     need = false;
                 } else {
                     si = got;
@@ -966,48 +1205,38 @@ __global__ __launch_bounds__(RTHREADS) void k_refine_dp(DevChunk Q, DevChunk R, 
     if (glane() == 0 && ab) atomicAdd(S.algbytes + (MODE == RM_DPREFINE ? KC_REFINE_DP : KC_REFINE_FIN), ab);
     SlotBest b;
     group_best_lds(slds, b);
-    if (threadIdx.x != 0) return;
-    const bool have = b.best > 0.f;
-    float* tg = S.tfm_grand + (size_t)p * 12;
-    if (!prod) {
-        S.best0[p] = have ? b.best : 0.f;
-        if (have) copy12(S.tfm_ibest + (size_t)p * 12, b.tfm);
-        if (MODE == RM_FIN1) { S.sfragndx[p] = have ? b.ndx : 0; S.sfragpos[p] = have ? b.pos : 0; }
-        if (have && S.grand[p] < b.best) { S.grand[p] = b.best; copy12(tg, b.tfm); }
-    } else {
-        const float grand = (MODE == RM_PROD2) ? S.grand[p] : 0.f;
-        if (have && grand < b.best) {
-            S.grand[p] = b.best;
-            if (MODE == RM_PROD1) { S.sfragndx[p] = b.ndx; S.sfragpos[p] = b.pos; }
-            float sq = b.best, sr = b.gbest;
-            if (rlen < qlen) { float t = sq; sq = sr; sr = t; }
-            float* o = S.out + (size_t)p * 15;
-            o[9] = sq / (float)qlen;
-            o[10] = sr / (float)rlen;
-            o[11] = GetD0fin(qlen, qlen, Q.na[qi]);
-            o[12] = GetD0fin(rlen, rlen, rna);
-            copy12(tg, b.tfm);
-        }
+    if (threadIdx.x == 0) {
+        if (parts) parts[(size_t)blockIdx.x * gridDim.y + blockIdx.y] = b;
+        else refine_dp_epilogue<MODE>(Q, R, S, prm, p, b);
     }
-    if (MODE == RM_DPREFINE && prm.epi) {
-        int fl = S.flags[p];
-        if (0 < prm.dpi && !(fl & (CONV_SCOREDP | CONV_LOWTMSC)) &&
-            fabsf(S.best0[p] - S.prevbest0[p]) < SCORE_CONVEPSILON)
-            fl |= CONV_SCOREDP;
-        if (prm.dpi + 1 < prm.maxndpiters && !(fl & (CONV_SCOREDP | CONV_LOWTMSC))) S.prevbest0[p] = S.best0[p];
-        if (prm.maxndpiters <= prm.dpi + 1) {
-            if (0.0f < prm.prescore && S.grand[p] < prm.prescore * (float)min(qlen, rlen)) fl |= CONV_LOWTMSC;
-            fl &= ~CONV_SCOREDP;
+}
+
+// One workgroup per pair (plist[b], or b), each pair's slots split over
+// gridDim.y workgroups when parts is given, or a persistent grid draining a
+// work queue of pairs.
+template <int P, int MODE>
+__global__ __launch_bounds__(RTHREADS) void k_refine_dp(DevChunk Q, DevChunk R, DevState S, Geom G,
+                                                         RefineParams prm, int ldscap, const int* plist,
+                                                         SlotBest* parts, WorkQ wq) {
+    extern __shared__ float lds_rf[];
+    __shared__ RefineLds<P> rl;
+    __shared__ int item;
+    if (wq.items) {
+        const int n = *wq.count;
+        for (int it = wq_next_block(wq, &item); it < n; it = wq_next_block(wq, &item)) {
+            refine_pair<P, MODE>(Q, R, S, G, prm, ldscap, wq.items[it], nullptr, lds_rf, rl);
+            __syncthreads();
         }
-        S.flags[p] = fl;
+    } else {
+        refine_pair<P, MODE>(Q, R, S, G, prm, ldscap, plist ? plist[blockIdx.x] : (int)blockIdx.x, parts,
+                             lds_rf, rl);
     }
 }
 
 // ============================================================================
 // production alignment rows and statistics, one wave per pair
-__global__ __launch_bounds__(GTA_WAVE) void k_prod_aln(DevChunk Q, DevChunk R, DevState S, char* alns,
-                                                       int nodel) {
-    const int p = blockIdx.x;
+__device__ void prod_aln_pair(const DevChunk& Q, const DevChunk& R, const DevState& S, char* alns, int nodel,
+                              int p) {
     const int qi = pair_q(S, p), ri = pair_r(S, p);
     if (S.flags[p] & CONV_LOWTMSC) return;
     const int lane = lane_id();
@@ -1067,6 +1296,79 @@ __global__ __launch_bounds__(GTA_WAVE) void k_prod_aln(DevChunk Q, DevChunk R, D
     }
 }
 
+__global__ __launch_bounds__(GTA_WAVE) void k_prod_aln(DevChunk Q, DevChunk R, DevState S, char* alns,
+                                                       int nodel, WorkQ wq) {
+    if (wq.items) {
+        const int n = *wq.count;
+        for (int it = wq_next_wave(wq); it < n; it = wq_next_wave(wq)) prod_aln_pair(Q, R, S, alns, nodel, wq.items[it]);
+    } else {
+        prod_aln_pair(Q, R, S, alns, nodel, (int)blockIdx.x);
+    }
+}
+
+// ============================================================================
+// work lists of live pairs (no CONV_LOWTMSC), in pair order, for the batched
+// form: list A for the one-wave DP (pairs of the multi-wave bucket left out,
+// bigmask bit 0), list B for one-workgroup refinement (split pairs left out,
+// bit 1).  Three passes: per-block counts, a scan, the scatter.
+constexpr int LBLK = 1024;
+
+__device__ __forceinline__ void live_flags(const DevState& S, const unsigned char* bigmask, int p, bool& a,
+                                           bool& b) {
+    const int np = S.nq * S.nr;
+    const bool live = p < np && !(S.flags[p] & CONV_LOWTMSC);
+    const unsigned m = (bigmask && p < np) ? bigmask[p] : 0u;
+    a = live && !(m & 1u);
+    b = live && !(m & 2u);
+}
+
+__global__ __launch_bounds__(LBLK) void k_live_count(DevState S, const unsigned char* bigmask, int* cnt) {
+    const int p = blockIdx.x * LBLK + threadIdx.x;
+    bool a, b;
+    live_flags(S, bigmask, p, a, b);
+    const int ca = __syncthreads_count(a);
+    const int cb = __syncthreads_count(b);
+    if (threadIdx.x == 0) { cnt[2 * blockIdx.x] = ca; cnt[2 * blockIdx.x + 1] = cb; }
+}
+
+// exclusive scan of the per-block counts (one thread; nblk is np/1024) and
+// the totals of both lists
+__global__ void k_live_scan(int* cnt, int nblk, int* totals) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int sa = 0, sb = 0;
+    for (int i = 0; i < nblk; i++) {
+        const int ca = cnt[2 * i], cb = cnt[2 * i + 1];
+        cnt[2 * i] = sa; cnt[2 * i + 1] = sb;
+        sa += ca; sb += cb;
+    }
+    totals[0] = sa;
+    totals[1] = sb;
+}
+
+__global__ __launch_bounds__(LBLK) void k_live_scatter(DevState S, const unsigned char* bigmask, const int* off,
+                                                       int* lista, int* listb) {
+    __shared__ int wa[LBLK / GTA_WAVE], wb[LBLK / GTA_WAVE];
+    const int p = blockIdx.x * LBLK + threadIdx.x;
+    bool a, b;
+    live_flags(S, bigmask, p, a, b);
+    const unsigned long long ba = __ballot(a), bb = __ballot(b);
+    const int w = threadIdx.x / GTA_WAVE, lane = lane_id();
+    if (lane == 0) { wa[w] = __popcll(ba); wb[w] = __popcll(bb); }
+    __syncthreads();
+    int oa = off[2 * blockIdx.x], ob = off[2 * blockIdx.x + 1];
+    for (int i = 0; i < w; i++) { oa += wa[i]; ob += wb[i]; }
+    const unsigned long long below = (1ull << lane) - 1ull;
+    if (a) lista[oa + __popcll(ba & below)] = p;
+    if (b) listb[ob + __popcll(bb & below)] = p;
+}
+
+// fills n ints with v (a kernel rather than hipMemsetAsync: the batched form
+// replays the launch sequence as a HIP graph)
+__global__ void k_fill_i32(int* d, int v, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) d[i] = v;
+}
+
 // ============================================================================
 __global__ void k_setlow(DevChunk Q, DevChunk R, DevState S, float thr) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1107,18 +1409,68 @@ hipError_t launch_init(const DevChunk& Q, const DevChunk& R, const DevState& S, 
     hipLaunchKernelGGL(k_init, dim3(blocks_for(np, 256)), dim3(256), 0, st, Q, R, S);
     return hipGetLastError();
 }
+// Part records of split pairs merged (highest score, lowest slot on ties) and
+// the kernel's epilogue applied: KIND 0 seed search, 1 gapless refinement,
+// 2 DP refinement in mode MODE.  One thread per split pair.
+template <int KIND, int MODE>
+__global__ void k_merge_parts(DevChunk Q, DevChunk R, DevState S, RefineParams prm, const int* plist,
+                              int nitems, int nparts, const SlotBest* parts) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nitems) return;
+    const int p = plist[i];
+    const int flags = S.flags[p];
+    if (KIND < 2 && (flags & CONV_LOWTMSC)) {
+        S.best0[p] = 0.f;
+        return;
+    }
+    if (KIND == 2 && MODE != RM_DPREFINE && (flags & CONV_LOWTMSC)) return;
+    SlotBest b = parts[(size_t)i * nparts];
+    for (int k = 1; k < nparts; k++) {
+        const SlotBest& c = parts[(size_t)i * nparts + k];
+        if (b.beats(c.best, c.slot)) b = c;
+    }
+    if (KIND == 0) findfrag_epilogue(S, p, b);
+    else if (KIND == 1) gapless_epilogue(S, p, b);
+    else refine_dp_epilogue<MODE>(Q, R, S, prm, p, b);
+}
+
+static int nparts_for(int nslots, int per) { return max(1, (nslots + per - 1) / per); }
+size_t part_record_bytes() { return sizeof(SlotBest); }
+
 hipError_t launch_findfrag(int prec, const DevChunk& Q, const DevChunk& R, const DevState& S, const Geom& G,
-                           int dinv, hipStream_t st) {
-    const int np = S.nq * S.nr;
-    if (prec) hipLaunchKernelGGL(k_findfrag<1>, dim3(np), dim3(RTHREADS), 0, st, Q, R, S, G, dinv);
-    else hipLaunchKernelGGL(k_findfrag<0>, dim3(np), dim3(RTHREADS), 0, st, Q, R, S, G, dinv);
+                           int dinv, const RefSplit& Sp, hipStream_t st) {
+    const int nsmall = Sp.small ? Sp.nsmall : S.nq * S.nr;
+    if (nsmall > 0) {
+        if (prec) hipLaunchKernelGGL(k_findfrag<1>, dim3(nsmall), dim3(RTHREADS), 0, st, Q, R, S, G, dinv, Sp.small, nullptr);
+        else hipLaunchKernelGGL(k_findfrag<0>, dim3(nsmall), dim3(RTHREADS), 0, st, Q, R, S, G, dinv, Sp.small, nullptr);
+    }
+    if (Sp.nbig > 0) {
+        const int np = nparts_for(G.nseeds, Sp.seeds_per_part);
+        SlotBest* parts = (SlotBest*)Sp.partbuf;
+        if (prec) hipLaunchKernelGGL(k_findfrag<1>, dim3(Sp.nbig, np), dim3(RTHREADS), 0, st, Q, R, S, G, dinv, Sp.big, parts);
+        else hipLaunchKernelGGL(k_findfrag<0>, dim3(Sp.nbig, np), dim3(RTHREADS), 0, st, Q, R, S, G, dinv, Sp.big, parts);
+        RefineParams prm{};
+        hipLaunchKernelGGL((k_merge_parts<0, 0>), dim3((Sp.nbig + 63) / 64), dim3(64), 0, st, Q, R, S, prm, Sp.big,
+                           Sp.nbig, np, parts);
+    }
     return hipGetLastError();
 }
 hipError_t launch_refine_gapless(int prec, const DevChunk& Q, const DevChunk& R, const DevState& S,
-                                 const Geom& G, int dinv, hipStream_t st) {
-    const int np = S.nq * S.nr;
-    if (prec) hipLaunchKernelGGL(k_refine_gapless<1>, dim3(np), dim3(RTHREADS), 0, st, Q, R, S, G, dinv);
-    else hipLaunchKernelGGL(k_refine_gapless<0>, dim3(np), dim3(RTHREADS), 0, st, Q, R, S, G, dinv);
+                                 const Geom& G, int dinv, const RefSplit& Sp, hipStream_t st) {
+    const int nsmall = Sp.small ? Sp.nsmall : S.nq * S.nr;
+    if (nsmall > 0) {
+        if (prec) hipLaunchKernelGGL(k_refine_gapless<1>, dim3(nsmall), dim3(RTHREADS), 0, st, Q, R, S, G, dinv, Sp.small, nullptr);
+        else hipLaunchKernelGGL(k_refine_gapless<0>, dim3(nsmall), dim3(RTHREADS), 0, st, Q, R, S, G, dinv, Sp.small, nullptr);
+    }
+    if (Sp.nbig > 0) {
+        const int np = nparts_for(G.nlocsteps, Sp.slots_per_part);
+        SlotBest* parts = (SlotBest*)Sp.partbuf;
+        if (prec) hipLaunchKernelGGL(k_refine_gapless<1>, dim3(Sp.nbig, np), dim3(RTHREADS), 0, st, Q, R, S, G, dinv, Sp.big, parts);
+        else hipLaunchKernelGGL(k_refine_gapless<0>, dim3(Sp.nbig, np), dim3(RTHREADS), 0, st, Q, R, S, G, dinv, Sp.big, parts);
+        RefineParams prm{};
+        hipLaunchKernelGGL((k_merge_parts<1, 0>), dim3((Sp.nbig + 63) / 64), dim3(64), 0, st, Q, R, S, prm, Sp.big,
+                           Sp.nbig, np, parts);
+    }
     return hipGetLastError();
 }
 namespace {
@@ -1142,21 +1494,49 @@ hipError_t upload_length_tables() {
 }
 
 hipError_t launch_dp(const DevChunk& Q, const DevChunk& R, const DevState& S, const DpLaunch& L,
-                     hipStream_t st) {
-    const int np = S.nq * S.nr;
+                     const DpBuckets& B, const WorkQ& wq, int pgrid, hipStream_t st) {
     DpParams prm{L.mode, L.gap0, L.gapopen, L.usess, L.ssw, L.dpscan};
-    if (L.usess)
-        hipLaunchKernelGGL(k_dp<true>, dim3(np), dim3(GTA_WAVE), 0, st, Q, R, S, prm);
-    else
-        hipLaunchKernelGGL(k_dp<false>, dim3(np), dim3(GTA_WAVE), 0, st, Q, R, S, prm);
+    const int nsmall = wq.items ? pgrid : (B.small ? B.nsmall : S.nq * S.nr);
+    if (nsmall > 0) {
+        if (L.usess)
+            hipLaunchKernelGGL(k_dp<true>, dim3(nsmall), dim3(GTA_WAVE), 0, st, Q, R, S, prm, B.small, wq);
+        else
+            hipLaunchKernelGGL(k_dp<false>, dim3(nsmall), dim3(GTA_WAVE), 0, st, Q, R, S, prm, B.small, wq);
+    }
+    if (B.nbig > 0) {
+#define GTA_MW(US, WW)                                                                                 \
+    hipLaunchKernelGGL((k_dp_mw<US, WW>), dim3(B.nbig), dim3(WW * GTA_WAVE), 0, st, Q, R, S, prm, B.big, \
+                       B.ring, B.ringoff, B.failflag)
+        if (L.usess) {
+            if (B.waves >= 16) GTA_MW(true, 16); else if (B.waves >= 8) GTA_MW(true, 8); else GTA_MW(true, 4);
+        } else {
+            if (B.waves >= 16) GTA_MW(false, 16); else if (B.waves >= 8) GTA_MW(false, 8); else GTA_MW(false, 4);
+        }
+#undef GTA_MW
+    }
     return hipGetLastError();
 }
 hipError_t launch_refine_dp(int prec, int mode, const DevChunk& Q, const DevChunk& R, const DevState& S,
-                            const Geom& G, const RefineLaunch& L, hipStream_t st) {
-    const int np = S.nq * S.nr;
+                            const Geom& G, const RefineLaunch& L, const RefSplit& Sp, const WorkQ& wq,
+                            int pgrid, hipStream_t st) {
     RefineParams prm{L.dinv, L.epi, L.dpi, L.maxndpiters, L.prescore};
     const size_t lds = (size_t)L.ldscap * 6 * sizeof(float);
-#define GTA_RL(PP, MM) hipLaunchKernelGGL((k_refine_dp<PP, MM>), dim3(np), dim3(RTHREADS), lds, st, Q, R, S, G, prm, L.ldscap)
+    const int nsmall = wq.items ? pgrid : (Sp.small ? Sp.nsmall : S.nq * S.nr);
+    const bool phase2 = (mode == RM_FIN2 || mode == RM_PROD2);
+    const int np = Sp.nbig > 0 ? nparts_for(phase2 ? G.maxnfragfcts : G.nlocsteps, Sp.slots_per_part) : 1;
+    SlotBest* parts = (SlotBest*)Sp.partbuf;
+#define GTA_RL(PP, MM)                                                                                        \
+    do {                                                                                                      \
+        if (nsmall > 0)                                                                                       \
+            hipLaunchKernelGGL((k_refine_dp<PP, MM>), dim3(nsmall), dim3(RTHREADS), lds, st, Q, R, S, G, prm, \
+                               L.ldscap, Sp.small, (SlotBest*)nullptr, wq);                                   \
+        if (Sp.nbig > 0) {                                                                                    \
+            hipLaunchKernelGGL((k_refine_dp<PP, MM>), dim3(Sp.nbig, np), dim3(RTHREADS), lds, st, Q, R, S, G, \
+                               prm, L.ldscap, Sp.big, parts, WorkQ{});                                        \
+            hipLaunchKernelGGL((k_merge_parts<2, MM>), dim3((Sp.nbig + 63) / 64), dim3(64), 0, st, Q, R, S,  \
+                               prm, Sp.big, Sp.nbig, np, parts);                                              \
+        }                                                                                                     \
+    } while (0)
 #define GTA_RLM(PP)                                      \
     switch (mode) {                                      \
         case RM_DPREFINE: GTA_RL(PP, RM_DPREFINE); break; \
@@ -1171,9 +1551,22 @@ hipError_t launch_refine_dp(int prec, int mode, const DevChunk& Q, const DevChun
     return hipGetLastError();
 }
 hipError_t launch_prod_aln(const DevChunk& Q, const DevChunk& R, const DevState& S, char* alns, int nodel,
-                           hipStream_t st) {
-    const int np = S.nq * S.nr;
-    hipLaunchKernelGGL(k_prod_aln, dim3(np), dim3(GTA_WAVE), 0, st, Q, R, S, alns, nodel);
+                           const WorkQ& wq, int pgrid, hipStream_t st) {
+    const int n = wq.items ? pgrid : S.nq * S.nr;
+    hipLaunchKernelGGL(k_prod_aln, dim3(n), dim3(GTA_WAVE), 0, st, Q, R, S, alns, nodel, wq);
+    return hipGetLastError();
+}
+hipError_t launch_live_lists(const DevState& S, const unsigned char* bigmask, int* cnt, int* totals, int* lista,
+                             int* listb, hipStream_t st) {
+    const long long np = (long long)S.nq * S.nr;
+    const int nblk = (int)((np + LBLK - 1) / LBLK);
+    hipLaunchKernelGGL(k_live_count, dim3(nblk), dim3(LBLK), 0, st, S, bigmask, cnt);
+    hipLaunchKernelGGL(k_live_scan, dim3(1), dim3(64), 0, st, cnt, nblk, totals);
+    hipLaunchKernelGGL(k_live_scatter, dim3(nblk), dim3(LBLK), 0, st, S, bigmask, cnt, lista, listb);
+    return hipGetLastError();
+}
+hipError_t launch_fill_i32(int* d, int v, size_t n, hipStream_t st) {
+    if (n) hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for((long long)n, 256)), dim3(256), 0, st, d, v, n);
     return hipGetLastError();
 }
 hipError_t launch_setlow(const DevChunk& Q, const DevChunk& R, const DevState& S, float thr, hipStream_t st) {

@@ -18,6 +18,34 @@ struct DpLaunch {
     int dpscan;
 };
 
+// Pairs of a chunk split by DP size: the one-wave kernel takes `small`
+// (nullptr: every pair), the W-wave kernel takes `big` with W x Lr floats
+// of ring rows per pair at ringoff[i].
+struct DpBuckets {
+    const int* small;
+    int nsmall;
+    const int* big;
+    int nbig;
+    int waves;         // W: 4, 8 or 16
+    float* ring;
+    const unsigned long long* ringoff;
+    int* failflag;     // set by a W-wave fill whose row wait gave up
+};
+
+// Pairs of a chunk split by refinement size: `small` pairs (nullptr: every
+// pair) take one workgroup each; each `big` pair's slots are split over
+// several workgroups whose part records (partbuf, nbig x parts entries) are
+// merged by a second kernel.
+struct RefSplit {
+    const int* small;
+    int nsmall;
+    const int* big;
+    int nbig;
+    int seeds_per_part;    // seed diagonals per workgroup in the seed search
+    int slots_per_part;    // fragment slots per workgroup in the refinements
+    void* partbuf;
+};
+
 struct RefineLaunch {
     int dinv;
     int epi, dpi, maxndpiters;
@@ -28,17 +56,27 @@ struct RefineLaunch {
 hipError_t launch_secstr(const DevChunk& C, hipStream_t st);
 hipError_t launch_init(const DevChunk& Q, const DevChunk& R, const DevState& S, hipStream_t st);
 hipError_t launch_findfrag(int prec, const DevChunk& Q, const DevChunk& R, const DevState& S,
-                           const Geom& G, int dinv, hipStream_t st);
+                           const Geom& G, int dinv, const RefSplit& Sp, hipStream_t st);
 hipError_t launch_refine_gapless(int prec, const DevChunk& Q, const DevChunk& R, const DevState& S,
-                                 const Geom& G, int dinv, hipStream_t st);
+                                 const Geom& G, int dinv, const RefSplit& Sp, hipStream_t st);
 // per-length d0/d8 tables on the current device (once per device, before any launch)
 hipError_t upload_length_tables();
+// wq.items != nullptr: a persistent grid of pgrid workgroups drains the
+// work queue instead of the small bucket (batched form)
 hipError_t launch_dp(const DevChunk& Q, const DevChunk& R, const DevState& S, const DpLaunch& L,
-                     hipStream_t st);
+                     const DpBuckets& B, const WorkQ& wq, int pgrid, hipStream_t st);
 hipError_t launch_refine_dp(int prec, int mode, const DevChunk& Q, const DevChunk& R,
-                            const DevState& S, const Geom& G, const RefineLaunch& L, hipStream_t st);
+                            const DevState& S, const Geom& G, const RefineLaunch& L, const RefSplit& Sp,
+                            const WorkQ& wq, int pgrid, hipStream_t st);
+// bytes of one part record (partbuf entries)
+size_t part_record_bytes();
 hipError_t launch_prod_aln(const DevChunk& Q, const DevChunk& R, const DevState& S, char* alns,
-                           int nodel, hipStream_t st);
+                           int nodel, const WorkQ& wq, int pgrid, hipStream_t st);
+// work lists of live pairs (batched form): lista for the one-wave DP, listb for
+// one-workgroup refinement; cnt holds 2 ints per 1024 pairs, totals 2 ints
+hipError_t launch_live_lists(const DevState& S, const unsigned char* bigmask, int* cnt, int* totals, int* lista,
+                             int* listb, hipStream_t st);
+hipError_t launch_fill_i32(int* d, int v, size_t n, hipStream_t st);
 hipError_t launch_setlow(const DevChunk& Q, const DevChunk& R, const DevState& S, float thr,
                          hipStream_t st);
 hipError_t launch_revert(const DevState& S, hipStream_t st);

@@ -150,6 +150,15 @@ void gta_synth_chain(uint64_t seed, int L, float* xyz) {
 // random rotation and a translation of up to maxtrans A.  out must hold
 // maxout*3 floats; srcidx (optional) receives the source index of each output
 // atom (-1 for inserted atoms).  Returns the homolog length.
+// n chains at once (chain k of lens[k] residues from seeds[k] at
+// xyz + 3*offs[k]) on nthreads threads; each chain depends only on its seed,
+// so the result equals n calls of gta_synth_chain
+void gta_synth_chain_batch(const uint64_t* seeds, const int* lens, const long long* offs, int n, float* xyz,
+                           int nthreads) {
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads)
+    for (int k = 0; k < n; k++) gta_synth_chain(seeds[k], lens[k], xyz + 3 * offs[k]);
+}
+
 int gta_synth_homolog(uint64_t seed, const float* src, int Lsrc, float sigma, float maxtrans,
                       float del, float ins, float* out, int maxout, int* srcidx) {
     Rng g(seed);

@@ -31,7 +31,24 @@ def _synth():
         _lib.gta_synth_homolog.argtypes = [C.c_uint64, C.c_void_p, C.c_int, C.c_float, C.c_float,
                                            C.c_float, C.c_float, C.c_void_p, C.c_int, C.c_void_p]
         _lib.gta_synth_homolog.restype = C.c_int
+        _lib.gta_synth_chain_batch.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                                               C.c_int]
+        _lib.gta_synth_chain_batch.restype = None
     return _lib
+
+
+def chains(seeds, lens):
+    """chain(seeds[k], lens[k]) for every k, generated on the host's threads."""
+    seeds = np.ascontiguousarray(seeds, dtype=np.uint64)
+    lens = np.ascontiguousarray(lens, dtype=np.int32)
+    offs = np.zeros(len(lens), dtype=np.int64)
+    if len(lens) > 1:
+        offs[1:] = np.cumsum(lens[:-1], dtype=np.int64)
+    out = np.empty((int(lens.sum()), 3), dtype=np.float32)
+    nthreads = max(1, min(16, os.cpu_count() or 1))
+    _synth().gta_synth_chain_batch(seeds.ctypes.data, lens.ctypes.data, offs.ctypes.data, len(lens),
+                                   out.ctypes.data, nthreads)
+    return [out[o:o + L] for o, L in zip(offs, lens)]
 
 
 def chain(seed, L):
@@ -72,6 +89,7 @@ def make_set(seed, n, lens, queries, qres, hom_frac, sigma_lo, sigma_hi, del_hi,
     """n reference chains; a fraction hom_frac are homologs of a random query."""
     rng = np.random.default_rng(seed)
     structs, res, src = [], [], np.full(n, -1, dtype=np.int64)
+    later, lseeds, llens = [], [], []     # unrelated chains, generated in one batch
     for i in range(n):
         s = int(rng.integers(0, 2**62))
         if len(queries) and rng.random() < hom_frac:
@@ -84,8 +102,13 @@ def make_set(seed, n, lens, queries, qres, hom_frac, sigma_lo, sigma_hi, del_hi,
             src[i] = qi
         else:
             L = int(lens[i])
-            structs.append(chain(s, L))
+            structs.append(None)
+            later.append(i)
+            lseeds.append(s)
+            llens.append(L)
             res.append(residues(rng, L))
+    for i, xyz in zip(later, chains(np.array(lseeds, dtype=np.uint64), llens)):
+        structs[i] = xyz
     return structs, res, src
 
 
@@ -135,6 +158,26 @@ def rigid_copy(rng, x, sigma, maxtrans=50.0):
     t = rng.uniform(-maxtrans, maxtrans, 3)
     y = np.asarray(x, np.float64) @ R.T + t + rng.normal(scale=sigma, size=x.shape)
     return y.astype(np.float32)
+
+
+def assembly(rng, ndom=150, total=37860, min_sep=40.0):
+    """ndom generated domains (lengths summing to total, ~total/ndom each)
+    placed by random rigid motions with domain centres at least min_sep A
+    apart (no overlap), concatenated into one structure."""
+    base, extra = divmod(total, ndom)
+    lens = [base + 1] * extra + [base] * (ndom - extra)
+    side = (ndom * min_sep ** 3) ** (1.0 / 3.0) * 1.6
+    centres = []
+    while len(centres) < ndom:
+        c = rng.uniform(0, side, 3)
+        if all(np.linalg.norm(c - d) >= min_sep for d in centres):
+            centres.append(c)
+    parts = []
+    for L, c in zip(lens, centres):
+        x = chain(int(rng.integers(0, 2**62)), L).astype(np.float64)
+        x -= x.mean(0)
+        parts.append(rigid_copy(rng, x, 0.0, maxtrans=0.0).astype(np.float64) + c)
+    return np.concatenate(parts).astype(np.float32)
 
 
 def config(index, scale=1.0, seed=None):

@@ -89,6 +89,10 @@ typedef struct gta_options {
     int nodeletions;         /* --no-deletions (default 0) */
     int referenced;          /* --referenced  (default 0) */
     int precision;           /* gta_precision (default FP32) */
+    int batched;             /* execution form: 0 plain (one workgroup per pair in every launch),
+                                1 batched (after each filter the live pairs are listed on the device
+                                and persistent grids drain those lists; the launch sequence is
+                                captured once as a HIP graph and replayed) -- default 0 */
 } gta_options;
 
 /* results of one gta_process call; host memory owned by the context and

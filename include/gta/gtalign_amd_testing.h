@@ -25,6 +25,22 @@ extern "C" {
 int gta_test_dp(gta_ctx* ctx, const float* tfms, int gap0, float gapopen, int usess, int dpscan,
                 int* nmatched, int* qpos, int* rpos);
 
+/* Thresholds of the DP's size buckets: a pair is filled by a workgroup of
+ * several waves (strips pipelined down the query) when Lq*Lr >= min_cells,
+ * it has at least min_strips 64-row strips and Lr >= 256; otherwise by one
+ * wave.  Defaults 4,194,304 cells and 16 strips.  Takes effect at the next
+ * gta_process / gta_test_dp.  Lets tests drive the multi-wave fill on
+ * moderate sizes. */
+int gta_test_set_dp_buckets(gta_ctx* ctx, long long min_cells, int min_strips);
+
+/* Split of the superposition searches over workgroups: a pair whose shorter
+ * structure has at least min_len residues has its seed diagonals
+ * (seeds_per_part per workgroup) and fragment slots (slots_per_part per
+ * workgroup) spread over several workgroups, whose best slots are merged by
+ * a second kernel.  Defaults 2048, 256 and 32.  Takes effect at the next
+ * gta_process. */
+int gta_test_set_split(gta_ctx* ctx, int min_len, int seeds_per_part, int slots_per_part);
+
 #ifdef __cplusplus
 }
 #endif

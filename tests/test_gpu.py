@@ -299,6 +299,135 @@ def test_dp_kernel_long_reference():
             assert np.array_equal(got[i, j], want), (i, j)
 
 
+@pytest.mark.parametrize("lq,nstrips_min,waves", [(700, 2, 4), (1100, 16, 8), (2100, 32, 16)],
+                         ids=["W4", "W8", "W16"])
+def test_dp_multiwave_bit_exact(lq, nstrips_min, waves):
+    """Pairs filled by a workgroup of W waves (strips pipelined down the query
+    through ring rows) give the one-wave fill's alignments bit for bit."""
+    rng = np.random.default_rng(lq)
+    src = synth.chain(lq, lq)
+    keep = rng.random(lq) > 0.15
+    r0 = src[keep] + rng.normal(scale=1.0, size=(int(keep.sum()), 3)).astype(np.float32)
+    rs = [r0, synth.chain(lq + 1, 300), synth.chain(lq + 2, 900)]
+    q = Chunk.from_structures([src, synth.chain(lq + 3, 200)], ["A" * lq, "A" * 200])
+    r = Chunk.from_structures(rs, [synth.residues(rng, len(x)) for x in rs])
+    tf = np.zeros((q.nstrs, r.nstrs, 12), np.float32)
+    for i in range(q.nstrs):
+        for j in range(r.nstrs):
+            tf[i, j, :9] = np.eye(3).reshape(-1)
+            tf[i, j, 9:] = rng.normal(scale=1.0, size=3)
+    for gap0, gapopen in ((True, 0.0), (False, -0.6)):
+        with GtaBatch(0, default_options(), max_qrys=q.nstrs, max_qry_poss=q.nposs, max_refs=r.nstrs,
+                      max_ref_poss=r.nposs) as b:
+            b.set_dp_buckets(0, nstrips_min)
+            b.TransferQueryPMDataToDevice(q)
+            b.TransferCPMDataToDevice(r)
+            got = b.dp_under(tf, gap0, gapopen)
+        for i in range(q.nstrs):
+            for j in range(r.nstrs):
+                d02 = float(oracle.d0(int(q.len[i]), int(r.len[j]))["d0"]) ** 2
+                _, want = oracle.dp(q.coords(i), r.coords(j), tf[i, j], d02, gapopen, gap0)
+                assert np.array_equal(got[i, j], want), (gap0, i, j, int(q.len[i]), int(r.len[j]))
+
+
+def test_stage_with_multiwave_dp():
+    """The whole stage with every pair of >= 2 strips and >= 256 reference
+    residues in the multi-wave DP bucket equals the oracle."""
+    qc, rc, src = synth.config(1, scale=0.002)
+    q = qc.subset([0, 5])
+    r = rc.subset([i for i in range(rc.nstrs) if rc.len[i] >= 256][:30])
+    with GtaBatch(0, default_options(), max_qrys=q.nstrs, max_qry_poss=q.nposs, max_refs=r.nstrs,
+                  max_ref_poss=r.nposs) as b:
+        b.set_dp_buckets(0, 2)
+        b.TransferQueryPMDataToDevice(q)
+        b.TransferCPMDataToDevice(r)
+        g = b.ProcessBlock()
+    assert_same(oracle.process_block(q, r), g, q, r)
+
+
+@pytest.mark.parametrize("prec", [PREC_FP32, PREC_MIXED], ids=["fp32", "mixed"])
+def test_stage_with_split_searches(prec):
+    """Seed diagonals and fragment slots of every pair spread over several
+    workgroups (a few per workgroup) and merged: the same results as one
+    workgroup per pair (bit for bit) and, for FP32, as the oracle."""
+    qc, rc, _ = synth.config(1, scale=0.002)
+    q = qc.subset([1, 6])
+    r = rc.subset(list(range(0, rc.nstrs, 3)))
+    with GtaBatch(0, default_options(precision=prec), max_qrys=q.nstrs, max_qry_poss=q.nposs,
+                  max_refs=r.nstrs, max_ref_poss=r.nposs) as b:
+        b.set_split(1, 7, 3)
+        b.TransferQueryPMDataToDevice(q)
+        b.TransferCPMDataToDevice(r)
+        g = b.ProcessBlock()
+    ref = run_gpu(q, r, prec)
+    assert np.array_equal(g.alns, ref.alns)
+    assert np.array_equal(g.alndata, ref.alndata)
+    assert np.array_equal(g.tfms, ref.tfms)
+    if prec == PREC_FP32:
+        assert_same(oracle.process_block(q, r), g, q, r)
+
+
+def test_large_pair_default_buckets():
+    """An assembly of 10 domains (2,520 residues) against its noisy rigid copy
+    and an unrelated assembly: with the default thresholds these pairs take
+    the 16-wave DP and the split searches.  Equal to the oracle (FP32); the
+    mixed form matches FP64 recomputations."""
+    rng = np.random.default_rng(5)
+    a = synth.assembly(rng, ndom=10, total=2520)
+    b = synth.rigid_copy(rng, a, 0.05)
+    c = synth.assembly(rng, ndom=10, total=2400)
+    q = Chunk.from_structures([a], ["A" * len(a)])
+    r = Chunk.from_structures([b, c], ["A" * len(b), "A" * len(c)])
+    g = run_gpu(q, r)
+    assert_same(oracle.process_block(q, r), g, q, r)
+    assert g.alndata[0, 0, ALN_SCOREQ] > 0.99
+    gm = run_gpu(q, r, PREC_MIXED)
+    assert gm.alndata[0, 0, ALN_SCOREQ] > 0.99
+    assert check_fp64(gm, q, r, PREC_MIXED) >= 1
+
+
+@pytest.mark.parametrize("prec", [PREC_FP32, PREC_MIXED], ids=["fp32", "mixed"])
+def test_batched_form_equals_plain(prec):
+    """The batched form (live-pair lists built on the device after each filter,
+    persistent grids draining them, the launch sequence captured as a HIP graph
+    on the first call and replayed on the next ones) gives the plain form's
+    results bit for bit on every replay, with other device work between."""
+    qc, rc, _ = synth.config(1, scale=0.002)
+    q = qc.subset([0, 7, 15])
+    plain = run_gpu(q, rc, prec)
+    with GtaBatch(0, default_options(precision=prec, batched=1), max_qrys=q.nstrs, max_qry_poss=q.nposs,
+                  max_refs=rc.nstrs, max_ref_poss=rc.nposs) as b:
+        b.TransferQueryPMDataToDevice(q)
+        b.TransferCPMDataToDevice(rc)
+        for k in range(3):
+            g = b.ProcessBlock()
+            assert np.array_equal(g.alns, plain.alns), k
+            assert np.array_equal(g.alndata, plain.alndata), k
+            assert np.array_equal(g.tfms, plain.tfms), k
+            run_gpu(q, rc.subset(list(range(20))), prec)     # other work between replays
+
+
+def test_batched_form_with_large_pair_buckets():
+    """Batched form with every pair in the multi-wave DP bucket or split
+    searches where they apply: equal to the plain form."""
+    qc, rc, _ = synth.config(1, scale=0.002)
+    q = qc.subset([0, 5])
+    r = rc.subset([i for i in range(rc.nstrs) if rc.len[i] >= 256][:24] + [rc.nstrs - 1])
+    out = []
+    for batched in (0, 1):
+        with GtaBatch(0, default_options(batched=batched), max_qrys=q.nstrs, max_qry_poss=q.nposs,
+                      max_refs=r.nstrs, max_ref_poss=r.nposs) as b:
+            b.set_dp_buckets(0, 2)
+            b.set_split(300, 64, 8)
+            b.TransferQueryPMDataToDevice(q)
+            b.TransferCPMDataToDevice(r)
+            out.append([b.ProcessBlock() for _ in range(2)])
+    for g in out[1]:
+        assert np.array_equal(g.alns, out[0][0].alns)
+        assert np.array_equal(g.alndata, out[0][0].alndata)
+        assert np.array_equal(g.tfms, out[0][0].tfms)
+
+
 # ---- the boundary's error behaviour ----------------------------------------------------
 
 def _code(name):
