@@ -189,7 +189,10 @@ def config(index, scale=1.0, seed=None):
     index 3: precision stress, 20 query chains of straight and helical 3-8 residue
              segments (lengths 60-300) x 2,000 references; reference j is a copy of
              query j mod 20 under a random rigid motion with noise sigma 0.01-0.1 A
-    index 4: all-vs-all family set, lognormal(200, 0.5) in [40,800] (chunked by the caller)
+    index 4: all-vs-all family set: 500 families x 40 chains (seed lengths lognormal(200, 0.5)
+             in [40, 800]; members under rigid motions, noise 0.5-3 A, 0-15% indels) = 20,000
+             references; 16 queries (seed chains of evenly spaced families; scale < 1 gives
+             fewer queries)
     `scale` shrinks the number of references (tests and bounded CPU samples).
     Returns (queries, references, src): src[j] is the query-chunk position of
     the query that reference-chunk position j is a homolog (copy) of, or -1.
@@ -211,6 +214,28 @@ def config(index, scale=1.0, seed=None):
         qlens = np.array([400])
         rlens = lognormal_lengths(rng, nr, 220, 0.7, 20, 4000)
         hom, slo, shi, dhi, ihi = 0.1, 0.5, 3.0, 0.075, 0.075
+    elif index == 4:
+        # 500 families of 1 seed chain + 39 members (rigid motion, noise 0.5-3 A,
+        # 0-7.5% deletions and 0-7.5% insertions); all 20,000 chains are the
+        # references and the first chain of every 500/nq-th family a query
+        nfam, per = 500, 40
+        nq = max(1, int(round(16 * scale))) if scale < 1.0 else 16
+        seeds = lognormal_lengths(rng, nfam, 200, 0.5, 40, 800)
+        fs = chains(rng.integers(0, 2**62, nfam).astype(np.uint64), seeds)
+        fr = [residues(rng, int(L)) for L in seeds]
+        rs, rr, src = [], [], []
+        for f in range(nfam):
+            rs.append(fs[f]); rr.append(fr[f]); src.append(f)
+            for m in range(per - 1):
+                xyz, idx = homolog(int(rng.integers(0, 2**62)), fs[f], rng.uniform(0.5, 3.0), 50.0,
+                                   rng.uniform(0, 0.075), rng.uniform(0, 0.075))
+                rs.append(xyz); rr.append(_hom_residues(rng, fr[f], idx)); src.append(f)
+        qf = np.linspace(0, nfam - 1, nq).astype(int)
+        qs = [fs[f] for f in qf]
+        qr = [fr[f] for f in qf]
+        fam_to_q = np.full(nfam, -1, dtype=np.int64)
+        fam_to_q[qf] = np.arange(nq)
+        return _chunks(qs, qr, rs, rr, fam_to_q[np.asarray(src)])
     elif index == 3:
         nq, nr = 20, max(1, int(round(2000 * scale)))
         qs = [stress_chain(rng, int(L)) for L in rng.integers(60, 301, nq)]
