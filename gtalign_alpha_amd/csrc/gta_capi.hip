@@ -96,7 +96,7 @@ struct gta_ctx {
     int split_min_len = 2048;
     int split_seeds = 256, split_slots = 32;
     // batched form: live-pair work lists, queue counters, the captured graph
-    DevBuf s_bigmask, s_lista, s_listb, s_lcnt, s_ltot, s_heads;
+    DevBuf s_bigmask, s_lista, s_listb, s_listc, s_lcnt, s_ltot, s_heads;
     const unsigned char* bigmask = nullptr;
     int ncu = 256;
     hipGraphExec_t gexec = nullptr;
@@ -207,7 +207,7 @@ void gta_destroy(gta_ctx* c) {
                       &c->s_aln, &c->s_btck, &c->s_btckoff, &c->s_rowbuf, &c->s_cells, &c->s_alns,
                       &c->s_algb, &c->s_dpsmall, &c->s_dpbig, &c->s_ring, &c->s_ringoff, &c->s_fail,
                       &c->s_rsmall, &c->s_rbig, &c->s_parts, &c->s_bigmask, &c->s_lista, &c->s_listb,
-                      &c->s_lcnt, &c->s_ltot, &c->s_heads};
+                      &c->s_listc, &c->s_lcnt, &c->s_ltot, &c->s_heads};
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     for (DevBuf* b : bufs) b->release();
     for (hipEvent_t e : c->evpool) (void)hipEventDestroy(e);
@@ -427,8 +427,9 @@ static int prepare_state(gta_ctx* c, DevState& S, size_t& alnsize) {
         }
         HIPCHK(c, c->s_lista.ensure(np * 4));
         HIPCHK(c, c->s_listb.ensure(np * 4));
-        HIPCHK(c, c->s_lcnt.ensure(((np + 1023) / 1024) * 8));
-        HIPCHK(c, c->s_ltot.ensure(8));
+        if (c->dpb.nbig > 0) HIPCHK(c, c->s_listc.ensure(np * 4));
+        HIPCHK(c, c->s_lcnt.ensure(((np + 1023) / 1024) * 12));
+        HIPCHK(c, c->s_ltot.ensure(12));
         HIPCHK(c, c->s_heads.ensure(GTA_MAX_QUEUES * 4));
     }
     alnsize = gta_alignment_section_size(nq, c->q.nposs, nr, c->r.nposs);
@@ -505,6 +506,8 @@ static int run_stage(gta_ctx* c, double* seconds, double* cells) {
     int nqueue = 0;
     int* lista = batched ? c->s_lista.as<int>() : nullptr;
     int* listb = batched ? c->s_listb.as<int>() : nullptr;
+    // every live pair, when the one-wave DP list leaves some out
+    int* listc = batched && c->dpb.nbig > 0 ? c->s_listc.as<int>() : nullptr;
     int* ltot = batched ? c->s_ltot.as<int>() : nullptr;
     int* heads = batched ? c->s_heads.as<int>() : nullptr;
     const int pgrid_dp = c->ncu * 32, pgrid_rf = c->ncu * 4;
@@ -514,7 +517,7 @@ static int run_stage(gta_ctx* c, double* seconds, double* cells) {
     };
     auto relist = [&]() -> hipError_t {
         listed = true;
-        return launch_live_lists(S, c->bigmask, c->s_lcnt.as<int>(), ltot, lista, listb, st);
+        return launch_live_lists(S, c->bigmask, c->s_lcnt.as<int>(), ltot, lista, listb, listc, st);
     };
     auto dp = [&](int mode, int gap0, float gapopen, int usess, int dpscan) -> hipError_t {
         DpLaunch L{mode, gap0, gapopen, usess, usess ? 0.5f : 0.0f, dpscan};
@@ -573,7 +576,8 @@ static int run_stage(gta_ctx* c, double* seconds, double* cells) {
         if (!(o.depth == GTA_DEPTH_SHALLOW || o.refinement == GTA_REF_COARSE))
             GTA_TIMED(KC_REFINE_FIN, refine(RM_FIN2_L, 0, 0, 0, 0.f));
         GTA_TIMED(KC_DP, dp(DPM_CONSTRAINED_L, 1, 0.0f, 0, 0));
-        GTA_TIMED(KC_PROD_ALN, launch_prod_aln(Q, R, S, c->s_alns.as<char>(), o.nodeletions, queue(lista, ltot),
+        GTA_TIMED(KC_PROD_ALN, launch_prod_aln(Q, R, S, c->s_alns.as<char>(), o.nodeletions,
+                                               listc ? queue(listc, ltot + 2) : queue(lista, ltot),
                                                pgrid_dp, st));
         GTA_TIMED(KC_REFINE_FIN, refine(RM_PROD1_L, 0, 0, 0, 0.f));
         if (o.refinement == GTA_REF_ONE || o.refinement == GTA_REF_COARSE)

@@ -952,6 +952,24 @@ __device__ __forceinline__ void gcc_dp_frag(const Aln& a, int L, int fragpos, in
         for (int i = 0; i < 6; i++) sq[i] = gsum(s2[i]);
     }
 }
+// mean squared residual |R q + t - r|^2 over the fragment's pairs, in FP64
+// (the mixed-precision form's RMSD: no cancellation, unlike the E0 identity,
+// whose eigenvalues lose accuracy for near-collinear or near-planar sets)
+__device__ __forceinline__ double gresid_frag(const Aln& a, int L, int fragpos, int fl, const float* t) {
+    int ql = L, rl = L, qp = fragpos, rp = fragpos;
+    UpdateLengths(ql, rl, qp, rp, fl);
+    double s = 0.0;
+    for (int k = glane(); k < fl; k += GL) {
+        const int idx = L - 1 - (rp + k);
+        const double qx = a.qx[idx], qy = a.qy[idx], qz = a.qz[idx];
+        const double dx = (double)t[0] * qx + (double)t[1] * qy + (double)t[2] * qz + (double)t[9] - (double)a.rx[idx];
+        const double dy = (double)t[3] * qx + (double)t[4] * qy + (double)t[5] * qz + (double)t[10] - (double)a.ry[idx];
+        const double dz = (double)t[6] * qx + (double)t[7] * qy + (double)t[8] * qz + (double)t[11] - (double)a.rz[idx];
+        s += dx * dx + dy * dy + dz * dz;
+    }
+    s = gsum(s);
+    return fl > 0 ? s / (double)fl : 0.0;
+}
 // covariance of the aligned pairs within thr under tprev; returns the count
 template <int P>
 __device__ __forceinline__ int gcc_dp_ext(const Aln& a, int L, const float* tprev, float thr,
@@ -1151,7 +1169,7 @@ __device__ void refine_pair(const DevChunk& Q, const DevChunk& R, const DevState
             T ncc;
             bool done = false;
             if (cit == 0) {
-                if (MODE == RM_PROD1 && si == 0) {
+                if (MODE == RM_PROD1 && si == 0 && P == 0) {
                     // the RMSD of the whole alignment comes with the first slot's first pass
                     T sq[6];
                     gcc_dp_frag<P>(a, L, sfragpos, fl, cc, sq);
@@ -1177,6 +1195,13 @@ __device__ void refine_pair(const DevChunk& Q, const DevChunk& R, const DevState
                     last[15] = ncc;
                 }
                 kabsch<T>(cc, ncc, dinv, tfm);
+                if (P == 1 && MODE == RM_PROD1 && si == 0 && cit == 0) {
+                    // mixed precision: RMSD of the whole alignment as the FP64
+                    // residual under its own superposition (the first slot's
+                    // first pass covers the whole alignment)
+                    const double ms = gresid_frag(a, L, sfragpos, fl, tfm);
+                    if (glane() == 0) S.out[(size_t)p * 15 + 8] = (float)sqrt(ms);
+                }
                 const float sc = gscore_dp<P>(a, L, tfm, d0, d02, d82, chk, cit < 1, &thr);
                 ab += AB_TM * (unsigned long long)L + AB_SLOT;
                 if (best < sc) {
@@ -1313,53 +1338,65 @@ __global__ __launch_bounds__(GTA_WAVE) void k_prod_aln(DevChunk Q, DevChunk R, D
 // bit 1).  Three passes: per-block counts, a scan, the scatter.
 constexpr int LBLK = 1024;
 
+// list membership of pair p: a = live and filled by the one-wave DP, b = live
+// and refined by one workgroup, c = live (every pair that produces output)
 __device__ __forceinline__ void live_flags(const DevState& S, const unsigned char* bigmask, int p, bool& a,
-                                           bool& b) {
+                                           bool& b, bool& c) {
     const int np = S.nq * S.nr;
     const bool live = p < np && !(S.flags[p] & CONV_LOWTMSC);
     const unsigned m = (bigmask && p < np) ? bigmask[p] : 0u;
     a = live && !(m & 1u);
     b = live && !(m & 2u);
+    c = live;
 }
 
 __global__ __launch_bounds__(LBLK) void k_live_count(DevState S, const unsigned char* bigmask, int* cnt) {
     const int p = blockIdx.x * LBLK + threadIdx.x;
-    bool a, b;
-    live_flags(S, bigmask, p, a, b);
+    bool a, b, c;
+    live_flags(S, bigmask, p, a, b, c);
     const int ca = __syncthreads_count(a);
     const int cb = __syncthreads_count(b);
-    if (threadIdx.x == 0) { cnt[2 * blockIdx.x] = ca; cnt[2 * blockIdx.x + 1] = cb; }
+    const int cc = __syncthreads_count(c);
+    if (threadIdx.x == 0) {
+        cnt[3 * blockIdx.x] = ca;
+        cnt[3 * blockIdx.x + 1] = cb;
+        cnt[3 * blockIdx.x + 2] = cc;
+    }
 }
 
 // exclusive scan of the per-block counts (one thread; nblk is np/1024) and
-// the totals of both lists
+// the totals of the three lists
 __global__ void k_live_scan(int* cnt, int nblk, int* totals) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    int sa = 0, sb = 0;
+    int sa = 0, sb = 0, sc = 0;
     for (int i = 0; i < nblk; i++) {
-        const int ca = cnt[2 * i], cb = cnt[2 * i + 1];
-        cnt[2 * i] = sa; cnt[2 * i + 1] = sb;
-        sa += ca; sb += cb;
+        const int ca = cnt[3 * i], cb = cnt[3 * i + 1], cc = cnt[3 * i + 2];
+        cnt[3 * i] = sa; cnt[3 * i + 1] = sb; cnt[3 * i + 2] = sc;
+        sa += ca; sb += cb; sc += cc;
     }
     totals[0] = sa;
     totals[1] = sb;
+    totals[2] = sc;
 }
 
+// listc may be null: the caller then drains lista for the output pass (the
+// lists coincide when no pair is in the multi-wave DP bucket)
 __global__ __launch_bounds__(LBLK) void k_live_scatter(DevState S, const unsigned char* bigmask, const int* off,
-                                                       int* lista, int* listb) {
-    __shared__ int wa[LBLK / GTA_WAVE], wb[LBLK / GTA_WAVE];
+                                                       int* lista, int* listb, int* listc) {
+    __shared__ int wa[LBLK / GTA_WAVE], wb[LBLK / GTA_WAVE], wc[LBLK / GTA_WAVE];
     const int p = blockIdx.x * LBLK + threadIdx.x;
-    bool a, b;
-    live_flags(S, bigmask, p, a, b);
-    const unsigned long long ba = __ballot(a), bb = __ballot(b);
+    bool a, b, c;
+    live_flags(S, bigmask, p, a, b, c);
+    const unsigned long long ba = __ballot(a), bb = __ballot(b), bc = __ballot(c);
     const int w = threadIdx.x / GTA_WAVE, lane = lane_id();
-    if (lane == 0) { wa[w] = __popcll(ba); wb[w] = __popcll(bb); }
+    if (lane == 0) { wa[w] = __popcll(ba); wb[w] = __popcll(bb); wc[w] = __popcll(bc); }
     __syncthreads();
-    int oa = off[2 * blockIdx.x], ob = off[2 * blockIdx.x + 1];
-    for (int i = 0; i < w; i++) { oa += wa[i]; ob += wb[i]; }
+    int oa = off[3 * blockIdx.x], ob = off[3 * blockIdx.x + 1], oc = off[3 * blockIdx.x + 2];
+    for (int i = 0; i < w; i++) { oa += wa[i]; ob += wb[i]; oc += wc[i]; }
     const unsigned long long below = (1ull << lane) - 1ull;
     if (a) lista[oa + __popcll(ba & below)] = p;
     if (b) listb[ob + __popcll(bb & below)] = p;
+    if (c && listc) listc[oc + __popcll(bc & below)] = p;
 }
 
 // fills n ints with v (a kernel rather than hipMemsetAsync: the batched form
@@ -1557,12 +1594,12 @@ hipError_t launch_prod_aln(const DevChunk& Q, const DevChunk& R, const DevState&
     return hipGetLastError();
 }
 hipError_t launch_live_lists(const DevState& S, const unsigned char* bigmask, int* cnt, int* totals, int* lista,
-                             int* listb, hipStream_t st) {
+                             int* listb, int* listc, hipStream_t st) {
     const long long np = (long long)S.nq * S.nr;
     const int nblk = (int)((np + LBLK - 1) / LBLK);
     hipLaunchKernelGGL(k_live_count, dim3(nblk), dim3(LBLK), 0, st, S, bigmask, cnt);
     hipLaunchKernelGGL(k_live_scan, dim3(1), dim3(64), 0, st, cnt, nblk, totals);
-    hipLaunchKernelGGL(k_live_scatter, dim3(nblk), dim3(LBLK), 0, st, S, bigmask, cnt, lista, listb);
+    hipLaunchKernelGGL(k_live_scatter, dim3(nblk), dim3(LBLK), 0, st, S, bigmask, cnt, lista, listb, listc);
     return hipGetLastError();
 }
 hipError_t launch_fill_i32(int* d, int v, size_t n, hipStream_t st) {

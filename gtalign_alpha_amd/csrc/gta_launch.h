@@ -73,9 +73,10 @@ size_t part_record_bytes();
 hipError_t launch_prod_aln(const DevChunk& Q, const DevChunk& R, const DevState& S, char* alns,
                            int nodel, const WorkQ& wq, int pgrid, hipStream_t st);
 // work lists of live pairs (batched form): lista for the one-wave DP, listb for
-// one-workgroup refinement; cnt holds 2 ints per 1024 pairs, totals 2 ints
+// one-workgroup refinement, listc (optional) every live pair for the output
+// pass; cnt holds 3 ints per 1024 pairs, totals 3 ints
 hipError_t launch_live_lists(const DevState& S, const unsigned char* bigmask, int* cnt, int* totals, int* lista,
-                             int* listb, hipStream_t st);
+                             int* listb, int* listc, hipStream_t st);
 hipError_t launch_fill_i32(int* d, int v, size_t n, hipStream_t st);
 hipError_t launch_setlow(const DevChunk& Q, const DevChunk& R, const DevState& S, float thr,
                          hipStream_t st);
