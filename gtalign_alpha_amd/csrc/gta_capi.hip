@@ -551,8 +551,8 @@ static int run_stage(gta_ctx* c, double* seconds, double* cells) {
         GTA_TIMED(KC_SECSTR, launch_secstr(R, st));
         GTA_TIMED(KC_INIT, launch_init(Q, R, S, st));
         // stage 1 (MpStage1.h:78-85)
-        GTA_TIMED(KC_FINDFRAG, launch_findfrag(prec, Q, R, S, G, dinv, c->rsp, st));
-        GTA_TIMED(KC_REFINE_GAPLESS, launch_refine_gapless(prec, Q, R, S, G, dinv, c->rsp, st));
+        GTA_TIMED(KC_FINDFRAG, launch_findfrag(Q, R, S, G, dinv, c->rsp, st));
+        GTA_TIMED(KC_REFINE_GAPLESS, launch_refine_gapless(Q, R, S, G, dinv, c->rsp, st));
         int rc;
         if ((rc = dprefine(1, o.prescore, 2)) != GTA_OK) return rc;
         if (batched) HIPCHK(c, relist());

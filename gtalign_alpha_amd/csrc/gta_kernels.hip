@@ -1474,36 +1474,32 @@ __global__ void k_merge_parts(DevChunk Q, DevChunk R, DevState S, RefineParams p
 static int nparts_for(int nslots, int per) { return max(1, (nslots + per - 1) / per); }
 size_t part_record_bytes() { return sizeof(SlotBest); }
 
-hipError_t launch_findfrag(int prec, const DevChunk& Q, const DevChunk& R, const DevState& S, const Geom& G,
+hipError_t launch_findfrag(const DevChunk& Q, const DevChunk& R, const DevState& S, const Geom& G,
                            int dinv, const RefSplit& Sp, hipStream_t st) {
     const int nsmall = Sp.small ? Sp.nsmall : S.nq * S.nr;
     if (nsmall > 0) {
-        if (prec) hipLaunchKernelGGL(k_findfrag<1>, dim3(nsmall), dim3(RTHREADS), 0, st, Q, R, S, G, dinv, Sp.small, nullptr);
-        else hipLaunchKernelGGL(k_findfrag<0>, dim3(nsmall), dim3(RTHREADS), 0, st, Q, R, S, G, dinv, Sp.small, nullptr);
+        hipLaunchKernelGGL(k_findfrag<0>, dim3(nsmall), dim3(RTHREADS), 0, st, Q, R, S, G, dinv, Sp.small, nullptr);
     }
     if (Sp.nbig > 0) {
         const int np = nparts_for(G.nseeds, Sp.seeds_per_part);
         SlotBest* parts = (SlotBest*)Sp.partbuf;
-        if (prec) hipLaunchKernelGGL(k_findfrag<1>, dim3(Sp.nbig, np), dim3(RTHREADS), 0, st, Q, R, S, G, dinv, Sp.big, parts);
-        else hipLaunchKernelGGL(k_findfrag<0>, dim3(Sp.nbig, np), dim3(RTHREADS), 0, st, Q, R, S, G, dinv, Sp.big, parts);
+        hipLaunchKernelGGL(k_findfrag<0>, dim3(Sp.nbig, np), dim3(RTHREADS), 0, st, Q, R, S, G, dinv, Sp.big, parts);
         RefineParams prm{};
         hipLaunchKernelGGL((k_merge_parts<0, 0>), dim3((Sp.nbig + 63) / 64), dim3(64), 0, st, Q, R, S, prm, Sp.big,
                            Sp.nbig, np, parts);
     }
     return hipGetLastError();
 }
-hipError_t launch_refine_gapless(int prec, const DevChunk& Q, const DevChunk& R, const DevState& S,
+hipError_t launch_refine_gapless(const DevChunk& Q, const DevChunk& R, const DevState& S,
                                  const Geom& G, int dinv, const RefSplit& Sp, hipStream_t st) {
     const int nsmall = Sp.small ? Sp.nsmall : S.nq * S.nr;
     if (nsmall > 0) {
-        if (prec) hipLaunchKernelGGL(k_refine_gapless<1>, dim3(nsmall), dim3(RTHREADS), 0, st, Q, R, S, G, dinv, Sp.small, nullptr);
-        else hipLaunchKernelGGL(k_refine_gapless<0>, dim3(nsmall), dim3(RTHREADS), 0, st, Q, R, S, G, dinv, Sp.small, nullptr);
+        hipLaunchKernelGGL(k_refine_gapless<0>, dim3(nsmall), dim3(RTHREADS), 0, st, Q, R, S, G, dinv, Sp.small, nullptr);
     }
     if (Sp.nbig > 0) {
         const int np = nparts_for(G.nlocsteps, Sp.slots_per_part);
         SlotBest* parts = (SlotBest*)Sp.partbuf;
-        if (prec) hipLaunchKernelGGL(k_refine_gapless<1>, dim3(Sp.nbig, np), dim3(RTHREADS), 0, st, Q, R, S, G, dinv, Sp.big, parts);
-        else hipLaunchKernelGGL(k_refine_gapless<0>, dim3(Sp.nbig, np), dim3(RTHREADS), 0, st, Q, R, S, G, dinv, Sp.big, parts);
+        hipLaunchKernelGGL(k_refine_gapless<0>, dim3(Sp.nbig, np), dim3(RTHREADS), 0, st, Q, R, S, G, dinv, Sp.big, parts);
         RefineParams prm{};
         hipLaunchKernelGGL((k_merge_parts<1, 0>), dim3((Sp.nbig + 63) / 64), dim3(64), 0, st, Q, R, S, prm, Sp.big,
                            Sp.nbig, np, parts);
@@ -1576,7 +1572,7 @@ hipError_t launch_refine_dp(int prec, int mode, const DevChunk& Q, const DevChun
     } while (0)
 #define GTA_RLM(PP)                                      \
     switch (mode) {                                      \
-        case RM_DPREFINE: GTA_RL(PP, RM_DPREFINE); break; \
+        case RM_DPREFINE: GTA_RL(0, RM_DPREFINE); break;  \
         case RM_FIN1: GTA_RL(PP, RM_FIN1); break;         \
         case RM_FIN2: GTA_RL(PP, RM_FIN2); break;         \
         case RM_PROD1: GTA_RL(PP, RM_PROD1); break;       \

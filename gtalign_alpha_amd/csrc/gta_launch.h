@@ -55,9 +55,12 @@ struct RefineLaunch {
 
 hipError_t launch_secstr(const DevChunk& C, hipStream_t st);
 hipError_t launch_init(const DevChunk& Q, const DevChunk& R, const DevState& S, hipStream_t st);
-hipError_t launch_findfrag(int prec, const DevChunk& Q, const DevChunk& R, const DevState& S,
+// the superposition searches run in the reference's FP32 arithmetic in both
+// precision forms (their result is a choice of superposition, which the final
+// stage refines in the precision of the form)
+hipError_t launch_findfrag(const DevChunk& Q, const DevChunk& R, const DevState& S,
                            const Geom& G, int dinv, const RefSplit& Sp, hipStream_t st);
-hipError_t launch_refine_gapless(int prec, const DevChunk& Q, const DevChunk& R, const DevState& S,
+hipError_t launch_refine_gapless(const DevChunk& Q, const DevChunk& R, const DevState& S,
                                  const Geom& G, int dinv, const RefSplit& Sp, hipStream_t st);
 // per-length d0/d8 tables on the current device (once per device, before any launch)
 hipError_t upload_length_tables();
@@ -65,6 +68,8 @@ hipError_t upload_length_tables();
 // work queue instead of the small bucket (batched form)
 hipError_t launch_dp(const DevChunk& Q, const DevChunk& R, const DevState& S, const DpLaunch& L,
                      const DpBuckets& B, const WorkQ& wq, int pgrid, hipStream_t st);
+// prec selects FP64 covariance and solve for the final-stage modes (FIN1,
+// FIN2, PROD1, PROD2); DPREFINE is a search and stays FP32
 hipError_t launch_refine_dp(int prec, int mode, const DevChunk& Q, const DevChunk& R,
                             const DevState& S, const Geom& G, const RefineLaunch& L, const RefSplit& Sp,
                             const WorkQ& wq, int pgrid, hipStream_t st);

@@ -76,7 +76,8 @@ enum gta_depth { GTA_DEPTH_DEEP = 0, GTA_DEPTH_HIGH, GTA_DEPTH_MEDIUM, GTA_DEPTH
 enum gta_refinement { GTA_REF_LOG = 0, GTA_REF_COARSE, GTA_REF_ONE, GTA_REF_FULL, GTA_REF_FULLA };
 enum gta_precision {
     GTA_PREC_FP32 = 0,       /* plain form: FP32 throughout, as the reference */
-    GTA_PREC_MIXED = 1       /* compensated FP32 covariance + FP64 3x3 solve + FP64 RMSD */
+    GTA_PREC_MIXED = 1       /* searches as FP32; final refinement with FP64 covariance
+                                and 3x3 solve; RMSD as an FP64 residual */
 };
 
 typedef struct gta_options {
