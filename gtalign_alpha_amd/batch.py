@@ -64,6 +64,8 @@ def lib():
         L.gta_test_dp.restype = C.c_int
         L.gta_test_set_dp_buckets.argtypes = [C.c_void_p, C.c_longlong, C.c_int]
         L.gta_test_set_dp_buckets.restype = C.c_int
+        L.gta_test_set_dp_grid.argtypes = [C.c_void_p, C.c_int]
+        L.gta_test_set_dp_grid.restype = C.c_int
         L.gta_test_set_split.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
         L.gta_test_set_split.restype = C.c_int
         _lib = L
@@ -154,6 +156,12 @@ class GtaBatch:
         """Thresholds of the multi-wave DP bucket (testing hook,
         include/gta/gtalign_amd_testing.h)."""
         self._check(lib().gta_test_set_dp_buckets(self._h, int(min_cells), int(min_strips)))
+
+    def set_dp_grid(self, mode):
+        """Device-wide strip pipeline for the multi-wave DP bucket: 0 when
+        those pairs number at most half the CUs, 1 always, 2 never (testing
+        hook, include/gta/gtalign_amd_testing.h)."""
+        self._check(lib().gta_test_set_dp_grid(self._h, int(mode)))
 
     def set_split(self, min_len, seeds_per_part=256, slots_per_part=32):
         """Workgroup split of the superposition searches for long pairs

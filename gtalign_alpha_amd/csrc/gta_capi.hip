@@ -83,12 +83,15 @@ struct gta_ctx {
     DevBuf s_flags, s_grand, s_best0, s_prev, s_naln, s_qpos, s_rpos, s_fndx, s_fpos;
     DevBuf s_tibest, s_tgrand, s_out, s_aln, s_btck, s_btckoff, s_rowbuf, s_cells, s_alns, s_algb;
     // DP size buckets: pairs filled by W waves (and their ring rows)
-    DevBuf s_dpsmall, s_dpbig, s_ring, s_ringoff, s_fail;
+    DevBuf s_dpsmall, s_dpbig, s_ring, s_ringoff, s_fail, s_gstrip0, s_gsync;
     DpBuckets dpb{};
     // a pair is filled by several waves when Lq*Lr >= mw_min_cells and it
     // has at least mw_min_strips 64-row strips (and Lr >= 256)
     long long mw_min_cells = 1ll << 22;
     int mw_min_strips = 16;
+    // those pairs are pipelined over the whole device instead when they number
+    // at most half the CUs (dp_grid 0), always (1) or never (2)
+    int dp_grid = 0;
     // refinement split: pairs whose shorter structure has at least
     // split_min_len residues spread their slots over several workgroups
     DevBuf s_rsmall, s_rbig, s_parts;
@@ -206,6 +209,7 @@ void gta_destroy(gta_ctx* c) {
                       &c->s_rpos, &c->s_fndx, &c->s_fpos, &c->s_tibest, &c->s_tgrand, &c->s_out,
                       &c->s_aln, &c->s_btck, &c->s_btckoff, &c->s_rowbuf, &c->s_cells, &c->s_alns,
                       &c->s_algb, &c->s_dpsmall, &c->s_dpbig, &c->s_ring, &c->s_ringoff, &c->s_fail,
+                      &c->s_gstrip0, &c->s_gsync,
                       &c->s_rsmall, &c->s_rbig, &c->s_parts, &c->s_bigmask, &c->s_lista, &c->s_listb,
                       &c->s_listc, &c->s_lcnt, &c->s_ltot, &c->s_heads};
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
@@ -349,7 +353,42 @@ static int prepare_state(gta_ctx* c, DevState& S, size_t& alnsize) {
             }
         }
         DpBuckets B{};
-        if (!big.empty()) {
+        // a few large pairs would leave most CUs idle with one workgroup each:
+        // then the whole device pipelines their strips
+        const bool usegrid = !big.empty() && (c->dp_grid == 1 || (c->dp_grid == 0 && (int)big.size() * 2 <= c->ncu));
+        if (usegrid) {
+            std::vector<int> s0;
+            int tot = 0;
+            for (int pidx : big) {
+                const int Lq = c->q.len[pidx / nr], Lr = c->r.len[pidx % nr];
+                s0.push_back(tot);
+                roff.push_back(ringtot);
+                ringtot += (unsigned long long)((Lq + 63) / 64) * (unsigned long long)Lr;
+                tot += (Lq + 63) / 64;
+            }
+            s0.push_back(tot);
+            HIPCHK(c, c->s_dpsmall.ensure(std::max<size_t>(small.size(), 1) * 4));
+            HIPCHK(c, c->s_dpbig.ensure(big.size() * 4));
+            HIPCHK(c, c->s_gstrip0.ensure(s0.size() * 4));
+            HIPCHK(c, c->s_gsync.ensure(((size_t)tot + 1) * 4));
+            HIPCHK(c, c->s_ringoff.ensure(big.size() * 8));
+            HIPCHK(c, c->s_ring.ensure(ringtot * 4));
+            if (!small.empty())
+                HIPCHK(c, hipMemcpy(c->s_dpsmall.p, small.data(), small.size() * 4, hipMemcpyHostToDevice));
+            HIPCHK(c, hipMemcpy(c->s_dpbig.p, big.data(), big.size() * 4, hipMemcpyHostToDevice));
+            HIPCHK(c, hipMemcpy(c->s_gstrip0.p, s0.data(), s0.size() * 4, hipMemcpyHostToDevice));
+            HIPCHK(c, hipMemcpy(c->s_ringoff.p, roff.data(), roff.size() * 8, hipMemcpyHostToDevice));
+            B.small = c->s_dpsmall.as<int>();
+            B.nsmall = (int)small.size();
+            B.grid = c->s_dpbig.as<int>();
+            B.ngrid = (int)big.size();
+            B.gstrip0 = c->s_gstrip0.as<int>();
+            B.gtotal = tot;
+            B.gsync = c->s_gsync.as<int>();
+            B.grows = c->s_ring.as<float>();
+            B.growoff = c->s_ringoff.as<unsigned long long>();
+            B.gworkers = std::min(tot, 2 * c->ncu);
+        } else if (!big.empty()) {
             B.waves = minstrips >= 32 ? 16 : (minstrips >= 16 ? 8 : 4);
             for (int pidx : big) {
                 roff.push_back(ringtot);
@@ -409,11 +448,12 @@ static int prepare_state(gta_ctx* c, DevState& S, size_t& alnsize) {
     // batched form: per-pair bucket marks and the live-pair lists
     if (c->opts.batched) {
         c->bigmask = nullptr;
-        if (c->dpb.nbig > 0 || c->rsp.nbig > 0) {
+        if (c->dpb.nbig > 0 || c->dpb.ngrid > 0 || c->rsp.nbig > 0) {
             std::vector<unsigned char> m(np, 0);
-            if (c->dpb.nbig > 0) {
-                std::vector<int> big(c->dpb.nbig);
-                HIPCHK(c, hipMemcpy(big.data(), c->dpb.big, big.size() * 4, hipMemcpyDeviceToHost));
+            if (c->dpb.nbig > 0 || c->dpb.ngrid > 0) {
+                const bool g = c->dpb.ngrid > 0;
+                std::vector<int> big(g ? c->dpb.ngrid : c->dpb.nbig);
+                HIPCHK(c, hipMemcpy(big.data(), g ? c->dpb.grid : c->dpb.big, big.size() * 4, hipMemcpyDeviceToHost));
                 for (int pidx : big) m[pidx] |= 1;
             }
             if (c->rsp.nbig > 0) {
@@ -427,7 +467,7 @@ static int prepare_state(gta_ctx* c, DevState& S, size_t& alnsize) {
         }
         HIPCHK(c, c->s_lista.ensure(np * 4));
         HIPCHK(c, c->s_listb.ensure(np * 4));
-        if (c->dpb.nbig > 0) HIPCHK(c, c->s_listc.ensure(np * 4));
+        if (c->dpb.nbig > 0 || c->dpb.ngrid > 0) HIPCHK(c, c->s_listc.ensure(np * 4));
         HIPCHK(c, c->s_lcnt.ensure(((np + 1023) / 1024) * 12));
         HIPCHK(c, c->s_ltot.ensure(12));
         HIPCHK(c, c->s_heads.ensure(GTA_MAX_QUEUES * 4));
@@ -507,7 +547,7 @@ static int run_stage(gta_ctx* c, double* seconds, double* cells) {
     int* lista = batched ? c->s_lista.as<int>() : nullptr;
     int* listb = batched ? c->s_listb.as<int>() : nullptr;
     // every live pair, when the one-wave DP list leaves some out
-    int* listc = batched && c->dpb.nbig > 0 ? c->s_listc.as<int>() : nullptr;
+    int* listc = batched && (c->dpb.nbig > 0 || c->dpb.ngrid > 0) ? c->s_listc.as<int>() : nullptr;
     int* ltot = batched ? c->s_ltot.as<int>() : nullptr;
     int* heads = batched ? c->s_heads.as<int>() : nullptr;
     const int pgrid_dp = c->ncu * 32, pgrid_rf = c->ncu * 4;
@@ -580,6 +620,7 @@ static int run_stage(gta_ctx* c, double* seconds, double* cells) {
                                                listc ? queue(listc, ltot + 2) : queue(lista, ltot),
                                                pgrid_dp, st));
         GTA_TIMED(KC_REFINE_FIN, refine(RM_PROD1_L, 0, 0, 0, 0.f));
+        if (prec) GTA_TIMED(KC_REFINE_FIN, launch_rmsd_fp64(Q, R, S, st));
         if (o.refinement == GTA_REF_ONE || o.refinement == GTA_REF_COARSE)
             GTA_TIMED(KC_REFINE_FIN, refine(RM_PROD2_L, 0, 0, 0, 0.f));
         if (o.referenced) GTA_TIMED(KC_MISC, launch_revert(S, st));
@@ -638,7 +679,7 @@ static int run_stage(gta_ctx* c, double* seconds, double* cells) {
     }
     int dpfail = 0;
     HIPCHK(c, hipMemcpy(&dpfail, c->s_fail.p, 4, hipMemcpyDeviceToHost));
-    if (dpfail) return fail(c, GTA_ERR_HIP, "multi-wave DP: a row wait exceeded its spin limit");
+    if (dpfail) return fail(c, GTA_ERR_HIP, "multi-wave or grid DP: a row wait exceeded its spin limit");
     unsigned long long ab[KC_COUNT];
     HIPCHK(c, hipMemcpy(ab, c->s_algb.p, sizeof(ab), hipMemcpyDeviceToHost));
     for (int k = 0; k < KC_COUNT; k++) c->kc_bytes[k] = (double)ab[k];
@@ -681,8 +722,12 @@ int gta_test_dp(gta_ctx* c, const float* tfms, int gap0, float gapopen, int uses
     HIPCHK(c, launch_init(Q, R, S, st));
     HIPCHK(c, hipMemcpyAsync(S.tfm_grand, tfms, np * 12 * 4, hipMemcpyHostToDevice, st));
     DpLaunch L{DPM_GRAND_L, gap0, gapopen, usess, usess ? 0.5f : 0.0f, dpscan};
+    HIPCHK(c, launch_fill_i32(c->dpb.failflag, 0, 1, st));
     HIPCHK(c, launch_dp(Q, R, S, L, c->dpb, WorkQ{}, 0, st));
     HIPCHK(c, hipStreamSynchronize(st));
+    int dpfail = 0;
+    HIPCHK(c, hipMemcpy(&dpfail, c->dpb.failflag, 4, hipMemcpyDeviceToHost));
+    if (dpfail) return fail(c, GTA_ERR_HIP, "multi-wave or grid DP: a row wait exceeded its spin limit");
     std::vector<float> pl(S.alnplane);
     HIPCHK(c, hipMemcpy(nmatched, S.nalnposs, np * 4, hipMemcpyDeviceToHost));
     HIPCHK(c, hipMemcpy(pl.data(), S.aln + 7 * S.alnplane, S.alnplane * 4, hipMemcpyDeviceToHost));
@@ -696,6 +741,12 @@ int gta_test_set_dp_buckets(gta_ctx* c, long long min_cells, int min_strips) {
     if (!c || min_cells < 0 || min_strips < 1) return fail(c, GTA_ERR_ARG, "invalid DP bucket thresholds");
     c->mw_min_cells = min_cells;
     c->mw_min_strips = min_strips;
+    return GTA_OK;
+}
+
+int gta_test_set_dp_grid(gta_ctx* c, int mode) {
+    if (!c || mode < 0 || mode > 2) return fail(c, GTA_ERR_ARG, "invalid DP grid mode");
+    c->dp_grid = mode;
     return GTA_OK;
 }
 

@@ -906,6 +906,84 @@ __global__ __launch_bounds__(W * GTA_WAVE) void k_dp_mw(DevChunk Q, DevChunk R, 
     if (w == 0) dp_walk(Q, R, S, prm, P, bt);
 }
 
+// Row handoff between workgroups anywhere on the device (k_dp_grid): strip s
+// of a pair publishes in global memory how many columns of its last row are
+// stored; the strip below waits (bounded spin) for the columns it loads next.
+// Agent-scope release/acquire fences order the row stores and loads.
+struct GridSync {
+    int* done;      // per strip of the pair: columns of its row stored
+    int* fail;      // set when a wait gave up; every other wait then gives up too
+    __device__ __forceinline__ bool wait(int sp, int cols) {
+        for (long long it = 0;; it++) {
+            const int cd = __hip_atomic_load(done + sp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (cd >= cols) break;
+            if (it > (1ll << 22) || __hip_atomic_load(fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                if (lane_id() == 0) __hip_atomic_store(fail, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                return false;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        return true;
+    }
+    __device__ __forceinline__ void publish(int s, int cols) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        if (lane_id() == 0) __hip_atomic_store(done + s, cols, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+};
+
+// DP fill of a few very large pairs by the whole device: one-wave workgroups
+// take (pair, strip) items in increasing order from a global counter
+// (sync[total]) and sweep them; strip s waits on strip s-1's published row
+// (rows: nstrips x Lr floats per pair at rowoff[k]; strip0[k] is pair k's
+// first item).  Items are handed out in order and a strip waits only on the
+// strip before it, which a running workgroup already holds, so the lowest
+// unfinished strip never waits and the fill always drains.  Same cells and
+// codes as k_dp; k_dp_grid_walk then walks each pair.
+template <bool USESS>
+__global__ __launch_bounds__(GTA_WAVE) void k_dp_grid(DevChunk Q, DevChunk R, DevState S, DpParams prm,
+                                                       const int* plist, int npairs, const int* strip0,
+                                                       int* sync, float* rows, const unsigned long long* rowoff,
+                                                       int* failflag) {
+    const int total = strip0[npairs];
+    int* next = sync + total;
+    for (;;) {
+        int item = 0;
+        if (lane_id() == 0) item = atomicAdd(next, 1);
+        item = __shfl(item, 0, GTA_WAVE);
+        if (item >= total) break;
+        int lo = 0, hi = npairs - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (strip0[mid] <= item) lo = mid; else hi = mid - 1;
+        }
+        const int k = lo, s = item - strip0[k];
+        DpPair P;
+        if (!dp_setup(Q, R, S, prm, plist[k], P)) continue;
+        const int nstrips = strip0[k + 1] - strip0[k];
+        float* prow = rows + rowoff[k];
+        unsigned* bt = S.btck + S.btckoff[P.p];
+        GridSync sy{sync + strip0[k], failflag};
+        const float* rin = s > 0 ? prow + (size_t)(s - 1) * P.Lr : nullptr;
+        float* rout = s + 1 < nstrips ? prow + (size_t)s * P.Lr : nullptr;
+        if (!dp_strip<USESS>(Q, R, S, prm, P, s, rin, rout, bt, sy)) break;
+        sy.publish(s, P.Lr);
+    }
+}
+
+// backtrace walk of the pairs k_dp_grid filled (one wave per pair)
+__global__ __launch_bounds__(GTA_WAVE) void k_dp_grid_walk(DevChunk Q, DevChunk R, DevState S, DpParams prm,
+                                                            const int* plist, const int* failflag) {
+    DpPair P;
+    if (!dp_setup(Q, R, S, prm, plist[blockIdx.x], P)) return;
+    if (*failflag) {
+        if (threadIdx.x == 0) S.nalnposs[P.p] = 0;
+        return;
+    }
+    dp_count(S, P);
+    dp_walk(Q, R, S, prm, P, S.btck + S.btckoff[P.p]);
+}
+
 // ============================================================================
 // refinement over the DP alignment (reverse-ordered matched pairs)
 enum { RM_DPREFINE = 0, RM_FIN1 = 1, RM_FIN2 = 2, RM_PROD1 = 3, RM_PROD2 = 4 };
@@ -951,24 +1029,6 @@ __device__ __forceinline__ void gcc_dp_frag(const Aln& a, int L, int fragpos, in
 #pragma unroll
         for (int i = 0; i < 6; i++) sq[i] = gsum(s2[i]);
     }
-}
-// mean squared residual |R q + t - r|^2 over the fragment's pairs, in FP64
-// (the mixed-precision form's RMSD: no cancellation, unlike the E0 identity,
-// whose eigenvalues lose accuracy for near-collinear or near-planar sets)
-__device__ __forceinline__ double gresid_frag(const Aln& a, int L, int fragpos, int fl, const float* t) {
-    int ql = L, rl = L, qp = fragpos, rp = fragpos;
-    UpdateLengths(ql, rl, qp, rp, fl);
-    double s = 0.0;
-    for (int k = glane(); k < fl; k += GL) {
-        const int idx = L - 1 - (rp + k);
-        const double qx = a.qx[idx], qy = a.qy[idx], qz = a.qz[idx];
-        const double dx = (double)t[0] * qx + (double)t[1] * qy + (double)t[2] * qz + (double)t[9] - (double)a.rx[idx];
-        const double dy = (double)t[3] * qx + (double)t[4] * qy + (double)t[5] * qz + (double)t[10] - (double)a.ry[idx];
-        const double dz = (double)t[6] * qx + (double)t[7] * qy + (double)t[8] * qz + (double)t[11] - (double)a.rz[idx];
-        s += dx * dx + dy * dy + dz * dz;
-    }
-    s = gsum(s);
-    return fl > 0 ? s / (double)fl : 0.0;
 }
 // covariance of the aligned pairs within thr under tprev; returns the count
 template <int P>
@@ -1170,7 +1230,8 @@ __device__ void refine_pair(const DevChunk& Q, const DevChunk& R, const DevState
             bool done = false;
             if (cit == 0) {
                 if (MODE == RM_PROD1 && si == 0 && P == 0) {
-                    // the RMSD of the whole alignment comes with the first slot's first pass
+                    // the RMSD of the whole alignment comes with the first slot's
+                    // first pass (the mixed form's comes from k_rmsd_fp64)
                     T sq[6];
                     gcc_dp_frag<P>(a, L, sfragpos, fl, cc, sq);
                     T ccr[15];
@@ -1195,13 +1256,6 @@ __device__ void refine_pair(const DevChunk& Q, const DevChunk& R, const DevState
                     last[15] = ncc;
                 }
                 kabsch<T>(cc, ncc, dinv, tfm);
-                if (P == 1 && MODE == RM_PROD1 && si == 0 && cit == 0) {
-                    // mixed precision: RMSD of the whole alignment as the FP64
-                    // residual under its own superposition (the first slot's
-                    // first pass covers the whole alignment)
-                    const double ms = gresid_frag(a, L, sfragpos, fl, tfm);
-                    if (glane() == 0) S.out[(size_t)p * 15 + 8] = (float)sqrt(ms);
-                }
                 const float sc = gscore_dp<P>(a, L, tfm, d0, d02, d82, chk, cit < 1, &thr);
                 ab += AB_TM * (unsigned long long)L + AB_SLOT;
                 if (best < sc) {
@@ -1407,6 +1461,101 @@ __global__ void k_fill_i32(int* d, int v, size_t n) {
 }
 
 // ============================================================================
+// Mixed precision: the RMSD of each pair's final alignment under its optimal
+// superposition, Σ|Rq+t-r|²/n = (Sq + Sr - 2 λmax) / n, where λmax is the
+// largest eigenvalue of Horn's 4x4 quaternion matrix of the centred
+// covariance (Horn 1987, J. Opt. Soc. Am. A 4:629).  The sums are FP64 from
+// exact products of the FP32 coordinates and λmax comes from cyclic Jacobi
+// rotations, accurate to FP64 rounding of |N| even when eigenvalues nearly
+// coincide; the E0 identity's cubic (rmsd_e0) loses half its digits there,
+// i.e. on near-collinear or near-planar alignments.  Same pairs and positions
+// as the E0 RMSD of the PROD1 refinement: the whole alignment of every pair
+// that is not low-scoring.
+
+// one Jacobi rotation annihilating a[I][J] (indices fixed at compile time so
+// the matrix stays in registers)
+template <int I, int J>
+__device__ __forceinline__ void jacobi_rot(double (&a)[4][4]) {
+    const double apq = a[I][J];
+    if (apq == 0.0) return;
+    const double th = (a[J][J] - a[I][I]) / (2.0 * apq);
+    const double t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+    const double c = 1.0 / sqrt(t * t + 1.0), sn = t * c;
+    a[I][I] -= t * apq;
+    a[J][J] += t * apq;
+    a[I][J] = a[J][I] = 0.0;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        if (r == I || r == J) continue;
+        const double arp = a[r][I], arq = a[r][J];
+        a[r][I] = a[I][r] = c * arp - sn * arq;
+        a[r][J] = a[J][r] = sn * arp + c * arq;
+    }
+}
+
+__device__ __forceinline__ double wave_sum_d(double v) {
+#pragma unroll
+    for (int m = 1; m < GTA_WAVE; m <<= 1) v += __shfl_xor(v, m, GTA_WAVE);
+    return v;
+}
+
+__global__ __launch_bounds__(GTA_WAVE) void k_rmsd_fp64(DevChunk Q, DevChunk R, DevState S) {
+    const int p = blockIdx.x;
+    if (p >= S.nq * S.nr || (S.flags[p] & CONV_LOWTMSC)) return;
+    const int L = S.nalnposs[p];
+    if (L < 1) return;
+    const int qi = pair_q(S, p), ri = pair_r(S, p);
+    const size_t pl = S.alnplane;
+    const float* g = S.aln + (size_t)qi * (size_t)R.nposs + (size_t)R.dst[ri];
+    double s[17];
+#pragma unroll
+    for (int i = 0; i < 17; i++) s[i] = 0.0;
+    for (int k = lane_id(); k < L; k += GTA_WAVE) {
+        const double x = g[k], y = g[pl + k], z = g[2 * pl + k];
+        const double u = g[3 * pl + k], v = g[4 * pl + k], w = g[5 * pl + k];
+        s[0] += x; s[1] += y; s[2] += z; s[3] += u; s[4] += v; s[5] += w;
+        s[6] += x * u; s[7] += x * v; s[8] += x * w;
+        s[9] += y * u; s[10] += y * v; s[11] += y * w;
+        s[12] += z * u; s[13] += z * v; s[14] += z * w;
+        s[15] += x * x + y * y + z * z;
+        s[16] += u * u + v * v + w * w;
+    }
+#pragma unroll
+    for (int i = 0; i < 17; i++) s[i] = wave_sum_d(s[i]);
+    const double n = (double)L;
+    // centred covariance C[a][b] = Σ (q_a - q̄_a)(r_b - r̄_b) and the sums of squares
+    double C[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) C[i][j] = s[6 + 3 * i + j] - s[i] * s[3 + j] / n;
+    const double sq = s[15] - (s[0] * s[0] + s[1] * s[1] + s[2] * s[2]) / n;
+    const double sr = s[16] - (s[3] * s[3] + s[4] * s[4] + s[5] * s[5]) / n;
+    double a[4][4];
+    a[0][0] = C[0][0] + C[1][1] + C[2][2];
+    a[1][1] = C[0][0] - C[1][1] - C[2][2];
+    a[2][2] = -C[0][0] + C[1][1] - C[2][2];
+    a[3][3] = -C[0][0] - C[1][1] + C[2][2];
+    a[0][1] = a[1][0] = C[1][2] - C[2][1];
+    a[0][2] = a[2][0] = C[2][0] - C[0][2];
+    a[0][3] = a[3][0] = C[0][1] - C[1][0];
+    a[1][2] = a[2][1] = C[0][1] + C[1][0];
+    a[1][3] = a[3][1] = C[2][0] + C[0][2];
+    a[2][3] = a[3][2] = C[1][2] + C[2][1];
+    const double scale = fabs(a[0][0]) + fabs(a[1][1]) + fabs(a[2][2]) + fabs(a[3][3]) + 1e-300;
+    for (int sweep = 0; sweep < 32; sweep++) {
+        const double off = fabs(a[0][1]) + fabs(a[0][2]) + fabs(a[0][3]) + fabs(a[1][2]) + fabs(a[1][3]) +
+                           fabs(a[2][3]);
+        if (off <= 1e-18 * scale) break;
+        jacobi_rot<0, 1>(a); jacobi_rot<0, 2>(a); jacobi_rot<0, 3>(a);
+        jacobi_rot<1, 2>(a); jacobi_rot<1, 3>(a); jacobi_rot<2, 3>(a);
+    }
+    const double lmax = fmax(fmax(a[0][0], a[1][1]), fmax(a[2][2], a[3][3]));
+    const double e = (sq + sr - 2.0 * lmax) / n;
+    if (lane_id() == 0) S.out[(size_t)p * 15 + 8] = (float)sqrt(e > 0.0 ? e : 0.0);
+}
+
+// ============================================================================
 __global__ void k_setlow(DevChunk Q, DevChunk R, DevState S, float thr) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= S.nq * S.nr) return;
@@ -1547,6 +1696,18 @@ hipError_t launch_dp(const DevChunk& Q, const DevChunk& R, const DevState& S, co
         }
 #undef GTA_MW
     }
+    if (B.ngrid > 0) {
+        // progress counters and the item counter start at zero for every fill
+        hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for((long long)B.gtotal + 1, 256)), dim3(256), 0, st, B.gsync, 0,
+                           (size_t)B.gtotal + 1);
+        if (L.usess)
+            hipLaunchKernelGGL(k_dp_grid<true>, dim3(B.gworkers), dim3(GTA_WAVE), 0, st, Q, R, S, prm, B.grid,
+                               B.ngrid, B.gstrip0, B.gsync, B.grows, B.growoff, B.failflag);
+        else
+            hipLaunchKernelGGL(k_dp_grid<false>, dim3(B.gworkers), dim3(GTA_WAVE), 0, st, Q, R, S, prm, B.grid,
+                               B.ngrid, B.gstrip0, B.gsync, B.grows, B.growoff, B.failflag);
+        hipLaunchKernelGGL(k_dp_grid_walk, dim3(B.ngrid), dim3(GTA_WAVE), 0, st, Q, R, S, prm, B.grid, B.failflag);
+    }
     return hipGetLastError();
 }
 hipError_t launch_refine_dp(int prec, int mode, const DevChunk& Q, const DevChunk& R, const DevState& S,
@@ -1600,6 +1761,11 @@ hipError_t launch_live_lists(const DevState& S, const unsigned char* bigmask, in
 }
 hipError_t launch_fill_i32(int* d, int v, size_t n, hipStream_t st) {
     if (n) hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for((long long)n, 256)), dim3(256), 0, st, d, v, n);
+    return hipGetLastError();
+}
+hipError_t launch_rmsd_fp64(const DevChunk& Q, const DevChunk& R, const DevState& S, hipStream_t st) {
+    const long long np = (long long)S.nq * S.nr;
+    if (np > 0) hipLaunchKernelGGL(k_rmsd_fp64, dim3((unsigned)np), dim3(GTA_WAVE), 0, st, Q, R, S);
     return hipGetLastError();
 }
 hipError_t launch_setlow(const DevChunk& Q, const DevChunk& R, const DevState& S, float thr, hipStream_t st) {

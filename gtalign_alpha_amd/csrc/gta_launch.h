@@ -29,7 +29,16 @@ struct DpBuckets {
     int waves;         // W: 4, 8 or 16
     float* ring;
     const unsigned long long* ringoff;
-    int* failflag;     // set by a W-wave fill whose row wait gave up
+    int* failflag;     // set by a W-wave or grid fill whose row wait gave up
+    // pairs filled by the device-wide strip pipeline (k_dp_grid)
+    const int* grid;
+    int ngrid;
+    const int* gstrip0;                     // [ngrid + 1] first strip item of each pair
+    int gtotal;                             // strips of all grid pairs
+    int* gsync;                             // [gtotal + 1] strip progress, then the item counter
+    float* grows;                           // per pair nstrips x Lr floats at growoff[k]
+    const unsigned long long* growoff;
+    int gworkers;                           // one-wave workgroups of the fill
 };
 
 // Pairs of a chunk split by refinement size: `small` pairs (nullptr: every
@@ -83,6 +92,9 @@ hipError_t launch_prod_aln(const DevChunk& Q, const DevChunk& R, const DevState&
 hipError_t launch_live_lists(const DevState& S, const unsigned char* bigmask, int* cnt, int* totals, int* lista,
                              int* listb, int* listc, hipStream_t st);
 hipError_t launch_fill_i32(int* d, int v, size_t n, hipStream_t st);
+// mixed precision: RMSD of every output pair's final alignment from FP64 sums
+// and the quaternion matrix's largest eigenvalue (after the PROD1 refinement)
+hipError_t launch_rmsd_fp64(const DevChunk& Q, const DevChunk& R, const DevState& S, hipStream_t st);
 hipError_t launch_setlow(const DevChunk& Q, const DevChunk& R, const DevState& S, float thr,
                          hipStream_t st);
 hipError_t launch_revert(const DevState& S, hipStream_t st);

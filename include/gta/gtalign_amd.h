@@ -77,7 +77,7 @@ enum gta_refinement { GTA_REF_LOG = 0, GTA_REF_COARSE, GTA_REF_ONE, GTA_REF_FULL
 enum gta_precision {
     GTA_PREC_FP32 = 0,       /* plain form: FP32 throughout, as the reference */
     GTA_PREC_MIXED = 1       /* searches as FP32; final refinement with FP64 covariance
-                                and 3x3 solve; RMSD as an FP64 residual */
+                                and 3x3 solve; RMSD from an FP64 eigen-solve */
 };
 
 typedef struct gta_options {

@@ -33,6 +33,13 @@ int gta_test_dp(gta_ctx* ctx, const float* tfms, int gap0, float gapopen, int us
  * moderate sizes. */
 int gta_test_set_dp_buckets(gta_ctx* ctx, long long min_cells, int min_strips);
 
+/* Pairs of the multi-wave bucket above are filled by the whole device
+ * instead (one-wave workgroups pipelining the pair's strips through global
+ * memory) when mode is 1, never when 2, and when they number at most half
+ * the device's compute units when 0 (the default).  Takes effect at the next
+ * gta_process / gta_test_dp. */
+int gta_test_set_dp_grid(gta_ctx* ctx, int mode);
+
 /* Split of the superposition searches over workgroups: a pair whose shorter
  * structure has at least min_len residues has its seed diagonals
  * (seeds_per_part per workgroup) and fragment slots (slots_per_part per

@@ -57,9 +57,10 @@ def assert_same(expect, got, q, r, tm_tol=1e-4, rmsd_tol=1e-3):
 
 
 def assert_close_mixed(expect, got, q, r):
-    """The mixed form solves the 3x3 superposition in FP64, so where the FP32
-    solve is inaccurate (alignments of a few residues) or seeds and slots are
-    within rounding of each other, a pair may differ from the FP32 oracle.
+    """The mixed form refines the final superposition with an FP64 covariance
+    and solve, so where the FP32 solve is inaccurate (alignments of a few
+    residues) or fragment slots are within rounding of each other, a pair may
+    differ from the FP32 oracle.
     Bar: at most 2% of pairs (at least 1) differ in alignment or by more than
     1e-4 in TM-score, and none by more than 0.05.  (Agreement with an FP64
     recomputation is tested separately.)"""
@@ -320,6 +321,7 @@ def test_dp_multiwave_bit_exact(lq, nstrips_min, waves):
         with GtaBatch(0, default_options(), max_qrys=q.nstrs, max_qry_poss=q.nposs, max_refs=r.nstrs,
                       max_ref_poss=r.nposs) as b:
             b.set_dp_buckets(0, nstrips_min)
+            b.set_dp_grid(2)
             b.TransferQueryPMDataToDevice(q)
             b.TransferCPMDataToDevice(r)
             got = b.dp_under(tf, gap0, gapopen)
@@ -330,15 +332,55 @@ def test_dp_multiwave_bit_exact(lq, nstrips_min, waves):
                 assert np.array_equal(got[i, j], want), (gap0, i, j, int(q.len[i]), int(r.len[j]))
 
 
-def test_stage_with_multiwave_dp():
+@pytest.mark.parametrize("lq,usess", [(700, False), (2100, False), (1300, True)],
+                         ids=["11strips", "33strips", "ss"])
+def test_dp_grid_bit_exact(lq, usess):
+    """Pairs filled by the device-wide strip pipeline (one-wave workgroups
+    taking strips in order, rows handed over through global memory) give the
+    one-wave fill's alignments bit for bit."""
+    rng = np.random.default_rng(lq + 7)
+    src = synth.chain(lq, lq)
+    keep = rng.random(lq) > 0.15
+    r0 = src[keep] + rng.normal(scale=1.0, size=(int(keep.sum()), 3)).astype(np.float32)
+    rs = [r0, synth.chain(lq + 1, 300), synth.chain(lq + 2, 1500)]
+    q = Chunk.from_structures([src, synth.chain(lq + 3, 200)], ["A" * lq, "A" * 200])
+    r = Chunk.from_structures(rs, [synth.residues(rng, len(x)) for x in rs])
+    tf = np.zeros((q.nstrs, r.nstrs, 12), np.float32)
+    for i in range(q.nstrs):
+        for j in range(r.nstrs):
+            tf[i, j, :9] = np.eye(3).reshape(-1)
+            tf[i, j, 9:] = rng.normal(scale=1.0, size=3)
+    for gap0, gapopen in ((True, 0.0), (False, -1.0 if usess else -0.6)):
+        got = []
+        for grid in (1, 2):
+            with GtaBatch(0, default_options(), max_qrys=q.nstrs, max_qry_poss=q.nposs, max_refs=r.nstrs,
+                          max_ref_poss=r.nposs) as b:
+                b.set_dp_buckets(0, 2)
+                b.set_dp_grid(grid)
+                b.TransferQueryPMDataToDevice(q)
+                b.TransferCPMDataToDevice(r)
+                got.append(b.dp_under(tf, gap0, gapopen, usess=usess))
+        for i in range(q.nstrs):
+            for j in range(r.nstrs):
+                assert np.array_equal(got[0][i, j], got[1][i, j]), (gap0, i, j)
+                if not usess:
+                    d02 = float(oracle.d0(int(q.len[i]), int(r.len[j]))["d0"]) ** 2
+                    _, want = oracle.dp(q.coords(i), r.coords(j), tf[i, j], d02, gapopen, gap0)
+                    assert np.array_equal(got[0][i, j], want), (gap0, i, j, int(q.len[i]), int(r.len[j]))
+
+
+@pytest.mark.parametrize("grid", [2, 1], ids=["W-waves", "grid"])
+def test_stage_with_multiwave_dp(grid):
     """The whole stage with every pair of >= 2 strips and >= 256 reference
-    residues in the multi-wave DP bucket equals the oracle."""
+    residues in the multi-wave DP bucket (one workgroup of W waves per pair,
+    or the device-wide strip pipeline) equals the oracle."""
     qc, rc, src = synth.config(1, scale=0.002)
     q = qc.subset([0, 5])
     r = rc.subset([i for i in range(rc.nstrs) if rc.len[i] >= 256][:30])
     with GtaBatch(0, default_options(), max_qrys=q.nstrs, max_qry_poss=q.nposs, max_refs=r.nstrs,
                   max_ref_poss=r.nposs) as b:
         b.set_dp_buckets(0, 2)
+        b.set_dp_grid(grid)
         b.TransferQueryPMDataToDevice(q)
         b.TransferCPMDataToDevice(r)
         g = b.ProcessBlock()
@@ -370,7 +412,8 @@ def test_stage_with_split_searches(prec):
 def test_large_pair_default_buckets():
     """An assembly of 10 domains (2,520 residues) against its noisy rigid copy
     and an unrelated assembly: with the default thresholds these pairs take
-    the 16-wave DP and the split searches.  Equal to the oracle (FP32); the
+    the device-wide DP strip pipeline (two large pairs) and the split
+    searches.  Equal to the oracle (FP32); the
     mixed form matches FP64 recomputations."""
     rng = np.random.default_rng(5)
     a = synth.assembly(rng, ndom=10, total=2520)
@@ -407,9 +450,11 @@ def test_batched_form_equals_plain(prec):
             run_gpu(q, rc.subset(list(range(20))), prec)     # other work between replays
 
 
-def test_batched_form_with_large_pair_buckets():
-    """Batched form with every pair in the multi-wave DP bucket or split
-    searches where they apply: equal to the plain form."""
+@pytest.mark.parametrize("grid", [2, 1], ids=["W-waves", "grid"])
+def test_batched_form_with_large_pair_buckets(grid):
+    """Batched form with every pair in the multi-wave DP bucket (W waves per
+    pair, or the device-wide strip pipeline) or split searches where they
+    apply: equal to the plain form."""
     qc, rc, _ = synth.config(1, scale=0.002)
     q = qc.subset([0, 5])
     r = rc.subset([i for i in range(rc.nstrs) if rc.len[i] >= 256][:24] + [rc.nstrs - 1])
@@ -418,6 +463,7 @@ def test_batched_form_with_large_pair_buckets():
         with GtaBatch(0, default_options(batched=batched), max_qrys=q.nstrs, max_qry_poss=q.nposs,
                       max_refs=r.nstrs, max_ref_poss=r.nposs) as b:
             b.set_dp_buckets(0, 2)
+            b.set_dp_grid(grid)
             b.set_split(300, 64, 8)
             b.TransferQueryPMDataToDevice(q)
             b.TransferCPMDataToDevice(r)
