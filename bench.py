@@ -31,6 +31,8 @@ from gtalign_alpha_amd.batch import GtaBatch  # noqa: E402
 from gtalign_alpha_amd.capi import default_options, PREC_FP32, PREC_MIXED  # noqa: E402
 
 HBM_PEAK = 8.0e12
+# rocprofv3 FETCH_SIZE/WRITE_SIZE summary of the 16 x 50,000 workload's kernels
+TRAFFIC_PROFILE = "profiles/refine_dp_16x50000_hbm_traffic.md"
 # kernel class -> the kernel (template instance) rocprofv3 lists for it in the plain FP32 form
 KERNEL_NAMES = {
     "findfrag": "gta::k_findfrag<0>",
@@ -39,6 +41,22 @@ KERNEL_NAMES = {
     "refine_dp": "gta::k_refine_dp<0, 0> (RM_DPREFINE)",
     "refine_final": "gta::k_refine_dp<0, 1..4> (final and production phases)",
 }
+
+
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` on the full 16 x 50,000 workload: the
+    committed rocprofv3 counter summary's figure (FETCH_SIZE and WRITE_SIZE
+    per dispatch, each scaled by what the counter reads for a copy of known
+    size); None when the summary has no line for the kernel."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), TRAFFIC_PROFILE)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) >= 3 and parts[0] == "traffic_bytes_per_launch" and " ".join(parts[1:-1]) == kernel:
+                return int(parts[-1])
+    return None
 
 
 def log(*a):
@@ -299,7 +317,7 @@ def all_vs_all(steps, warmup):
             "form": "batched, mixed precision", "pairs": npairs,
             "pairs_per_s": round(npairs * steps / wall, 1), "ms_per_step": round(1e3 * wall / steps, 2),
             "dp_gcups": round(cells / dev / 1e9, 2),
-            "roofline": {"kernel": KERNEL_NAMES.get(dom["name"], dom["name"]).replace("<0", "<1"),
+            "roofline": {"kernel": KERNEL_NAMES.get(dom["name"], dom["name"]),
                          "achieved_GBps": round(dom["alg_bytes"] / dom["seconds"] / 1e9, 1),
                          "frac": round(dom["alg_bytes"] / dom["seconds"] / HBM_PEAK, 4)},
             "aligned_pairs": int(len(dtm)),
@@ -369,9 +387,13 @@ def main():
         if dom is None or st["seconds"] > dom["seconds"]:
             dom = st
     achieved = dom["alg_bytes"] / dom["seconds"]
+    kname = KERNEL_NAMES.get(dom["name"], dom["name"])
+    traffic = profiled_traffic(kname.split(" (")[0]) if args.scale == 1.0 else None
     roofline = {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
-                "kernel": KERNEL_NAMES.get(dom["name"], dom["name"]),
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic,
+                "traffic_source": (TRAFFIC_PROFILE + ": rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 per dispatch"
+                                   if traffic else None),
+                "kernel": kname,
                 "launches": dom["launches"],
                 "launch_ms": round(1e3 * dom["seconds"] / dom["launches"], 3),
                 "alg_bytes_per_launch": dom["alg_bytes"] / dom["launches"]}
