@@ -529,11 +529,12 @@ __global__ __launch_bounds__(RTHREADS) void k_refine_gapless(DevChunk Q, DevChun
 // The Lq x Lr matrix is swept in strips of 64 query rows; lane l owns row
 // 64*s + l and at step t computes column t - l, so every step is one
 // anti-diagonal of the strip.  The up and diagonal predecessors come from lane
-// l-1 through a cross-lane shift; lane 0 reads the previous strip's last row
-// from a row buffer that lane 63 fills.  Backtrace codes (2 bits) are packed
-// with two ballots per step: 16 steps of a strip make 64 words, one per lane,
-// stored as one coalesced 256-byte write.  The walk from the bottom-right
-// cell reads runs of 64 cells along the current direction in parallel.
+// l-1 through a cross-lane shift; lane 0 takes the previous strip's last row
+// (staged with the reference columns) and lane 63's values leave through a
+// second shift into the row the next strip reads.  Every lane packs its own
+// 2-bit backtrace codes of 16 steps into one word: 64 words, one coalesced
+// 256-byte store.  The walk from the bottom-right cell reads runs of 64 cells
+// along the current direction in parallel.
 enum { DPM_IBEST = 0, DPM_GRAND = 1, DPM_CONSTRAINED = 2 };
 
 struct DpParams {
@@ -545,19 +546,25 @@ struct DpParams {
     int dpscan;
 };
 
+// Backtrace word of lane l in block (s, tb): the codes of row 64s+l at the
+// strip's steps 16tb..16tb+15, step t at bits 2(t&15); cell (i, j) is row i
+// at step t = j + (i & 63).
 __device__ __forceinline__ unsigned read_code(const unsigned* bt, int nblk, int i, int j) {
     const int s = i >> 6, l = i & 63, t = j + l;
-    const size_t base = ((size_t)(s * nblk + (t >> 4)) << 6) + ((t & 15) << 2);
-    const unsigned w0 = bt[base + (l >> 5)], w1 = bt[base + 2 + (l >> 5)];
-    const int b = l & 31;
-    return ((w0 >> b) & 1u) | (((w1 >> b) & 1u) << 1);
+    const unsigned w = bt[((size_t)(s * nblk + (t >> 4)) << 6) + l];
+    return (w >> ((t & 15) << 1)) & 3u;
 }
 
-// whole-wave shift by one lane (DPP wave_shr:1): lane l receives lane l-1's
-// value, lane 0 receives `fill`
+// whole-wave shifts by one lane (DPP wave_shr:1 and wave_shl:1): lane l
+// receives lane l-1's (l+1's) value, lane 0 (63) receives `fill`
 __device__ __forceinline__ float wave_shr1(float v, float fill) {
     return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, fill),
                                                                  __builtin_bit_cast(int, v), 0x138, 0xf, 0xf,
+                                                                 false));
+}
+__device__ __forceinline__ float wave_shl1(float v, float fill) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, fill),
+                                                                 __builtin_bit_cast(int, v), 0x130, 0xf, 0xf,
                                                                  false));
 }
 __device__ __forceinline__ float readlane_f(float v, int l) {
@@ -574,14 +581,17 @@ struct DpPair {
 // false when the pair takes no DP in this mode (its matched count is set)
 __device__ __forceinline__ bool dp_setup(const DevChunk& Q, const DevChunk& R, const DevState& S,
                                          const DpParams& prm, int p, DpPair& P) {
+    // one pair per wave: its indices and lengths are wave-uniform (scalar
+    // registers, scalar loop control in the strip sweep)
+    p = __builtin_amdgcn_readfirstlane(p);
     P.p = p;
     P.qi = pair_q(S, p);
     P.ri = pair_r(S, p);
-    P.Lq = Q.len[P.qi];
-    P.Lr = R.len[P.ri];
-    P.qd = Q.dst[P.qi];
-    P.rd = R.dst[P.ri];
-    P.flags = S.flags[p];
+    P.Lq = __builtin_amdgcn_readfirstlane(Q.len[P.qi]);
+    P.Lr = __builtin_amdgcn_readfirstlane(R.len[P.ri]);
+    P.qd = __builtin_amdgcn_readfirstlane(Q.dst[P.qi]);
+    P.rd = __builtin_amdgcn_readfirstlane(R.dst[P.ri]);
+    P.flags = __builtin_amdgcn_readfirstlane(S.flags[p]);
     if (prm.mode == DPM_IBEST) {
         if (P.flags & (CONV_LOWTMSC | CONV_SCOREDP)) {
             if (threadIdx.x == 0) S.nalnposs[p] = 0;
@@ -606,25 +616,32 @@ struct NoSync {
     __device__ __forceinline__ void publish(int, int) {}
 };
 
-// One strip of 64 query rows of the DP (k_dp's strip sweep).  Lane l owns row
-// 64s+l and at step t computes column t-l, so a step advances one
-// anti-diagonal.  The reference residues flow through the wave one lane per
-// step (DPP shifts), lane 0 taking column t from a 64-column block that the
-// whole wave loads coalesced one block ahead; the row above the strip
-// (rowin, written by the previous strip) and this strip's last row (rowout)
-// are exchanged in the same 64-column blocks.  No LDS, no divergent branches
-// in the step loop.  Returns false when a row wait gave up (Sync::wait).
-template <bool USESS, class Sync>
+// LDS staging of one wave's reference columns: a ring of DPRING entries (two
+// 64-column blocks) of {x, y, z, row-above value}, and the SS letters in a
+// float ring of the same size.  Column c lives at entry c mod DPRING.
+constexpr int DPRING = 128;
+
+// One strip of 64 query rows of the DP (the strip sweep of every DP kernel).
+// Lane l owns row 64s+l and at step t computes column t-l, so a step advances
+// one anti-diagonal.  Each lane reads its column's entry from the LDS ring
+// (prefetched a step ahead); the whole wave loads the next 64-column block
+// coalesced from HBM a block ahead and writes it into the ring when the block
+// it replaces has been read.  The row above the strip (rowin, written by the
+// previous strip) comes in the same blocks and reaches lane 0 with the
+// up-shift of the values; lane 63's values leave through a down-shift
+// conveyor into rowout, stored and published a block at a time.  No divergent
+// branches in the step loop.  GAP0 fixes the gap mode at compile time.
+// Returns false when a row wait gave up (Sync::wait).
+template <bool USESS, bool GAP0, class Sync>
 __device__ __forceinline__ bool dp_strip(const DevChunk& Q, const DevChunk& R, const DevState& S,
                                          const DpParams& prm, const DpPair& P, int s, const float* rowin,
-                                         float* rowout, unsigned* bt, Sync& sync) {
+                                         float* rowout, unsigned* bt, float4* ring, float* ssring, Sync& sync) {
     const int lane = lane_id();
     const int Lq = P.Lq, Lr = P.Lr;
     const float* Rx = R.x + P.rd;
     const float* Ry = R.y + P.rd;
     const float* Rz = R.z + P.rd;
     const char* Rs = R.ss + P.rd;
-    const bool gap0 = prm.gap0 != 0;
     const float gapopen = prm.gapopen, ssw = prm.ssw, d02 = P.d02;
     const int nsteps = Lr + 63;
     const int nblkw = (nsteps + 15) >> 4;
@@ -640,79 +657,85 @@ __device__ __forceinline__ bool dp_strip(const DevChunk& Q, const DevChunk& R, c
     const bool qok = rowok && qx < DEFCOORD_QRY_cmp;
     const bool haveup = rowin != nullptr;
     const bool putrow = rowout != nullptr;
-    // 64-column blocks: lane k holds column c0+k (current and next block)
-    float cbx, cby, cbz, cbs = 0.f, cbu = 0.f, nbx, nby, nbz, nbs = 0.f, nbu = 0.f;
+    if (haveup && !sync.wait(s - 1, min(128, Lr))) return false;
+    // block 0 at entries 0..63, columns -64..-1 (entries 64..127) zero
     {
         const int c = lane;
         const bool ok = c < Lr;
-        cbx = ok ? Rx[c] : 0.f; cby = ok ? Ry[c] : 0.f; cbz = ok ? Rz[c] : 0.f;
-        if (USESS) cbs = ok ? (float)Rs[c] : 0.f;
+        ring[lane] = make_float4(ok ? Rx[c] : 0.f, ok ? Ry[c] : 0.f, ok ? Rz[c] : 0.f,
+                                 (ok && haveup) ? rowin[c] : 0.f);
+        ring[64 + lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (USESS) {
+            ssring[lane] = ok ? (float)Rs[c] : 0.f;
+            ssring[64 + lane] = 0.f;
+        }
+    }
+    // the next block waits in registers
+    float nbx, nby, nbz, nbu, nbs = 0.f;
+    {
         const int c2 = 64 + lane;
         const bool ok2 = c2 < Lr;
         nbx = ok2 ? Rx[c2] : 0.f; nby = ok2 ? Ry[c2] : 0.f; nbz = ok2 ? Rz[c2] : 0.f;
         if (USESS) nbs = ok2 ? (float)Rs[c2] : 0.f;
-        if (haveup) {
-            if (!sync.wait(s - 1, min(128, Lr))) return false;
-            cbu = ok ? rowin[c] : 0.f;
-            nbu = ok2 ? rowin[c2] : 0.f;
-        }
+        nbu = (ok2 && haveup) ? rowin[c2] : 0.f;
     }
-    float rx = 0.f, ry = 0.f, rz = 0.f, rs = 0.f;
+    int ao = ((-lane) & (DPRING - 1)) << 4;  // byte offset of column t - lane's ring entry
+    float4 cur = *(const float4*)((const char*)ring + ao);
+    float curs = USESS ? ssring[ao >> 4] : 0.f;
     float val = 0.f, upprev = 0.f, oblk = 0.f;
     unsigned word = 0;
+    int j = -lane;
     for (int t = 0; t < nsteps; t++) {
-        const int k = t & 63;
-        if (k == 0 && t > 0) {
-            cbx = nbx; cby = nby; cbz = nbz; cbs = nbs; cbu = nbu;
-            const int c2 = t + 64 + lane;
+        const float4 rc = cur;
+        const float rss = curs;
+        if ((t & 63) == 63) {
+            // block (t+1)/64 replaces columns t-127..t-64, all read by now
+            const int e = (t + 1 + lane) & (DPRING - 1);
+            ring[e] = make_float4(nbx, nby, nbz, nbu);
+            if (USESS) ssring[e] = nbs;
+            const int c2 = t + 65 + lane;
             const bool ok2 = c2 < Lr;
             nbx = ok2 ? Rx[c2] : 0.f; nby = ok2 ? Ry[c2] : 0.f; nbz = ok2 ? Rz[c2] : 0.f;
             if (USESS) nbs = ok2 ? (float)Rs[c2] : 0.f;
-            if (haveup && t + 64 < Lr) {
-                if (!sync.wait(s - 1, min(t + 128, Lr))) return false;
-                nbu = ok2 ? rowin[c2] : 0.f;
+            if (haveup && t + 65 < Lr) {
+                if (!sync.wait(s - 1, min(t + 129, Lr))) return false;
             }
+            nbu = (ok2 && haveup) ? rowin[c2] : 0.f;
         }
-        rx = wave_shr1(rx, readlane_f(cbx, k));
-        ry = wave_shr1(ry, readlane_f(cby, k));
-        rz = wave_shr1(rz, readlane_f(cbz, k));
-        if (USESS) rs = wave_shr1(rs, readlane_f(cbs, k));
-        const float up = wave_shr1(val, haveup ? readlane_f(cbu, k) : 0.f);
+        ao = (ao + 16) & (DPRING * 16 - 1);
+        cur = *(const float4*)((const char*)ring + ao);
+        if (USESS) curs = ssring[ao >> 4];
+
+        const float up = wave_shr1(val, rc.w);
         const float diag = upprev;
         upprev = up;
-        const int j = t - lane;
         const bool valid = rowok && (unsigned)j < (unsigned)Lr;
-        float sc = 0.0f;
-        if (qok && DEFCOORD_RFN_cmp < rx) sc = pair_score(d02, dist2(qx, qy, qz, rx, ry, rz));
-        if (USESS) sc = __fadd_rn(sc, (qss == rs) ? ssw : 0.0f);
-        float v = __fadd_rn(sc, gap0 ? diag : fabsf(diag));
+        float sc = pair_score(d02, dist2(qx, qy, qz, rc.x, rc.y, rc.z));
+        sc = (qok && DEFCOORD_RFN_cmp < rc.x) ? sc : 0.0f;
+        if (USESS) sc = __fadd_rn(sc, (qss == rss) ? ssw : 0.0f);
+        float v = __fadd_rn(sc, GAP0 ? diag : fabsf(diag));
         unsigned bk = BT_DIAG;
         float w = val;
-        if (!gap0 && w < 0.0f) w = __fsub_rn(gapopen, w);
+        if (!GAP0 && w < 0.0f) w = __fsub_rn(gapopen, w);
         if (v < w) { v = w; bk = BT_LEFT; }
         w = up;
-        if (!gap0 && w < 0.0f) w = __fsub_rn(gapopen, w);
+        if (!GAP0 && w < 0.0f) w = __fsub_rn(gapopen, w);
         if (v < w) { v = w; bk = BT_UP; }
-        const float nv = (gap0 || bk != BT_DIAG) ? v : -v;
-        val = valid ? nv : (j < 0 ? 0.0f : val);
+        const float nv = (GAP0 || bk != BT_DIAG) ? v : -v;
+        // cells left of the strip's columns keep 0, cells past them the last value
+        val = valid ? nv : val;
         const unsigned code = valid ? bk : (unsigned)BT_STOP;
-        // lane 63's value of column t-63 goes to the row of the next strip
-        const int jo = t - 63;
-        if (putrow && jo >= 0) {
-            const float o = readlane_f(val, 63);
-            oblk = (lane == (jo & 63)) ? o : oblk;
-            if ((jo & 63) == 63 || jo == Lr - 1) {
-                if (lane <= (jo & 63)) rowout[(jo & ~63) + lane] = oblk;
+        j++;
+        if (putrow) {
+            // lane l now holds column t-126+l of row 64s+63
+            oblk = wave_shl1(oblk, val);
+            const int jo = t - 63;
+            if (jo >= 0 && ((jo & 63) == 63 || jo == Lr - 1)) {
+                if (lane >= 63 - (jo & 63)) rowout[jo - 63 + lane] = oblk;
                 sync.publish(s, jo + 1);
             }
         }
-        // 2-bit codes of 16 steps -> one 64-word block (lane 4k+{0,1}: bit 0, {2,3}: bit 1)
-        const unsigned long long b0 = __ballot(code & 1u);
-        const unsigned long long b1 = __ballot(code & 2u);
-        if ((lane >> 2) == (t & 15)) {
-            const unsigned long long m = (lane & 2) ? b1 : b0;
-            word = (lane & 1) ? (unsigned)(m >> 32) : (unsigned)m;
-        }
+        word |= code << ((t & 15) << 1);
         if ((t & 15) == 15 || t == nsteps - 1) {
             bt[((size_t)(s * nblkw + (t >> 4)) << 6) + lane] = word;
             word = 0;
@@ -801,8 +824,9 @@ __device__ __forceinline__ void dp_count(const DevState& S, const DpPair& P) {
 // DP fill with backtrace and the walk, one wave per pair (plist, when given,
 // maps blocks to pairs).  The strips run in order; each reads the row the
 // previous one wrote to the pair's row in HBM.
-template <bool USESS>
-__device__ void dp_pair(const DevChunk& Q, const DevChunk& R, const DevState& S, const DpParams& prm, int p) {
+template <bool USESS, bool GAP0>
+__device__ void dp_pair(const DevChunk& Q, const DevChunk& R, const DevState& S, const DpParams& prm, int p,
+                        float4* ring, float* ssring) {
     DpPair P;
     if (!dp_setup(Q, R, S, prm, p, P)) return;
     float* row = S.rowbuf + (size_t)P.qi * (size_t)R.nposs + (size_t)P.rd;
@@ -810,7 +834,8 @@ __device__ void dp_pair(const DevChunk& Q, const DevChunk& R, const DevState& S,
     const int nstrips = (P.Lq + 63) >> 6;
     NoSync ns;
     for (int s = 0; s < nstrips; s++) {
-        dp_strip<USESS>(Q, R, S, prm, P, s, s > 0 ? row : nullptr, s + 1 < nstrips ? row : nullptr, bt, ns);
+        dp_strip<USESS, GAP0>(Q, R, S, prm, P, s, s > 0 ? row : nullptr, s + 1 < nstrips ? row : nullptr, bt,
+                              ring, ssring, ns);
         // the next strip reads this strip's row back
         __threadfence_block();
     }
@@ -822,14 +847,17 @@ __device__ void dp_pair(const DevChunk& Q, const DevChunk& R, const DevState& S,
 // One wave per pair: block b takes pair plist[b] (b without a list), or, with
 // a work queue, the waves of a persistent grid take queued pairs until the
 // queue is drained.
-template <bool USESS>
+template <bool USESS, bool GAP0>
 __global__ __launch_bounds__(GTA_WAVE) void k_dp(DevChunk Q, DevChunk R, DevState S, DpParams prm,
                                                   const int* plist, WorkQ wq) {
+    __shared__ float4 ring[DPRING];
+    __shared__ float ssring[USESS ? DPRING : 1];
     if (wq.items) {
         const int n = *wq.count;
-        for (int it = wq_next_wave(wq); it < n; it = wq_next_wave(wq)) dp_pair<USESS>(Q, R, S, prm, wq.items[it]);
+        for (int it = wq_next_wave(wq); it < n; it = wq_next_wave(wq))
+            dp_pair<USESS, GAP0>(Q, R, S, prm, wq.items[it], ring, ssring);
     } else {
-        dp_pair<USESS>(Q, R, S, prm, plist ? plist[blockIdx.x] : (int)blockIdx.x);
+        dp_pair<USESS, GAP0>(Q, R, S, prm, plist ? plist[blockIdx.x] : (int)blockIdx.x, ring, ssring);
     }
 }
 
@@ -874,11 +902,13 @@ struct LdsSync {
 // DP of one large pair per workgroup of W waves (pairs from plist; ring rows
 // of W x Lr floats per pair at ringoff[item]).  Same cells, codes and walk as
 // k_dp; only the strips run concurrently, pipelined down the query.
-template <bool USESS, int W>
+template <bool USESS, bool GAP0, int W>
 __global__ __launch_bounds__(W * GTA_WAVE) void k_dp_mw(DevChunk Q, DevChunk R, DevState S, DpParams prm,
                                                          const int* plist, float* ring,
                                                          const unsigned long long* ringoff, int* failflag) {
     __shared__ int strip[W], done[W], fail;
+    __shared__ float4 cring[W][DPRING];
+    __shared__ float ssring[USESS ? W : 1][USESS ? DPRING : 1];
     DpPair P;
     if (!dp_setup(Q, R, S, prm, plist[blockIdx.x], P)) return;
     const int w = threadIdx.x / GTA_WAVE;
@@ -893,7 +923,7 @@ __global__ __launch_bounds__(W * GTA_WAVE) void k_dp_mw(DevChunk Q, DevChunk R, 
         sy.begin(s);
         const float* rin = s > 0 ? rows + (size_t)((s - 1) % W) * P.Lr : nullptr;
         float* rout = s + 1 < nstrips ? rows + (size_t)(s % W) * P.Lr : nullptr;
-        if (!dp_strip<USESS>(Q, R, S, prm, P, s, rin, rout, bt, sy)) break;
+        if (!dp_strip<USESS, GAP0>(Q, R, S, prm, P, s, rin, rout, bt, cring[w], ssring[USESS ? w : 0], sy)) break;
         // the strip's row is complete
         sy.publish(s, P.Lr);
     }
@@ -940,11 +970,13 @@ struct GridSync {
 // strip before it, which a running workgroup already holds, so the lowest
 // unfinished strip never waits and the fill always drains.  Same cells and
 // codes as k_dp; k_dp_grid_walk then walks each pair.
-template <bool USESS>
+template <bool USESS, bool GAP0>
 __global__ __launch_bounds__(GTA_WAVE) void k_dp_grid(DevChunk Q, DevChunk R, DevState S, DpParams prm,
                                                        const int* plist, int npairs, const int* strip0,
                                                        int* sync, float* rows, const unsigned long long* rowoff,
                                                        int* failflag) {
+    __shared__ float4 cring[DPRING];
+    __shared__ float ssring[USESS ? DPRING : 1];
     const int total = strip0[npairs];
     int* next = sync + total;
     for (;;) {
@@ -966,7 +998,7 @@ __global__ __launch_bounds__(GTA_WAVE) void k_dp_grid(DevChunk Q, DevChunk R, De
         GridSync sy{sync + strip0[k], failflag};
         const float* rin = s > 0 ? prow + (size_t)(s - 1) * P.Lr : nullptr;
         float* rout = s + 1 < nstrips ? prow + (size_t)s * P.Lr : nullptr;
-        if (!dp_strip<USESS>(Q, R, S, prm, P, s, rin, rout, bt, sy)) break;
+        if (!dp_strip<USESS, GAP0>(Q, R, S, prm, P, s, rin, rout, bt, cring, ssring, sy)) break;
         sy.publish(s, P.Lr);
     }
 }
@@ -1678,36 +1710,48 @@ hipError_t upload_length_tables() {
 hipError_t launch_dp(const DevChunk& Q, const DevChunk& R, const DevState& S, const DpLaunch& L,
                      const DpBuckets& B, const WorkQ& wq, int pgrid, hipStream_t st) {
     DpParams prm{L.mode, L.gap0, L.gapopen, L.usess, L.ssw, L.dpscan};
+    // kernels specialised on (secondary-structure bonus, gap mode)
+#define GTA_DPV(KCALL)                                   \
+    do {                                                 \
+        if (L.usess) {                                   \
+            if (L.gap0) KCALL(true, true); else KCALL(true, false);   \
+        } else {                                         \
+            if (L.gap0) KCALL(false, true); else KCALL(false, false); \
+        }                                                \
+    } while (0)
     const int nsmall = wq.items ? pgrid : (B.small ? B.nsmall : S.nq * S.nr);
     if (nsmall > 0) {
-        if (L.usess)
-            hipLaunchKernelGGL(k_dp<true>, dim3(nsmall), dim3(GTA_WAVE), 0, st, Q, R, S, prm, B.small, wq);
-        else
-            hipLaunchKernelGGL(k_dp<false>, dim3(nsmall), dim3(GTA_WAVE), 0, st, Q, R, S, prm, B.small, wq);
+#define GTA_K1(US, G0) \
+    hipLaunchKernelGGL((k_dp<US, G0>), dim3(nsmall), dim3(GTA_WAVE), 0, st, Q, R, S, prm, B.small, wq)
+        GTA_DPV(GTA_K1);
+#undef GTA_K1
     }
     if (B.nbig > 0) {
-#define GTA_MW(US, WW)                                                                                 \
-    hipLaunchKernelGGL((k_dp_mw<US, WW>), dim3(B.nbig), dim3(WW * GTA_WAVE), 0, st, Q, R, S, prm, B.big, \
+#define GTA_MW(US, G0, WW)                                                                                 \
+    hipLaunchKernelGGL((k_dp_mw<US, G0, WW>), dim3(B.nbig), dim3(WW * GTA_WAVE), 0, st, Q, R, S, prm, B.big, \
                        B.ring, B.ringoff, B.failflag)
-        if (L.usess) {
-            if (B.waves >= 16) GTA_MW(true, 16); else if (B.waves >= 8) GTA_MW(true, 8); else GTA_MW(true, 4);
-        } else {
-            if (B.waves >= 16) GTA_MW(false, 16); else if (B.waves >= 8) GTA_MW(false, 8); else GTA_MW(false, 4);
-        }
+#define GTA_KMW(US, G0)                                                         \
+    do {                                                                         \
+        if (B.waves >= 16) GTA_MW(US, G0, 16);                                   \
+        else if (B.waves >= 8) GTA_MW(US, G0, 8);                                \
+        else GTA_MW(US, G0, 4);                                                  \
+    } while (0)
+        GTA_DPV(GTA_KMW);
+#undef GTA_KMW
 #undef GTA_MW
     }
     if (B.ngrid > 0) {
         // progress counters and the item counter start at zero for every fill
         hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for((long long)B.gtotal + 1, 256)), dim3(256), 0, st, B.gsync, 0,
                            (size_t)B.gtotal + 1);
-        if (L.usess)
-            hipLaunchKernelGGL(k_dp_grid<true>, dim3(B.gworkers), dim3(GTA_WAVE), 0, st, Q, R, S, prm, B.grid,
-                               B.ngrid, B.gstrip0, B.gsync, B.grows, B.growoff, B.failflag);
-        else
-            hipLaunchKernelGGL(k_dp_grid<false>, dim3(B.gworkers), dim3(GTA_WAVE), 0, st, Q, R, S, prm, B.grid,
-                               B.ngrid, B.gstrip0, B.gsync, B.grows, B.growoff, B.failflag);
+#define GTA_KG(US, G0)                                                                                    \
+    hipLaunchKernelGGL((k_dp_grid<US, G0>), dim3(B.gworkers), dim3(GTA_WAVE), 0, st, Q, R, S, prm, B.grid, \
+                       B.ngrid, B.gstrip0, B.gsync, B.grows, B.growoff, B.failflag)
+        GTA_DPV(GTA_KG);
+#undef GTA_KG
         hipLaunchKernelGGL(k_dp_grid_walk, dim3(B.ngrid), dim3(GTA_WAVE), 0, st, Q, R, S, prm, B.grid, B.failflag);
     }
+#undef GTA_DPV
     return hipGetLastError();
 }
 hipError_t launch_refine_dp(int prec, int mode, const DevChunk& Q, const DevChunk& R, const DevState& S,
