@@ -63,12 +63,21 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def time_form(qc, rc, prec, steps, warmup, profile, batched=0):
+def time_form(qc, rc, prec, steps, warmup, profile, batched=0, tune=None):
+    """Times `steps` calls after `warmup` untimed ones.  With tune (a dict),
+    DP launch-form autotuning is on with a fresh cache file: the tuning runs
+    inside the first warmup call, and tune receives the seconds it took and
+    the pairs per chosen form."""
     opts = default_options(precision=prec, batched=batched)
     with GtaBatch(0, opts, max_qrys=qc.nstrs, max_qry_poss=qc.nposs, max_refs=rc.nstrs,
                   max_ref_poss=rc.nposs) as b:
         b.TransferQueryPMDataToDevice(qc)
         b.TransferCPMDataToDevice(rc)
+        if tune is not None:
+            import tempfile
+            cache = os.path.join(tempfile.mkdtemp(), "dp_forms.txt")
+            b.set_autotune(True, cache)
+            warmup = max(warmup, 1)
         for i in range(warmup):
             s, _ = b.process_resident()
             log("  warmup %d: %.3f s" % (i, s))
@@ -80,6 +89,10 @@ def time_form(qc, rc, prec, steps, warmup, profile, batched=0):
             cells += c
             log("  step %d: %.3f s" % (i, s))
         wall = time.perf_counter() - t0
+        if tune is not None:
+            forms, secs = b.dp_forms()
+            tune.update({"tuning_seconds": round(secs, 2), "pairs_per_form": forms,
+                         "cache_lines": open(cache).read().splitlines() if os.path.exists(cache) else []})
         stats = None
         if profile:
             b.set_profiling(True)
@@ -271,14 +284,20 @@ def batched_vs_plain(steps, warmup):
         (qc.nstrs, rc.nstrs, qc.nposs, rc.nposs, time.perf_counter() - t0))
     out = {"workload": "1 query of 400 x %d references, lognormal(220,0.7) in [20,4000], 10%% homologs"
                        % rc.nstrs, "pairs": npairs}
-    for name, prec, batched in (("plain_fp32", PREC_FP32, 0), ("batched_fp32", PREC_FP32, 1),
-                                ("batched_mixed", PREC_MIXED, 1)):
+    for name, prec, batched, tuned in (("plain_fp32", PREC_FP32, 0, False), ("batched_fp32", PREC_FP32, 1, False),
+                                       ("batched_mixed", PREC_MIXED, 1, False),
+                                       ("batched_mixed_autotuned", PREC_MIXED, 1, True)):
         log(" " + name)
-        wall, dev, cells, _ = time_form(qc, rc, prec, steps, warmup, False, batched)
+        tune = {} if tuned else None
+        wall, dev, cells, _ = time_form(qc, rc, prec, steps, warmup, False, batched, tune)
         out[name] = {"pairs_per_s": round(npairs * steps / wall, 1), "ms_per_step": round(1e3 * wall / steps, 2),
                      "dp_gcups": round(cells / dev / 1e9, 2)}
-    out["batched_mixed_vs_plain"] = round(out["batched_mixed"]["pairs_per_s"] / out["plain_fp32"]["pairs_per_s"], 3)
-    out["batched_fp32_vs_plain"] = round(out["batched_fp32"]["pairs_per_s"] / out["plain_fp32"]["pairs_per_s"], 3)
+        if tuned:
+            out[name]["dp_autotune"] = tune
+    plain = out["plain_fp32"]["pairs_per_s"]
+    out["batched_mixed_vs_plain"] = round(out["batched_mixed"]["pairs_per_s"] / plain, 3)
+    out["batched_mixed_autotuned_vs_plain"] = round(out["batched_mixed_autotuned"]["pairs_per_s"] / plain, 3)
+    out["batched_fp32_vs_plain"] = round(out["batched_fp32"]["pairs_per_s"] / plain, 3)
     return out
 
 

@@ -64,6 +64,13 @@ def lib():
         L.gta_test_dp.restype = C.c_int
         L.gta_test_set_dp_buckets.argtypes = [C.c_void_p, C.c_longlong, C.c_int]
         L.gta_test_set_dp_buckets.restype = C.c_int
+        L.gta_test_dp_score.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_float,
+                                         C.c_void_p]
+        L.gta_test_dp_score.restype = C.c_int
+        L.gta_set_autotune.argtypes = [C.c_void_p, C.c_int, C.c_char_p]
+        L.gta_set_autotune.restype = C.c_int
+        L.gta_test_dp_forms.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double)]
+        L.gta_test_dp_forms.restype = C.c_int
         L.gta_test_set_dp_grid.argtypes = [C.c_void_p, C.c_int]
         L.gta_test_set_dp_grid.restype = C.c_int
         L.gta_test_set_split.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
@@ -156,6 +163,34 @@ class GtaBatch:
         """Thresholds of the multi-wave DP bucket (testing hook,
         include/gta/gtalign_amd_testing.h)."""
         self._check(lib().gta_test_set_dp_buckets(self._h, int(min_cells), int(min_strips)))
+
+    def dp_score(self, tfms, slotflags=None, gap0=True, gapopen=0.0):
+        """Score-only DP fills (testing hook, include/gta/gtalign_amd_testing.h):
+        tfms[k, qi, ri, 12] -> scores[k, qi, ri], the fill's bottom-right cell."""
+        nq, nr = self._q.nstrs, self._r.nstrs
+        t = np.ascontiguousarray(tfms, np.float32).reshape(-1, nq, nr, GTA_TFM_NFIELDS)
+        k = t.shape[0]
+        f = None if slotflags is None else np.ascontiguousarray(slotflags, np.int32).reshape(k, nq, nr)
+        out = np.zeros((k, nq, nr), np.float32)
+        self._check(lib().gta_test_dp_score(self._h, t.ctypes.data, None if f is None else f.ctypes.data, k,
+                                            int(gap0), gapopen, out.ctypes.data))
+        return out
+
+    def set_autotune(self, on=True, cache_path=None):
+        """DP launch-form autotuning per length bucket (gta_set_autotune):
+        buckets without a cached choice are timed at the next ProcessBlock;
+        choices are read from and appended to cache_path when given."""
+        self._check(lib().gta_set_autotune(self._h, int(bool(on)),
+                                           None if cache_path is None else str(cache_path).encode()))
+
+    def dp_forms(self):
+        """Pairs per DP launch form (one wave, 4, 8, 16 waves, device-wide)
+        as of the last call, and the seconds spent autotuning (testing hook)."""
+        counts = (C.c_int * 5)()
+        secs = C.c_double(0.0)
+        self._check(lib().gta_test_dp_forms(self._h, counts, C.byref(secs)))
+        return {"one_wave": counts[0], "w4": counts[1], "w8": counts[2], "w16": counts[3],
+                "grid": counts[4]}, secs.value
 
     def set_dp_grid(self, mode):
         """Device-wide strip pipeline for the multi-wave DP bucket: 0 when

@@ -11,9 +11,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -92,6 +94,14 @@ struct gta_ctx {
     // those pairs are pipelined over the whole device instead when they number
     // at most half the CUs (dp_grid 0), always (1) or never (2)
     int dp_grid = 0;
+    // per pair: DP launch form (DPF_*) of the resident chunks
+    std::vector<unsigned char> dpform;
+    // DP launch-form autotuning: on/off, cache file, device tag and the
+    // chosen form per length bucket ("strip class, Lr class, count class")
+    int autotune = 0;
+    std::string tune_path, tune_tag;
+    std::map<std::string, int> tune;
+    double tune_seconds = 0.0;
     // refinement split: pairs whose shorter structure has at least
     // split_min_len residues spread their slots over several workgroups
     DevBuf s_rsmall, s_rbig, s_parts;
@@ -100,6 +110,7 @@ struct gta_ctx {
     int split_seeds = 256, split_slots = 32;
     // batched form: live-pair work lists, queue counters, the captured graph
     DevBuf s_bigmask, s_lista, s_listb, s_listc, s_lcnt, s_ltot, s_heads;
+    DevBuf s_sctfm, s_scflag, s_scout;      // score-only DP test hook
     const unsigned char* bigmask = nullptr;
     int ncu = 256;
     hipGraphExec_t gexec = nullptr;
@@ -211,7 +222,8 @@ void gta_destroy(gta_ctx* c) {
                       &c->s_algb, &c->s_dpsmall, &c->s_dpbig, &c->s_ring, &c->s_ringoff, &c->s_fail,
                       &c->s_gstrip0, &c->s_gsync,
                       &c->s_rsmall, &c->s_rbig, &c->s_parts, &c->s_bigmask, &c->s_lista, &c->s_listb,
-                      &c->s_listc, &c->s_lcnt, &c->s_ltot, &c->s_heads};
+                      &c->s_listc, &c->s_lcnt, &c->s_ltot, &c->s_heads, &c->s_sctfm, &c->s_scflag,
+                      &c->s_scout};
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     for (DevBuf* b : bufs) b->release();
     for (hipEvent_t e : c->evpool) (void)hipEventDestroy(e);
@@ -297,6 +309,212 @@ static Geom make_geom(const gta_ctx* c) {
     return g;
 }
 
+// ---- DP launch forms ----------------------------------------------------------
+// Every pair's fill runs in one of five forms: one wave (k_dp), one workgroup
+// of 4, 8 or 16 waves (k_dp_mw) or the device-wide strip pipeline
+// (k_dp_grid).  By default large pairs (Lq*Lr >= mw_min_cells, >= mw_min_strips
+// strips, Lr >= 256) take the grid form when they number at most half the CUs
+// and the W-wave form otherwise.  With autotuning on, pairs of >= 4 strips and
+// >= 2^18 cells are grouped by length bucket (log2 of strips, log2 of Lr,
+// log2 of the bucket's pair count) and each bucket's form is the fastest of
+// the candidates in a timed fill of the whole chunk, kept per context and in
+// the cache file.
+enum { DPF_ONE = 0, DPF_W4 = 1, DPF_W8 = 2, DPF_W16 = 3, DPF_GRID = 4, DPF_N = 5 };
+
+static int ilog2ll(long long v) {
+    int r = 0;
+    while (v > 1) { v >>= 1; r++; }
+    return r;
+}
+
+// per-pair forms -> device lists [one | W4 | W8 | W16 | grid], ring offsets and
+// strip counters
+static int upload_dp_buckets(gta_ctx* c, const std::vector<unsigned char>& form, DpBuckets& B) {
+    const int nr = c->r.nstrs;
+    const size_t np = form.size();
+    std::vector<int> lists[DPF_N];
+    for (size_t p = 0; p < np; p++) lists[form[p]].push_back((int)p);
+    B = DpBuckets{};
+    HIPCHK(c, c->s_fail.ensure(4));
+    B.failflag = c->s_fail.as<int>();
+    if (lists[DPF_ONE].size() == np) return GTA_OK;     // small == nullptr: every pair one wave
+    std::vector<int> all;
+    std::vector<unsigned long long> offs;
+    size_t start[DPF_N];
+    unsigned long long ringtot = 0;
+    for (int f = 0; f < DPF_N; f++) {
+        start[f] = all.size();
+        for (int pidx : lists[f]) {
+            all.push_back(pidx);
+            const int Lq = c->q.len[pidx / nr], Lr = c->r.len[pidx % nr];
+            offs.push_back(ringtot);
+            if (f == DPF_ONE) continue;
+            const unsigned long long rows =
+                f == DPF_GRID ? (unsigned long long)((Lq + 63) / 64) : (4ull << (f - DPF_W4));
+            ringtot += rows * (unsigned long long)Lr;
+        }
+    }
+    std::vector<int> s0;
+    int tot = 0;
+    for (int pidx : lists[DPF_GRID]) {
+        s0.push_back(tot);
+        tot += (c->q.len[pidx / nr] + 63) / 64;
+    }
+    s0.push_back(tot);
+    HIPCHK(c, c->s_dpbig.ensure(all.size() * 4));
+    HIPCHK(c, c->s_ringoff.ensure(offs.size() * 8));
+    HIPCHK(c, c->s_ring.ensure(std::max<unsigned long long>(ringtot, 1) * 4));
+    HIPCHK(c, c->s_gstrip0.ensure(s0.size() * 4));
+    HIPCHK(c, c->s_gsync.ensure(((size_t)tot + 1) * 4));
+    HIPCHK(c, hipMemcpy(c->s_dpbig.p, all.data(), all.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->s_ringoff.p, offs.data(), offs.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->s_gstrip0.p, s0.data(), s0.size() * 4, hipMemcpyHostToDevice));
+    const int* L = c->s_dpbig.as<int>();
+    const unsigned long long* O = c->s_ringoff.as<unsigned long long>();
+    B.small = L + start[DPF_ONE];
+    B.nsmall = (int)lists[DPF_ONE].size();
+    for (int g = 0; g < 3; g++) {
+        B.mw[g] = L + start[DPF_W4 + g];
+        B.nmw[g] = (int)lists[DPF_W4 + g].size();
+        B.mwoff[g] = O + start[DPF_W4 + g];
+    }
+    B.ring = c->s_ring.as<float>();
+    B.grid = L + start[DPF_GRID];
+    B.ngrid = (int)lists[DPF_GRID].size();
+    B.gstrip0 = c->s_gstrip0.as<int>();
+    B.gtotal = tot;
+    B.gsync = c->s_gsync.as<int>();
+    B.growoff = O + start[DPF_GRID];
+    B.gworkers = std::max(1, std::min(tot, 2 * c->ncu));
+    return GTA_OK;
+}
+
+static void default_dp_forms(const gta_ctx* c, std::vector<unsigned char>& form) {
+    const int nq = c->q.nstrs, nr = c->r.nstrs;
+    std::vector<int> big;
+    int minstrips = 1 << 30;
+    for (int qi = 0; qi < nq; qi++) {
+        const int Lq = c->q.len[qi], ns = (Lq + 63) / 64;
+        for (int ri = 0; ri < nr; ri++) {
+            const int Lr = c->r.len[ri];
+            if ((long long)Lq * Lr >= c->mw_min_cells && ns >= c->mw_min_strips && Lr >= 256) {
+                big.push_back(qi * nr + ri);
+                minstrips = std::min(minstrips, ns);
+            }
+        }
+    }
+    if (big.empty()) return;
+    const bool grid = c->dp_grid == 1 || (c->dp_grid == 0 && (int)big.size() * 2 <= c->ncu);
+    const int f = grid ? DPF_GRID : (minstrips >= 32 ? DPF_W16 : (minstrips >= 16 ? DPF_W8 : DPF_W4));
+    for (int pidx : big) form[pidx] = (unsigned char)f;
+}
+
+// time of one DP fill of the whole chunk under the given forms (ms)
+static int time_dp_fill(gta_ctx* c, const DevState& S, const std::vector<unsigned char>& form, float& ms) {
+    DpBuckets B;
+    int rc = upload_dp_buckets(c, form, B);
+    if (rc != GTA_OK) return rc;
+    const DevChunk Q = c->q.view(), R = c->r.view();
+    const DpLaunch L{DPM_GRAND_L, 1, 0.0f, 0, 0.0f, 0};
+    hipStream_t st = c->stream;
+    HIPCHK(c, launch_fill_i32(B.failflag, 0, 1, st));
+    HIPCHK(c, launch_dp(Q, R, S, L, B, WorkQ{}, 0, st));
+    HIPCHK(c, hipEventRecord(c->ev0, st));
+    for (int k = 0; k < 2; k++) HIPCHK(c, launch_dp(Q, R, S, L, B, WorkQ{}, 0, st));
+    HIPCHK(c, hipEventRecord(c->ev1, st));
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    ms *= 0.5f;
+    int dpfail = 0;
+    HIPCHK(c, hipMemcpy(&dpfail, B.failflag, 4, hipMemcpyDeviceToHost));
+    if (dpfail) return fail(c, GTA_ERR_HIP, "DP autotuning: a row wait exceeded its spin limit");
+    return GTA_OK;
+}
+
+static int tune_dp_forms(gta_ctx* c, const DevState& S, std::vector<unsigned char>& form) {
+    const int nq = c->q.nstrs, nr = c->r.nstrs;
+    struct Bucket {
+        std::vector<int> pairs;
+        long long cells = 0, maxcells = 0;
+        int minstrips = 1 << 30;
+    };
+    std::map<std::pair<int, int>, Bucket> buckets;
+    for (int qi = 0; qi < nq; qi++) {
+        const int Lq = c->q.len[qi], ns = (Lq + 63) / 64;
+        for (int ri = 0; ri < nr; ri++) {
+            const int Lr = c->r.len[ri];
+            const long long cells = (long long)Lq * Lr;
+            if (ns < 4 || cells < (1ll << 18)) continue;
+            Bucket& b = buckets[{ilog2ll(ns), ilog2ll(Lr)}];
+            b.pairs.push_back(qi * nr + ri);
+            b.cells += cells;
+            b.maxcells = std::max(b.maxcells, cells);
+            b.minstrips = std::min(b.minstrips, ns);
+        }
+    }
+    std::vector<std::pair<std::string, Bucket*>> order;
+    for (auto& kv : buckets) {
+        const std::string key = std::to_string(kv.first.first) + " " + std::to_string(kv.first.second) + " " +
+                                std::to_string(ilog2ll((long long)kv.second.pairs.size()));
+        order.push_back({key, &kv.second});
+    }
+    std::sort(order.begin(), order.end(), [](const auto& a, const auto& b) { return a.second->cells > b.second->cells; });
+    bool prepared = false;
+    for (auto& kb : order) {
+        Bucket& b = *kb.second;
+        auto it = c->tune.find(kb.first);
+        if (it != c->tune.end()) {
+            for (int pidx : b.pairs) form[pidx] = (unsigned char)it->second;
+            continue;
+        }
+        const auto t0 = std::chrono::steady_clock::now();
+        if (!prepared) {
+            // the fill's cost does not depend on the transform: identity for every pair
+            const DevChunk Q = c->q.view(), R = c->r.view();
+            HIPCHK(c, launch_init(Q, R, S, c->stream));
+            std::vector<float> id((size_t)nq * nr * 12, 0.f);
+            for (size_t p = 0; p < (size_t)nq * nr; p++) id[p * 12] = id[p * 12 + 4] = id[p * 12 + 8] = 1.f;
+            HIPCHK(c, hipMemcpyAsync(S.tfm_grand, id.data(), id.size() * 4, hipMemcpyHostToDevice, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            prepared = true;
+        }
+        float tms[DPF_N];
+        int best = -1;
+        for (int f = 0; f < DPF_N; f++) {
+            tms[f] = -1.f;
+            if (f == DPF_ONE && b.maxcells > (1ll << 27)) continue;        // one wave would take seconds
+            if (f >= DPF_W4 && f <= DPF_W16 && (4 << (f - DPF_W4)) > b.minstrips) continue;
+            for (int pidx : b.pairs) form[pidx] = (unsigned char)f;
+            int rc = time_dp_fill(c, S, form, tms[f]);
+            if (rc != GTA_OK) return rc;
+            if (best < 0 || tms[f] < tms[best]) best = f;
+        }
+        for (int pidx : b.pairs) form[pidx] = (unsigned char)best;
+        c->tune[kb.first] = best;
+        if (!c->tune_path.empty()) {
+            if (FILE* fp = fopen(c->tune_path.c_str(), "a")) {
+                fprintf(fp, "%s %s %d", c->tune_tag.c_str(), kb.first.c_str(), best);
+                for (int f = 0; f < DPF_N; f++) fprintf(fp, " %.3f", tms[f]);
+                fprintf(fp, "\n");
+                fclose(fp);
+            }
+        }
+        c->tune_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return GTA_OK;
+}
+
+static int build_dp_buckets(gta_ctx* c, const DevState& S) {
+    const size_t np = (size_t)c->q.nstrs * c->r.nstrs;
+    c->dpform.assign(np, DPF_ONE);
+    default_dp_forms(c, c->dpform);
+    if (c->autotune) {
+        int rc = tune_dp_forms(c, S, c->dpform);
+        if (rc != GTA_OK) return rc;
+    }
+    return upload_dp_buckets(c, c->dpform, c->dpb);
+}
+
 // device buffers of the pair state for the resident chunks
 static int prepare_state(gta_ctx* c, DevState& S, size_t& alnsize) {
     const int nq = c->q.nstrs, nr = c->r.nstrs;
@@ -333,86 +551,25 @@ static int prepare_state(gta_ctx* c, DevState& S, size_t& alnsize) {
     HIPCHK(c, c->s_btckoff.ensure(np * 8));
     HIPCHK(c, hipMemcpyAsync(c->s_btckoff.p, off.data(), np * 8, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, c->s_rowbuf.ensure(alnplane * 4));
-    // DP size buckets
+    alnsize = gta_alignment_section_size(nq, c->q.nposs, nr, c->r.nposs);
+    HIPCHK(c, c->s_alns.ensure((alnsize + 3) & ~(size_t)3));
+
+    S.nq = nq; S.nr = nr;
+    S.flags = c->s_flags.as<int>(); S.grand = c->s_grand.as<float>(); S.best0 = c->s_best0.as<float>();
+    S.prevbest0 = c->s_prev.as<float>(); S.nalnposs = c->s_naln.as<int>();
+    S.qrypos = c->s_qpos.as<int>(); S.rfnpos = c->s_rpos.as<int>();
+    S.sfragndx = c->s_fndx.as<int>(); S.sfragpos = c->s_fpos.as<int>();
+    S.tfm_ibest = c->s_tibest.as<float>(); S.tfm_grand = c->s_tgrand.as<float>();
+    S.out = c->s_out.as<float>();
+    S.aln = c->s_aln.as<float>(); S.alnplane = alnplane;
+    S.btck = c->s_btck.as<unsigned>(); S.btckoff = c->s_btckoff.as<unsigned long long>();
+    S.rowbuf = c->s_rowbuf.as<float>();
+    S.cells = c->s_cells.as<unsigned long long>();
+    S.algbytes = c->s_algb.as<unsigned long long>();
+    // DP launch forms (one wave, W waves or the whole device per pair)
     {
-        std::vector<int> small, big;
-        std::vector<unsigned long long> roff;
-        unsigned long long ringtot = 0;
-        int minstrips = 1 << 30;
-        for (int qi = 0; qi < nq; qi++) {
-            const int Lq = c->q.len[qi], ns = (Lq + 63) / 64;
-            for (int ri = 0; ri < nr; ri++) {
-                const int Lr = c->r.len[ri];
-                const int pidx = qi * nr + ri;
-                if ((long long)Lq * Lr >= c->mw_min_cells && ns >= c->mw_min_strips && Lr >= 256) {
-                    big.push_back(pidx);
-                    minstrips = std::min(minstrips, ns);
-                } else {
-                    small.push_back(pidx);
-                }
-            }
-        }
-        DpBuckets B{};
-        // a few large pairs would leave most CUs idle with one workgroup each:
-        // then the whole device pipelines their strips
-        const bool usegrid = !big.empty() && (c->dp_grid == 1 || (c->dp_grid == 0 && (int)big.size() * 2 <= c->ncu));
-        if (usegrid) {
-            std::vector<int> s0;
-            int tot = 0;
-            for (int pidx : big) {
-                const int Lq = c->q.len[pidx / nr], Lr = c->r.len[pidx % nr];
-                s0.push_back(tot);
-                roff.push_back(ringtot);
-                ringtot += (unsigned long long)((Lq + 63) / 64) * (unsigned long long)Lr;
-                tot += (Lq + 63) / 64;
-            }
-            s0.push_back(tot);
-            HIPCHK(c, c->s_dpsmall.ensure(std::max<size_t>(small.size(), 1) * 4));
-            HIPCHK(c, c->s_dpbig.ensure(big.size() * 4));
-            HIPCHK(c, c->s_gstrip0.ensure(s0.size() * 4));
-            HIPCHK(c, c->s_gsync.ensure(((size_t)tot + 1) * 4));
-            HIPCHK(c, c->s_ringoff.ensure(big.size() * 8));
-            HIPCHK(c, c->s_ring.ensure(ringtot * 4));
-            if (!small.empty())
-                HIPCHK(c, hipMemcpy(c->s_dpsmall.p, small.data(), small.size() * 4, hipMemcpyHostToDevice));
-            HIPCHK(c, hipMemcpy(c->s_dpbig.p, big.data(), big.size() * 4, hipMemcpyHostToDevice));
-            HIPCHK(c, hipMemcpy(c->s_gstrip0.p, s0.data(), s0.size() * 4, hipMemcpyHostToDevice));
-            HIPCHK(c, hipMemcpy(c->s_ringoff.p, roff.data(), roff.size() * 8, hipMemcpyHostToDevice));
-            B.small = c->s_dpsmall.as<int>();
-            B.nsmall = (int)small.size();
-            B.grid = c->s_dpbig.as<int>();
-            B.ngrid = (int)big.size();
-            B.gstrip0 = c->s_gstrip0.as<int>();
-            B.gtotal = tot;
-            B.gsync = c->s_gsync.as<int>();
-            B.grows = c->s_ring.as<float>();
-            B.growoff = c->s_ringoff.as<unsigned long long>();
-            B.gworkers = std::min(tot, 2 * c->ncu);
-        } else if (!big.empty()) {
-            B.waves = minstrips >= 32 ? 16 : (minstrips >= 16 ? 8 : 4);
-            for (int pidx : big) {
-                roff.push_back(ringtot);
-                ringtot += (unsigned long long)B.waves * (unsigned long long)c->r.len[pidx % nr];
-            }
-            HIPCHK(c, c->s_dpsmall.ensure(std::max<size_t>(small.size(), 1) * 4));
-            HIPCHK(c, c->s_dpbig.ensure(big.size() * 4));
-            HIPCHK(c, c->s_ringoff.ensure(big.size() * 8));
-            HIPCHK(c, c->s_ring.ensure(ringtot * 4));
-            if (!small.empty())
-                HIPCHK(c, hipMemcpy(c->s_dpsmall.p, small.data(), small.size() * 4, hipMemcpyHostToDevice));
-            HIPCHK(c, hipMemcpy(c->s_dpbig.p, big.data(), big.size() * 4, hipMemcpyHostToDevice));
-            HIPCHK(c, hipMemcpy(c->s_ringoff.p, roff.data(), roff.size() * 8, hipMemcpyHostToDevice));
-            B.small = c->s_dpsmall.as<int>();
-            B.nsmall = (int)small.size();
-            B.big = c->s_dpbig.as<int>();
-            B.nbig = (int)big.size();
-            B.ring = c->s_ring.as<float>();
-            B.ringoff = c->s_ringoff.as<unsigned long long>();
-        }
-        HIPCHK(c, c->s_fail.ensure(4));
-        HIPCHK(c, hipMemset(c->s_fail.p, 0, 4));
-        B.failflag = c->s_fail.as<int>();
-        c->dpb = B;
+        int rc = build_dp_buckets(c, S);
+        if (rc != GTA_OK) return rc;
     }
     // refinement split
     {
@@ -448,14 +605,10 @@ static int prepare_state(gta_ctx* c, DevState& S, size_t& alnsize) {
     // batched form: per-pair bucket marks and the live-pair lists
     if (c->opts.batched) {
         c->bigmask = nullptr;
-        if (c->dpb.nbig > 0 || c->dpb.ngrid > 0 || c->rsp.nbig > 0) {
+        if (c->dpb.any_big() || c->rsp.nbig > 0) {
             std::vector<unsigned char> m(np, 0);
-            if (c->dpb.nbig > 0 || c->dpb.ngrid > 0) {
-                const bool g = c->dpb.ngrid > 0;
-                std::vector<int> big(g ? c->dpb.ngrid : c->dpb.nbig);
-                HIPCHK(c, hipMemcpy(big.data(), g ? c->dpb.grid : c->dpb.big, big.size() * 4, hipMemcpyDeviceToHost));
-                for (int pidx : big) m[pidx] |= 1;
-            }
+            for (size_t pidx = 0; pidx < np; pidx++)
+                if (c->dpform[pidx] != DPF_ONE) m[pidx] |= 1;
             if (c->rsp.nbig > 0) {
                 std::vector<int> big(c->rsp.nbig);
                 HIPCHK(c, hipMemcpy(big.data(), c->rsp.big, big.size() * 4, hipMemcpyDeviceToHost));
@@ -467,26 +620,11 @@ static int prepare_state(gta_ctx* c, DevState& S, size_t& alnsize) {
         }
         HIPCHK(c, c->s_lista.ensure(np * 4));
         HIPCHK(c, c->s_listb.ensure(np * 4));
-        if (c->dpb.nbig > 0 || c->dpb.ngrid > 0) HIPCHK(c, c->s_listc.ensure(np * 4));
+        if (c->dpb.any_big()) HIPCHK(c, c->s_listc.ensure(np * 4));
         HIPCHK(c, c->s_lcnt.ensure(((np + 1023) / 1024) * 12));
         HIPCHK(c, c->s_ltot.ensure(12));
         HIPCHK(c, c->s_heads.ensure(GTA_MAX_QUEUES * 4));
     }
-    alnsize = gta_alignment_section_size(nq, c->q.nposs, nr, c->r.nposs);
-    HIPCHK(c, c->s_alns.ensure((alnsize + 3) & ~(size_t)3));
-
-    S.nq = nq; S.nr = nr;
-    S.flags = c->s_flags.as<int>(); S.grand = c->s_grand.as<float>(); S.best0 = c->s_best0.as<float>();
-    S.prevbest0 = c->s_prev.as<float>(); S.nalnposs = c->s_naln.as<int>();
-    S.qrypos = c->s_qpos.as<int>(); S.rfnpos = c->s_rpos.as<int>();
-    S.sfragndx = c->s_fndx.as<int>(); S.sfragpos = c->s_fpos.as<int>();
-    S.tfm_ibest = c->s_tibest.as<float>(); S.tfm_grand = c->s_tgrand.as<float>();
-    S.out = c->s_out.as<float>();
-    S.aln = c->s_aln.as<float>(); S.alnplane = alnplane;
-    S.btck = c->s_btck.as<unsigned>(); S.btckoff = c->s_btckoff.as<unsigned long long>();
-    S.rowbuf = c->s_rowbuf.as<float>();
-    S.cells = c->s_cells.as<unsigned long long>();
-    S.algbytes = c->s_algb.as<unsigned long long>();
 
     return GTA_OK;
 }
@@ -547,7 +685,7 @@ static int run_stage(gta_ctx* c, double* seconds, double* cells) {
     int* lista = batched ? c->s_lista.as<int>() : nullptr;
     int* listb = batched ? c->s_listb.as<int>() : nullptr;
     // every live pair, when the one-wave DP list leaves some out
-    int* listc = batched && (c->dpb.nbig > 0 || c->dpb.ngrid > 0) ? c->s_listc.as<int>() : nullptr;
+    int* listc = batched && c->dpb.any_big() ? c->s_listc.as<int>() : nullptr;
     int* ltot = batched ? c->s_ltot.as<int>() : nullptr;
     int* heads = batched ? c->s_heads.as<int>() : nullptr;
     const int pgrid_dp = c->ncu * 32, pgrid_rf = c->ncu * 4;
@@ -737,10 +875,76 @@ int gta_test_dp(gta_ctx* c, const float* tfms, int gap0, float gapopen, int uses
     return GTA_OK;
 }
 
+int gta_test_dp_score(gta_ctx* c, const float* tfms, const int* slotflags, int nslots, int gap0, float gapopen,
+                      float* scores) {
+    if (!c || !tfms || !scores || nslots < 1) return fail(c, GTA_ERR_ARG, "invalid argument");
+    if (!c->have_q || !c->have_r) return fail(c, GTA_ERR_STATE, "queries and references must be set first");
+    HIPCHK(c, hipSetDevice(c->device));
+    DevState S;
+    size_t alnsize = 0;
+    int rc = prepare_state(c, S, alnsize);
+    if (rc != GTA_OK) return rc;
+    const DevChunk Q = c->q.view(), R = c->r.view();
+    const size_t np = (size_t)c->q.nstrs * c->r.nstrs, n = np * (size_t)nslots;
+    hipStream_t st = c->stream;
+    HIPCHK(c, c->s_sctfm.ensure(n * 12 * 4));
+    HIPCHK(c, c->s_scout.ensure(n * 4));
+    HIPCHK(c, hipMemcpyAsync(c->s_sctfm.p, tfms, n * 12 * 4, hipMemcpyHostToDevice, st));
+    const int* dflags = nullptr;
+    if (slotflags) {
+        HIPCHK(c, c->s_scflag.ensure(n * 4));
+        HIPCHK(c, hipMemcpyAsync(c->s_scflag.p, slotflags, n * 4, hipMemcpyHostToDevice, st));
+        dflags = c->s_scflag.as<int>();
+    }
+    HIPCHK(c, launch_init(Q, R, S, st));
+    HIPCHK(c, launch_dp_score(Q, R, S, gap0, gapopen, c->s_sctfm.as<float>(), dflags, nslots,
+                              c->s_scout.as<float>(), st));
+    HIPCHK(c, hipStreamSynchronize(st));
+    HIPCHK(c, hipMemcpy(scores, c->s_scout.p, n * 4, hipMemcpyDeviceToHost));
+    return GTA_OK;
+}
+
 int gta_test_set_dp_buckets(gta_ctx* c, long long min_cells, int min_strips) {
     if (!c || min_cells < 0 || min_strips < 1) return fail(c, GTA_ERR_ARG, "invalid DP bucket thresholds");
     c->mw_min_cells = min_cells;
     c->mw_min_strips = min_strips;
+    return GTA_OK;
+}
+
+int gta_set_autotune(gta_ctx* c, int on, const char* cache_path) {
+    if (!c) return fail(nullptr, GTA_ERR_ARG, "null context");
+    c->autotune = on ? 1 : 0;
+    c->gvalid = false;
+    c->tune.clear();
+    c->tune_path = cache_path ? cache_path : "";
+    if (c->tune_tag.empty()) {
+        hipDeviceProp_t prop;
+        HIPCHK(c, hipGetDeviceProperties(&prop, c->device));
+        c->tune_tag = std::string(prop.gcnArchName) + "/" + std::to_string(c->ncu) + "cu";
+        for (char& ch : c->tune_tag)
+            if (ch == ' ') ch = '_';
+    }
+    if (!c->tune_path.empty()) {
+        if (FILE* fp = fopen(c->tune_path.c_str(), "r")) {
+            char tag[256];
+            int sc, lc, cc, f;
+            char line[1024];
+            while (fgets(line, sizeof line, fp)) {
+                if (sscanf(line, "%255s %d %d %d %d", tag, &sc, &lc, &cc, &f) == 5 && c->tune_tag == tag &&
+                    f >= 0 && f < DPF_N)
+                    c->tune[std::to_string(sc) + " " + std::to_string(lc) + " " + std::to_string(cc)] = f;
+            }
+            fclose(fp);
+        }
+    }
+    return GTA_OK;
+}
+
+int gta_test_dp_forms(const gta_ctx* c, int* counts, double* tune_seconds) {
+    if (!c || !counts) return GTA_ERR_ARG;
+    for (int f = 0; f < DPF_N; f++) counts[f] = 0;
+    for (unsigned char f : c->dpform) counts[f]++;
+    if (tune_seconds) *tune_seconds = c->tune_seconds;
     return GTA_OK;
 }
 

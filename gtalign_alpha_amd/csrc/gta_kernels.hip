@@ -630,12 +630,15 @@ constexpr int DPRING = 128;
 // previous strip) comes in the same blocks and reaches lane 0 with the
 // up-shift of the values; lane 63's values leave through a down-shift
 // conveyor into rowout, stored and published a block at a time.  No divergent
-// branches in the step loop.  GAP0 fixes the gap mode at compile time.
+// branches in the step loop.  GAP0 fixes the gap mode at compile time;
+// without BTCK no backtrace is written (score-only fills).  vend, when given,
+// receives each lane's value at the strip's last column (row 64s+l).
 // Returns false when a row wait gave up (Sync::wait).
-template <bool USESS, bool GAP0, class Sync>
+template <bool USESS, bool GAP0, bool BTCK = true, class Sync>
 __device__ __forceinline__ bool dp_strip(const DevChunk& Q, const DevChunk& R, const DevState& S,
                                          const DpParams& prm, const DpPair& P, int s, const float* rowin,
-                                         float* rowout, unsigned* bt, float4* ring, float* ssring, Sync& sync) {
+                                         float* rowout, unsigned* bt, float4* ring, float* ssring, Sync& sync,
+                                         float* vend = nullptr) {
     const int lane = lane_id();
     const int Lq = P.Lq, Lr = P.Lr;
     const float* Rx = R.x + P.rd;
@@ -735,12 +738,15 @@ __device__ __forceinline__ bool dp_strip(const DevChunk& Q, const DevChunk& R, c
                 sync.publish(s, jo + 1);
             }
         }
-        word |= code << ((t & 15) << 1);
-        if ((t & 15) == 15 || t == nsteps - 1) {
-            bt[((size_t)(s * nblkw + (t >> 4)) << 6) + lane] = word;
-            word = 0;
+        if (BTCK) {
+            word |= code << ((t & 15) << 1);
+            if ((t & 15) == 15 || t == nsteps - 1) {
+                bt[((size_t)(s * nblkw + (t >> 4)) << 6) + lane] = word;
+                word = 0;
+            }
         }
     }
+    if (vend) *vend = val;
     return true;
 }
 
@@ -858,6 +864,58 @@ __global__ __launch_bounds__(GTA_WAVE) void k_dp(DevChunk Q, DevChunk R, DevStat
             dp_pair<USESS, GAP0>(Q, R, S, prm, wq.items[it], ring, ssring);
     } else {
         dp_pair<USESS, GAP0>(Q, R, S, prm, plist ? plist[blockIdx.x] : (int)blockIdx.x, ring, ssring);
+    }
+}
+
+// Score-only DP fills for several superpositions per pair (ExecDPScore128xKernel,
+// MpDPHub.cpp:389-647, used by the frg3 superposition search to rank
+// candidate transforms): for slot k of pair p the transform is
+// tfms[(k*np + p)*12], and scores[k*np + p] receives the fill's value at the
+// bottom-right cell (Lq-1, Lr-1), which the reference reads back as the
+// slot's order-dependent score (MpStageFrg3.h:1186-1191).  Pairs flagged
+// low-scoring and slots with a nonzero slotflags[k*np + p] are not filled and
+// score 0.  One wave per pair sweeps its slots in turn through the pair's row
+// buffer; no backtrace.
+template <bool GAP0>
+__global__ __launch_bounds__(GTA_WAVE) void k_dp_score(DevChunk Q, DevChunk R, DevState S, float gapopen,
+                                                        const float* tfms, const int* slotflags, int nslots,
+                                                        float* scores) {
+    __shared__ float4 ring[DPRING];
+    __shared__ float ssring[1];
+    const int np = S.nq * S.nr;
+    const int p = blockIdx.x;
+    DpPair P;
+    P.p = p;
+    P.qi = pair_q(S, p);
+    P.ri = pair_r(S, p);
+    P.Lq = __builtin_amdgcn_readfirstlane(Q.len[P.qi]);
+    P.Lr = __builtin_amdgcn_readfirstlane(R.len[P.ri]);
+    P.qd = __builtin_amdgcn_readfirstlane(Q.dst[P.qi]);
+    P.rd = __builtin_amdgcn_readfirstlane(R.dst[P.ri]);
+    P.flags = __builtin_amdgcn_readfirstlane(S.flags[p]);
+    const float d0 = GetD0(P.Lq, P.Lr);
+    P.d02 = d0 * d0;
+    const DpParams prm{DPM_GRAND, GAP0 ? 1 : 0, gapopen, 0, 0.f, 0};
+    float* row = S.rowbuf + (size_t)P.qi * (size_t)R.nposs + (size_t)P.rd;
+    const int nstrips = (P.Lq + 63) >> 6;
+    const int lastlane = (P.Lq - 1) & 63;
+    NoSync ns;
+    for (int k = 0; k < nslots; k++) {
+        const size_t o = (size_t)k * np + p;
+        if ((P.flags & CONV_LOWTMSC) || (slotflags && slotflags[o])) {
+            if (threadIdx.x == 0) scores[o] = 0.f;
+            continue;
+        }
+#pragma unroll
+        for (int i = 0; i < 12; i++) P.tfm[i] = tfms[o * 12 + i];
+        float vend = 0.f;
+        for (int s = 0; s < nstrips; s++) {
+            dp_strip<false, GAP0, false>(Q, R, S, prm, P, s, s > 0 ? row : nullptr, s + 1 < nstrips ? row : nullptr,
+                                         nullptr, ring, ssring, ns, &vend);
+            __threadfence_block();
+        }
+        const float sc = readlane_f(vend, lastlane);
+        if (threadIdx.x == 0) scores[o] = sc;
     }
 }
 
@@ -1726,32 +1784,44 @@ hipError_t launch_dp(const DevChunk& Q, const DevChunk& R, const DevState& S, co
         GTA_DPV(GTA_K1);
 #undef GTA_K1
     }
-    if (B.nbig > 0) {
-#define GTA_MW(US, G0, WW)                                                                                 \
-    hipLaunchKernelGGL((k_dp_mw<US, G0, WW>), dim3(B.nbig), dim3(WW * GTA_WAVE), 0, st, Q, R, S, prm, B.big, \
-                       B.ring, B.ringoff, B.failflag)
-#define GTA_KMW(US, G0)                                                         \
-    do {                                                                         \
-        if (B.waves >= 16) GTA_MW(US, G0, 16);                                   \
-        else if (B.waves >= 8) GTA_MW(US, G0, 8);                                \
-        else GTA_MW(US, G0, 4);                                                  \
-    } while (0)
-        GTA_DPV(GTA_KMW);
-#undef GTA_KMW
+#define GTA_MW(US, G0, WW, G)                                                                                \
+    hipLaunchKernelGGL((k_dp_mw<US, G0, WW>), dim3(B.nmw[G]), dim3(WW * GTA_WAVE), 0, st, Q, R, S, prm, B.mw[G], \
+                       B.ring, B.mwoff[G], B.failflag)
+#define GTA_KMW4(US, G0) GTA_MW(US, G0, 4, 0)
+#define GTA_KMW8(US, G0) GTA_MW(US, G0, 8, 1)
+#define GTA_KMW16(US, G0) GTA_MW(US, G0, 16, 2)
+    if (B.nmw[0] > 0) GTA_DPV(GTA_KMW4);
+    if (B.nmw[1] > 0) GTA_DPV(GTA_KMW8);
+    if (B.nmw[2] > 0) GTA_DPV(GTA_KMW16);
+#undef GTA_KMW4
+#undef GTA_KMW8
+#undef GTA_KMW16
 #undef GTA_MW
-    }
     if (B.ngrid > 0) {
         // progress counters and the item counter start at zero for every fill
         hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for((long long)B.gtotal + 1, 256)), dim3(256), 0, st, B.gsync, 0,
                            (size_t)B.gtotal + 1);
 #define GTA_KG(US, G0)                                                                                    \
     hipLaunchKernelGGL((k_dp_grid<US, G0>), dim3(B.gworkers), dim3(GTA_WAVE), 0, st, Q, R, S, prm, B.grid, \
-                       B.ngrid, B.gstrip0, B.gsync, B.grows, B.growoff, B.failflag)
+                       B.ngrid, B.gstrip0, B.gsync, B.ring, B.growoff, B.failflag)
         GTA_DPV(GTA_KG);
 #undef GTA_KG
         hipLaunchKernelGGL(k_dp_grid_walk, dim3(B.ngrid), dim3(GTA_WAVE), 0, st, Q, R, S, prm, B.grid, B.failflag);
     }
 #undef GTA_DPV
+    return hipGetLastError();
+}
+hipError_t launch_dp_score(const DevChunk& Q, const DevChunk& R, const DevState& S, int gap0, float gapopen,
+                           const float* tfms, const int* slotflags, int nslots, float* scores, hipStream_t st) {
+    const long long np = (long long)S.nq * S.nr;
+    if (np > 0 && nslots > 0) {
+        if (gap0)
+            hipLaunchKernelGGL(k_dp_score<true>, dim3((unsigned)np), dim3(GTA_WAVE), 0, st, Q, R, S, gapopen, tfms,
+                               slotflags, nslots, scores);
+        else
+            hipLaunchKernelGGL(k_dp_score<false>, dim3((unsigned)np), dim3(GTA_WAVE), 0, st, Q, R, S, gapopen, tfms,
+                               slotflags, nslots, scores);
+    }
     return hipGetLastError();
 }
 hipError_t launch_refine_dp(int prec, int mode, const DevChunk& Q, const DevChunk& R, const DevState& S,

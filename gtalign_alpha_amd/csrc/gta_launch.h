@@ -22,13 +22,14 @@ struct DpLaunch {
 // (nullptr: every pair), the W-wave kernel takes `big` with W x Lr floats
 // of ring rows per pair at ringoff[i].
 struct DpBuckets {
-    const int* small;
+    const int* small;                       // one wave per pair (nullptr: every pair)
     int nsmall;
-    const int* big;
-    int nbig;
-    int waves;         // W: 4, 8 or 16
+    // one workgroup of W = 4 << g waves per pair, g = 0, 1, 2; ring rows of
+    // W x Lr floats per pair at mwoff[g][k] in ring
+    const int* mw[3];
+    int nmw[3];
+    const unsigned long long* mwoff[3];
     float* ring;
-    const unsigned long long* ringoff;
     int* failflag;     // set by a W-wave or grid fill whose row wait gave up
     // pairs filled by the device-wide strip pipeline (k_dp_grid)
     const int* grid;
@@ -36,9 +37,10 @@ struct DpBuckets {
     const int* gstrip0;                     // [ngrid + 1] first strip item of each pair
     int gtotal;                             // strips of all grid pairs
     int* gsync;                             // [gtotal + 1] strip progress, then the item counter
-    float* grows;                           // per pair nstrips x Lr floats at growoff[k]
-    const unsigned long long* growoff;
+    const unsigned long long* growoff;      // per pair nstrips x Lr floats at growoff[k] in ring
     int gworkers;                           // one-wave workgroups of the fill
+    // any pair outside the one-wave list
+    bool any_big() const { return nmw[0] > 0 || nmw[1] > 0 || nmw[2] > 0 || ngrid > 0; }
 };
 
 // Pairs of a chunk split by refinement size: `small` pairs (nullptr: every
@@ -77,6 +79,9 @@ hipError_t upload_length_tables();
 // work queue instead of the small bucket (batched form)
 hipError_t launch_dp(const DevChunk& Q, const DevChunk& R, const DevState& S, const DpLaunch& L,
                      const DpBuckets& B, const WorkQ& wq, int pgrid, hipStream_t st);
+// score-only DP fills of nslots superpositions per pair (scores[k*np + p])
+hipError_t launch_dp_score(const DevChunk& Q, const DevChunk& R, const DevState& S, int gap0, float gapopen,
+                           const float* tfms, const int* slotflags, int nslots, float* scores, hipStream_t st);
 // prec selects FP64 covariance and solve for the final-stage modes (FIN1,
 // FIN2, PROD1, PROD2); DPREFINE is a search and stays FP32
 hipError_t launch_refine_dp(int prec, int mode, const DevChunk& Q, const DevChunk& R,

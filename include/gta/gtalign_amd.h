@@ -169,6 +169,22 @@ int gta_set_profiling(gta_ctx* ctx, int on);
 /* fills up to n entries; returns the number of kernel classes */
 int gta_kernel_stats(const gta_ctx* ctx, gta_kernel_stat* out, int n);
 
+/* ---- DP launch-form autotuning (no counterpart in the reference, whose DP
+ * tile shape and launch geometry are fixed, dpw_btck.cu:78-615) ------------
+ * Each pair's DP fill runs as one wave, as one workgroup of 4, 8 or 16 waves,
+ * or pipelined over the whole device.  Off (the default), large pairs take
+ * the grid form when they number at most half the compute units and the
+ * W-wave form otherwise.  On, pairs of >= 4 strips of 64 rows and >= 2^18
+ * cells are grouped by length bucket (log2 of the strip count, log2 of the
+ * reference length, log2 of the bucket's pair count); at the first
+ * gta_process on a chunk pair, each bucket without a cached choice is timed
+ * over the candidate forms (whole-chunk fills, identity superpositions) and
+ * takes the fastest.  Choices are kept in the context and, when cache_path is
+ * non-NULL, read from that text file at this call and appended to it after
+ * tuning (one line per bucket: device tag, bucket, form, milliseconds per
+ * candidate).  Results do not depend on the form (the fills are bit-identical). */
+int gta_set_autotune(gta_ctx* ctx, int on, const char* cache_path);
+
 /* message of the last error on ctx (thread-local when ctx is NULL) */
 const char* gta_last_error(const gta_ctx* ctx);
 

@@ -25,6 +25,16 @@ extern "C" {
 int gta_test_dp(gta_ctx* ctx, const float* tfms, int gap0, float gapopen, int usess, int dpscan,
                 int* nmatched, int* qpos, int* rpos);
 
+/* Score-only DP fills (the kernel behind ExecDPScore3264x / the CPU twin's
+ * ExecDPScore128xKernel, MpDPHub.cpp:389-647, which the frg3 superposition
+ * search uses to rank candidate transforms) of nslots superpositions per pair:
+ * slot k of pair p = qi*nrefs + ri uses tfms[(k*npairs + p)*12] and
+ * scores[k*npairs + p] receives the value of the fill's bottom-right cell
+ * (the score the reference reads back, MpStageFrg3.h:1186-1191), or 0 for a
+ * slot whose slotflags entry (may be NULL) is nonzero.  Search d0. */
+int gta_test_dp_score(gta_ctx* ctx, const float* tfms, const int* slotflags, int nslots, int gap0,
+                      float gapopen, float* scores);
+
 /* Thresholds of the DP's size buckets: a pair is filled by a workgroup of
  * several waves (strips pipelined down the query) when Lq*Lr >= min_cells,
  * it has at least min_strips 64-row strips and Lr >= 256; otherwise by one
@@ -39,6 +49,12 @@ int gta_test_set_dp_buckets(gta_ctx* ctx, long long min_cells, int min_strips);
  * the device's compute units when 0 (the default).  Takes effect at the next
  * gta_process / gta_test_dp. */
 int gta_test_set_dp_grid(gta_ctx* ctx, int mode);
+
+/* DP launch forms of the resident chunk pair as of the last gta_process or
+ * gta_test_dp: counts[0..4] receive the number of pairs filled by one wave,
+ * by 4, 8 and 16 waves and by the device-wide pipeline; tune_seconds (may be
+ * NULL) the host time spent autotuning so far. */
+int gta_test_dp_forms(const gta_ctx* ctx, int* counts, double* tune_seconds);
 
 /* Split of the superposition searches over workgroups: a pair whose shorter
  * structure has at least min_len residues has its seed diagonals

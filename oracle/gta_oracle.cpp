@@ -543,9 +543,12 @@ struct Ctx {
 };
 
 // ---- DP with backtrace (MpDPHub.cpp:73-388 and 687-984) ------------------
-// Fills the Lq x Lr matrix under transform tfm and returns the backtrace codes.
+// Fills the Lq x Lr matrix under transform tfm and returns the backtrace codes;
+// corner, when given, receives the bottom-right cell's value (the score that
+// the score-only fill ExecDPScore128xKernel, MpDPHub.cpp:389-647, leaves at the
+// reference's last position of its diagonal buffer, MpStageFrg3.h:1186-1191).
 void dp_fill(const Str& q, const Str& r, const float* tfm, float d02, float gapopen, bool gap0,
-             bool usess, float ssw, std::vector<uint8_t>& bt, double& cells) {
+             bool usess, float ssw, std::vector<uint8_t>& bt, double& cells, float* corner = nullptr) {
     const int Lq = q.len, Lr = r.len;
     bt.assign((size_t)Lq * Lr, 0);
     std::vector<float> prev(Lr + 1, 0.f), cur(Lr + 1, 0.f);  // index j+1; [0] is the zero boundary
@@ -579,6 +582,7 @@ void dp_fill(const Str& q, const Str& r, const float* tfm, float d02, float gapo
         }
         std::swap(prev, cur);
     }
+    if (corner) *corner = (Lq > 0 && Lr > 0) ? prev[Lr] : 0.f;
     cells += (double)Lq * Lr;
 }
 
@@ -1360,6 +1364,21 @@ int gta_oracle_dp(const float* qx, const float* qy, const float* qz, const char*
     backtrace(bt, Lq, Lr, qp, rp);
     for (size_t k = 0; k < qp.size(); k++) { qpos[k] = qp[k]; rpos[k] = rp[k]; }
     return (int)qp.size();
+}
+
+// Score-only DP fill (ExecDPScore128xKernel<GAP0>, MpDPHub.cpp:389-647) under
+// transform tfm with the search d0 (GetD02): the bottom-right cell's value.
+float gta_oracle_dp_score(const float* qx, const float* qy, const float* qz, int Lq, const float* rx,
+                          const float* ry, const float* rz, int Lr, const float* tfm, float gapopen,
+                          int gap0) {
+    Str q, r;
+    q.len = Lq; q.x = qx; q.y = qy; q.z = qz; q.ss = nullptr; q.rsd = nullptr;
+    r.len = Lr; r.x = rx; r.y = ry; r.z = rz; r.ss = nullptr; r.rsd = nullptr;
+    std::vector<uint8_t> bt;
+    double cells = 0;
+    float corner = 0.f;
+    dp_fill(q, r, tfm, GetD02(Lq, Lr), gapopen, gap0 != 0, false, 0.f, bt, cells, &corner);
+    return corner;
 }
 
 // Kabsch superposition of n pairs (x -> y) in the reference's FP32 arithmetic.

@@ -35,6 +35,9 @@ def lib():
                                        [C.c_int, C.c_void_p, C.c_float, C.c_float, C.c_int, C.c_int,
                                         C.c_float, C.c_void_p, C.c_void_p, C.c_void_p])
         _orc.gta_oracle_dp.restype = C.c_int
+        _orc.gta_oracle_dp_score.argtypes = ([C.c_void_p] * 3 + [C.c_int] + [C.c_void_p] * 3 +
+                                             [C.c_int, C.c_void_p, C.c_float, C.c_int])
+        _orc.gta_oracle_dp_score.restype = C.c_float
         _orc.gta_oracle_kabsch.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
         _orc.gta_oracle_rmsd.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
         _orc.gta_oracle_rmsd.restype = C.c_float
@@ -127,6 +130,19 @@ def dp(qxyz, rxyz, tfm, d02, gapopen=0.0, gap0=True, qss=None, rss=None, ssw=0.0
     return bt, np.stack([qp[:n], rp[:n]], axis=1)
 
 
+def dp_score(qxyz, rxyz, tfm, gap0=True, gapopen=0.0):
+    """Score-only DP fill (ExecDPScore128xKernel, MpDPHub.cpp:389-647) of one
+    pair under tfm with the search d0: the bottom-right cell's value."""
+    qxyz = np.ascontiguousarray(qxyz, np.float32)
+    rxyz = np.ascontiguousarray(rxyz, np.float32)
+    qc = [np.ascontiguousarray(qxyz[:, i]) for i in range(3)]
+    rc = [np.ascontiguousarray(rxyz[:, i]) for i in range(3)]
+    t = np.ascontiguousarray(tfm, np.float32)
+    return float(lib().gta_oracle_dp_score(qc[0].ctypes.data, qc[1].ctypes.data, qc[2].ctypes.data, len(qxyz),
+                                           rc[0].ctypes.data, rc[1].ctypes.data, rc[2].ctypes.data, len(rxyz),
+                                           t.ctypes.data, gapopen, int(gap0)))
+
+
 def kabsch(x, y):
     x = np.ascontiguousarray(x, np.float32)
     y = np.ascontiguousarray(y, np.float32)
@@ -149,6 +165,29 @@ def d0(qlen, rlen):
 
 def fraglen(qlen, rlen, qpos, rpos, ndx):
     return lib().gta_oracle_fraglen(qlen, rlen, qpos, rpos, ndx)
+
+
+def ref_dp_score(qxyz, rxyz, tfms):
+    """The reference's own score-only DP (oracle/_ref, ExecDPScore128xKernel
+    with GAP0) of one pair under K transforms tfms[K, 12]: K scores."""
+    L = ref_lib()
+    if not hasattr(L, "_dpscore_ready"):
+        L.gta_ref_dpscore.argtypes = ([C.c_void_p] * 3 + [C.c_int] + [C.c_void_p] * 3 +
+                                      [C.c_int, C.c_void_p, C.c_int, C.c_void_p])
+        L.gta_ref_dpscore.restype = C.c_int
+        L._dpscore_ready = True
+    qxyz = np.ascontiguousarray(qxyz, np.float32)
+    rxyz = np.ascontiguousarray(rxyz, np.float32)
+    qc = [np.ascontiguousarray(qxyz[:, i]) for i in range(3)]
+    rc = [np.ascontiguousarray(rxyz[:, i]) for i in range(3)]
+    t = np.ascontiguousarray(tfms, np.float32).reshape(-1, 12)
+    out = np.zeros(len(t), np.float32)
+    rc_ = L.gta_ref_dpscore(qc[0].ctypes.data, qc[1].ctypes.data, qc[2].ctypes.data, len(qxyz), rc[0].ctypes.data,
+                            rc[1].ctypes.data, rc[2].ctypes.data, len(rxyz), t.ctypes.data, len(t),
+                            out.ctypes.data)
+    if rc_ < 0:
+        raise RuntimeError("reference score-only DP failed (%d)" % rc_)
+    return out
 
 
 def ref_dp(qxyz, rxyz, tfm, gap0=True, gapopen=0.0):

@@ -341,5 +341,73 @@ int gta_ref_dp(const float* qx, const float* qy, const float* qz, int Lq, const 
     }
 }
 
+// The reference's score-only fill, MpDPHub::ExecDPScore128xKernel<GAP0=true>
+// (MpDPHub.cpp:389-647), for K superpositions of one pair (tfms[k*12]); each
+// score is read where MpStageFrg3::SortBestDPscoresAndTMsAmongDPswifts reads it
+// (MpStageFrg3.h:1180-1191).  Returns 0, or < 0 on error.
+int gta_ref_dpscore(const float* qx, const float* qy, const float* qz, int Lq, const float* rx,
+                    const float* ry, const float* rz, int Lr, const float* tfms, int K, float* scores) {
+    try {
+        std::vector<int> ql{Lq}, rl{Lr}, zq(Lq, 0), zr(Lr, 0);
+        std::vector<unsigned> d0{0};
+        std::vector<char> aq(Lq, 'A'), ar(Lr, 'A');
+        gta_chunk_view q{1, Lq, ql.data(), d0.data(), nullptr, qx, qy, qz, zq.data(), nullptr, aq.data(), nullptr};
+        gta_chunk_view r{1, Lr, rl.data(), d0.data(), nullptr, rx, ry, rz, zr.data(), nullptr, ar.data(), nullptr};
+        ChunkFields qf, rf;
+        fill(&q, qf);
+        fill(&r, rf);
+        static MpGlobalMemory* gmem = nullptr;
+        if (!gmem) gmem = new MpGlobalMemory((size_t)2 * ONEG, 1);
+        gmem->CalcMaxDbDataChunkSize(Lq);
+        const size_t dbxpad = ALIGN_UP((size_t)Lr, CUL2CLINESIZE) - Lr;
+        if (gmem->GetCurrentMaxDbPos() < Lr + dbxpad) return -4;
+        auto sec = [&](int s) { return gmem->GetHeap() + gmem->GetHeapSectionOffset(0, s); };
+        const uint maxnsteps = MpGlobalMemory::GetMaxNFragSteps();
+        const int depth = CLOptions::GetC_DEPTH();
+        const int nconfigs = CUS1_TBSP_DPSCORE_TOP_N *
+                             ((depth <= CLOptions::csdDeep) ? 3 : ((depth <= CLOptions::csdHigh) ? 2 : 1));
+        if (K < 1 || K > nconfigs || (uint)nconfigs > maxnsteps) return -2;
+        float* tmpdpdiagbuffers = (float*)sec(MpGlobalMemory::ddsBegOfDPDiagScores);
+        float* tmpdpbotbuffer = (float*)sec(MpGlobalMemory::ddsBegOfDPBottomScores);
+        float* wrkmemtm = (float*)sec(MpGlobalMemory::ddsBegOfWrkMemoryTM);
+        float* auxwrkmemory = (float*)sec(MpGlobalMemory::ddsBegOfAuxWrkMemory);
+        // one query and one reference: slot si's transform at si*12, its
+        // convergence flag at (si*nTAuxWorkingMemoryVars + tawmvConverged)
+        for (int si = 0; si < nconfigs; si++) {
+            for (int f = 0; f < nTTranformMatrix; f++)
+                wrkmemtm[si * nTTranformMatrix + f] = si < K ? tfms[si * 12 + f] : ((f == 0 || f == 4 || f == 8) ? 1.f : 0.f);
+            auxwrkmemory[si * nTAuxWorkingMemoryVars + tawmvConverged] = 0.f;
+        }
+        MpDPHub dph(maxnsteps, qf.beg, qf.end, rf.beg, rf.end, 1, 1, Lq, Lr, Lq, Lr, dbxpad,
+                    tmpdpdiagbuffers, tmpdpbotbuffer, (float*)sec(MpGlobalMemory::ddsBegOfDPAlignedPoss),
+                    (uint*)sec(MpGlobalMemory::ddsBegOfDPMaxCoords), sec(MpGlobalMemory::ddsBegOfDPBackTckData),
+                    (float*)sec(MpGlobalMemory::ddsBegOfWrkMemory),
+                    (float*)sec(MpGlobalMemory::ddsBegOfWrkMemoryCCD), wrkmemtm,
+                    (float*)sec(MpGlobalMemory::ddsBegOfWrkMemoryTMibest), auxwrkmemory,
+                    (float*)sec(MpGlobalMemory::ddsBegOfWrkMemory2),
+                    (float*)sec(MpGlobalMemory::ddsBegOfDP2AlnData), (float*)sec(MpGlobalMemory::ddsBegOfTfmMatrices),
+                    (uint*)sec(MpGlobalMemory::ddsBegOfGlobVars));
+        {
+            // no value of an earlier call may survive in the buffers read below
+            const size_t dblen0 = (size_t)Lr + dbxpad;
+            std::fill(tmpdpdiagbuffers, tmpdpdiagbuffers + (size_t)nTDPDiagScoreSections * nTDPDiagScoreSubsections *
+                                                               (size_t)nconfigs * dblen0, 0.f);
+            std::fill(tmpdpbotbuffer, tmpdpbotbuffer + (size_t)nTDPDiagScoreSubsections * (size_t)nconfigs * dblen0,
+                      0.f);
+        }
+        dph.ExecDPScore128xKernel<true>(0.0f, qf.beg, rf.beg, wrkmemtm, auxwrkmemory, tmpdpdiagbuffers,
+                                        tmpdpbotbuffer);
+        const int dblen = Lr + (int)dbxpad;
+        for (int si = 0; si < K; si++) {
+            const int yofff = si * dblen;
+            const int doffs = nTDPDiagScoreSections * nTDPDiagScoreSubsections * yofff;
+            scores[si] = tmpdpdiagbuffers[doffs + 0 + Lr - 1];
+        }
+        return 0;
+    } catch (...) {
+        return -1;
+    }
+}
+
 }  // extern "C"
 

@@ -429,6 +429,71 @@ def test_large_pair_default_buckets():
     assert check_fp64(gm, q, r, PREC_MIXED) >= 1
 
 
+def test_dp_score_only():
+    """Score-only DP fills (the frg3 ranking kernel) of three superpositions
+    per pair: the value of each fill's bottom-right cell equals the oracle's
+    exact fill bit for bit; flagged slots score 0."""
+    rng = np.random.default_rng(21)
+    qs = [synth.chain(60 + k, n) for k, n in enumerate((90, 300, 700))]
+    rs = [synth.chain(70 + k, n) for k, n in enumerate((50, 260, 1100, 130))]
+    q = Chunk.from_structures(qs, [synth.residues(rng, len(x)) for x in qs])
+    r = Chunk.from_structures(rs, [synth.residues(rng, len(x)) for x in rs])
+    K = 3
+    tf = np.zeros((K, q.nstrs, r.nstrs, 12), np.float32)
+    for k in range(K):
+        for i in range(q.nstrs):
+            for j in range(r.nstrs):
+                tf[k, i, j, :9] = random_rotation(rng).reshape(-1)
+                tf[k, i, j, 9:] = rng.normal(scale=3.0, size=3)
+    flags = np.zeros((K, q.nstrs, r.nstrs), np.int32)
+    flags[1, 0, 2] = 1
+    with GtaBatch(0, default_options(), max_qrys=q.nstrs, max_qry_poss=q.nposs, max_refs=r.nstrs,
+                  max_ref_poss=r.nposs) as b:
+        b.TransferQueryPMDataToDevice(q)
+        b.TransferCPMDataToDevice(r)
+        got = b.dp_score(tf, flags)
+    for k in range(K):
+        for i in range(q.nstrs):
+            for j in range(r.nstrs):
+                want = 0.0 if flags[k, i, j] else oracle.dp_score(q.coords(i), r.coords(j), tf[k, i, j])
+                assert got[k, i, j] == np.float32(want), (k, i, j, got[k, i, j], want)
+
+
+def test_dp_autotune_cache(tmp_path):
+    """DP launch-form autotuning: every length bucket of large pairs is timed
+    once over the candidate forms and takes the fastest; the results equal the
+    untuned run bit for bit; a second context reads the choices from the cache
+    file instead of timing again."""
+    rng = np.random.default_rng(8)
+    a = synth.assembly(rng, ndom=10, total=2520)
+    refs = [synth.rigid_copy(rng, a, 0.05), synth.assembly(rng, ndom=10, total=2400)]
+    refs += [synth.chain(40 + k, 300 + 150 * k) for k in range(6)]
+    qs = [a, synth.chain(39, 700)]
+    q = Chunk.from_structures(qs, [synth.residues(rng, len(x)) for x in qs])
+    r = Chunk.from_structures(refs, [synth.residues(rng, len(x)) for x in refs])
+    base = run_gpu(q, r)
+    cache = tmp_path / "dp_forms.txt"
+    got = []
+    for k in range(2):
+        with GtaBatch(0, default_options(), max_qrys=q.nstrs, max_qry_poss=q.nposs, max_refs=r.nstrs,
+                      max_ref_poss=r.nposs) as b:
+            b.set_autotune(True, cache)
+            b.TransferQueryPMDataToDevice(q)
+            b.TransferCPMDataToDevice(r)
+            g = b.ProcessBlock()
+            got.append(b.dp_forms())
+            lines = cache.read_text().splitlines()
+        assert np.array_equal(g.alns, base.alns), k
+        assert np.array_equal(g.alndata, base.alndata), k
+        assert np.array_equal(g.tfms, base.tfms), k
+        if k == 0:
+            lines0 = lines
+    (forms0, secs0), (forms1, secs1) = got
+    assert secs0 > 0 and len(lines0) >= 2
+    assert sum(forms0.values()) == q.nstrs * r.nstrs
+    assert secs1 == 0 and forms1 == forms0 and lines == lines0
+
+
 @pytest.mark.parametrize("prec", [PREC_FP32, PREC_MIXED], ids=["fp32", "mixed"])
 def test_batched_form_equals_plain(prec):
     """The batched form (live-pair lists built on the device after each filter,
