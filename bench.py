@@ -102,6 +102,26 @@ def time_form(qc, rc, prec, steps, warmup, profile, batched=0, tune=None):
     return wall, dev, cells, stats
 
 
+def pcie_inclusive(qc, rc):
+    """The boundary takes host buffers: the plain FP32 form timed with the
+    references' upload (host -> HBM) and the results' download (HBM -> host,
+    records, transforms and alignment rows) inside the clock, after one
+    untimed call on the same context."""
+    with GtaBatch(0, default_options(), max_qrys=qc.nstrs, max_qry_poss=qc.nposs, max_refs=rc.nstrs,
+                  max_ref_poss=rc.nposs) as b:
+        b.TransferQueryPMDataToDevice(qc)
+        b.TransferCPMDataToDevice(rc)
+        b.ProcessBlock()
+        t0 = time.perf_counter()
+        b.TransferCPMDataToDevice(rc)
+        t1 = time.perf_counter()
+        b.ProcessBlock()
+        t2 = time.perf_counter()
+    n = qc.nstrs * rc.nstrs
+    return {"pairs_per_s": round(n / (t2 - t0), 1), "upload_s": round(t1 - t0, 4),
+            "process_and_download_s": round(t2 - t1, 4)}
+
+
 def _ref_run(q, r, nthreads, tmp):
     """The reference's CPU path on (q, r) in a child process (its thread count
     is fixed per process); returns (alndata, seconds)."""
@@ -383,6 +403,8 @@ def main():
     wall, dev, cells, stats = time_form(qc, rc, PREC_FP32, args.steps, args.warmup, True)
     ms = 1e3 * wall / args.steps
     value = npairs * args.steps / wall
+    log("PCIe-inclusive call")
+    pcie = pcie_inclusive(qc, rc)
     mixed = None
     if not args.no_mixed:
         log("mixed-precision form")
@@ -452,7 +474,9 @@ def main():
                                "20%% homologs; default options" % rc.nstrs,
                    "pairs": npairs, "form": "plain FP32, one chunk per launch sequence"},
         "dp_gcups": round(cells / dev / 1e9, 2),
+        "dp_kernel_gcups": round(cells / args.steps / kern["dp"]["seconds"] / 1e9, 2) if "dp" in kern else None,
         "mixed_precision_pairs_per_s": round(mixed, 1) if mixed else None,
+        "pcie_inclusive": pcie,
         "kernels": kern,
         "roofline": roofline,
         "batched_vs_plain": batched,

@@ -89,7 +89,10 @@ class BlockResults:
             .reshape(nq, nr, GTA_ALN_NFIELDS)
         self.tfms = np.ctypeslib.as_array(res.tfms, (nq * nr * GTA_TFM_NFIELDS,)).copy() \
             .reshape(nq, nr, GTA_TFM_NFIELDS)
-        self.alns = np.frombuffer(C.string_at(res.alns, res.alnsize), dtype=np.uint8).copy()
+        # the alignment section can exceed 2 GiB (C.string_at takes an int size)
+        n = int(res.alnsize)
+        self.alns = (np.ctypeslib.as_array(C.cast(res.alns, C.POINTER(C.c_uint8)), (n,)).copy() if n
+                     else np.zeros(0, np.uint8))
         self.seconds = res.seconds
         self.cells = res.dp_cells
 

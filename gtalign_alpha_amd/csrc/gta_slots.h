@@ -13,8 +13,12 @@
 
 namespace gta {
 
-constexpr int GL = 16;                  // lanes per slot group (one DPP row)
+#ifndef GTA_SLOT_LANES
+#define GTA_SLOT_LANES 16
+#endif
+constexpr int GL = GTA_SLOT_LANES;      // lanes per slot group: a DPP row (16) or half row (8)
 constexpr int NGW = GTA_WAVE / GL;      // slot groups per wave
+static_assert(GL == 8 || GL == 16, "slot groups are a DPP row or half row");
 
 __device__ __forceinline__ int glane() { return lane_id() & (GL - 1); }
 
@@ -34,27 +38,27 @@ template <int CTRL> __device__ __forceinline__ double dpp_d(double v) {
     return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | (unsigned long long)lo);
 }
 
-// Sum over the 16 lanes of the row.  Partners at each step hold disjoint
-// partial sums and add them commutatively, so all 16 lanes end bitwise equal.
+// Sum over the GL lanes of the group.  Partners at each step hold disjoint
+// partial sums and add them commutatively, so all lanes end bitwise equal.
 __device__ __forceinline__ float gsum(float v) {
     v = v + dpp_f<DPP_XOR1>(v);
     v = v + dpp_f<DPP_XOR2>(v);
     v = v + dpp_f<DPP_HMIRROR>(v);
-    v = v + dpp_f<DPP_MIRROR>(v);
+    if (GL == 16) v = v + dpp_f<DPP_MIRROR>(v);
     return v;
 }
 __device__ __forceinline__ double gsum(double v) {
     v = v + dpp_d<DPP_XOR1>(v);
     v = v + dpp_d<DPP_XOR2>(v);
     v = v + dpp_d<DPP_HMIRROR>(v);
-    v = v + dpp_d<DPP_MIRROR>(v);
+    if (GL == 16) v = v + dpp_d<DPP_MIRROR>(v);
     return v;
 }
 __device__ __forceinline__ int gsum(int v) {
     v = v + dpp_i<DPP_XOR1>(v);
     v = v + dpp_i<DPP_XOR2>(v);
     v = v + dpp_i<DPP_HMIRROR>(v);
-    v = v + dpp_i<DPP_MIRROR>(v);
+    if (GL == 16) v = v + dpp_i<DPP_MIRROR>(v);
     return v;
 }
 
@@ -69,7 +73,7 @@ template <class M> __device__ __forceinline__ float gthird(M& m) {
     gmerge3_step<DPP_XOR1>(m);
     gmerge3_step<DPP_XOR2>(m);
     gmerge3_step<DPP_HMIRROR>(m);
-    gmerge3_step<DPP_MIRROR>(m);
+    if (GL == 16) gmerge3_step<DPP_MIRROR>(m);
     return m.c;
 }
 
