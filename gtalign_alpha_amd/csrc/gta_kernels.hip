@@ -196,14 +196,27 @@ struct SlotBest {
     __device__ __forceinline__ bool beats(float b, int s) const { return b > best || (b == best && s < slot); }
 };
 constexpr int NGROUPS = RWAVES * NGW;   // slot groups per workgroup
+// The seed search runs narrower groups (more seeds per wave): its seed
+// diagonals are many and short-lived, so each 3x3 solve, computed once per
+// group, weighs more than in the refinement kernels.  Measured on the
+// 16 x 50,000 workload: 16 lanes 0.623 s, 8 lanes 0.449 s, 4 lanes 0.378 s,
+// 2 lanes 0.372 s, 1 lane 0.392 s.
+#ifndef GTA_FINDFRAG_LANES
+#define GTA_FINDFRAG_LANES 4
+#endif
+static_assert(GTA_FINDFRAG_LANES == 1 || GTA_FINDFRAG_LANES == 2 || GTA_FINDFRAG_LANES == 4 ||
+                  GTA_FINDFRAG_LANES == 8 || GTA_FINDFRAG_LANES == 16,
+              "seed-search groups of 1, 2, 4, 8 or 16 lanes");
+constexpr int FF_GL = GTA_FINDFRAG_LANES;
 
-// the workgroup's best over its groups (every thread returns it)
+// the workgroup's best over its groups of G lanes (every thread returns it)
+template <int G = GL>
 __device__ void group_best(const SlotBest& mine, SlotBest* lds, SlotBest& out) {
-    const int g = threadIdx.x / GL;
-    if (glane() == 0) lds[g] = mine;
+    const int g = threadIdx.x / G;
+    if (glane<G>() == 0) lds[g] = mine;
     __syncthreads();
     out = lds[0];
-    for (int i = 1; i < NGROUPS; i++)
+    for (int i = 1; i < RTHREADS / G; i++)
         if (out.beats(lds[i].best, lds[i].slot)) out = lds[i];
 }
 // the same when each group already keeps its record in lds[group]
@@ -216,27 +229,27 @@ __device__ void group_best_lds(const SlotBest* lds, SlotBest& out) {
 
 // covariance over n diagonal positions from (qp, rp); with thr, only pairs
 // within thr (squared distance under tfm); returns the group's count
-template <int P>
+template <int P, int G = GL>
 __device__ __forceinline__ int gcc_gapless(const Coords& q, const Coords& r, int qp, int rp, int n,
                                            const float* tfm, bool usethr, float thr,
                                            typename GPrec<P>::T* cc) {
     typename GPrec<P>::Acc acc;
     acc.clear();
     int cnt = 0;
-    for (int k = glane(); k < n; k += GL) {
+    for (int k = glane<G>(); k < n; k += G) {
         const int a = qp + k, b = rp + k;
         const float qx = q.x[a], qy = q.y[a], qz = q.z[a], rx = r.x[b], ry = r.y[b], rz = r.z[b];
         if (usethr && thr < tfm_dist2(tfm, qx, qy, qz, rx, ry, rz)) continue;
         acc.add(qx, qy, qz, rx, ry, rz);
         cnt++;
     }
-    acc.reduce(cc);
-    return gsum(cnt);
+    acc.template reduce<G>(cc);
+    return gsum<G>(cnt);
 }
 
 // seed-diagonal scores (CalcScoresUnrl_Complete, MpStage1.h:700-790): all
 // pairs scored; next threshold from the 3rd smallest distinct distance
-template <int P>
+template <int P, int G = GL>
 __device__ __forceinline__ float gscore_seed(const Coords& q, const Coords& r, int qp, int rp, int n,
                                              const float* tfm, float d0, float d02, bool calc2, bool want_thr,
                                              float* thr) {
@@ -244,15 +257,15 @@ __device__ __forceinline__ float gscore_seed(const Coords& q, const Coords& r, i
     sc.clear();
     Min3d m3;
     m3.init();
-    for (int k = glane(); k < n; k += GL) {
+    for (int k = glane<G>(); k < n; k += G) {
         const int a = qp + k, b = rp + k;
         const float d = tfm_dist2(tfm, q.x[a], q.y[a], q.z[a], r.x[b], r.y[b], r.z[b]);
         sc.add(pair_score(d02, d));
         m3.put(d);
     }
-    const float s = sc.reduce();
+    const float s = sc.template reduce<G>();
     if (want_thr) {
-        float min3 = gthird(m3);
+        float min3 = gthird<G>(m3);
         const float d0s = GetD0s(d0);
         float d02s = d0s * d0s;
         if (calc2) d02s += 1.0f;
@@ -312,7 +325,8 @@ __device__ __forceinline__ void part_range(int n0, int n1, int part, int nparts,
 template <int P>
 __global__ __launch_bounds__(RTHREADS) void k_findfrag(DevChunk Q, DevChunk R, DevState S, Geom G,
                                                         int dinv_on, const int* plist, SlotBest* parts) {
-    __shared__ SlotBest lds[NGROUPS];
+    constexpr int FG = FF_GL;
+    __shared__ SlotBest lds[RTHREADS / FG];
     __shared__ int ctr;
     const int p = plist ? plist[blockIdx.x] : (int)blockIdx.x;
     const int qi = pair_q(S, p), ri = pair_r(S, p);
@@ -345,7 +359,7 @@ __global__ __launch_bounds__(RTHREADS) void k_findfrag(DevChunk Q, DevChunk R, D
     for (;;) {
         // groups without a slot take the next valid seed diagonal
         for (;;) {
-            const int got = fetch_slot(need, &ctr);
+            const int got = fetch_slot<FG>(need, &ctr);
             if (need) {
                 if (got >= hi) {
                     need = false;
@@ -369,14 +383,14 @@ __global__ __launch_bounds__(RTHREADS) void k_findfrag(DevChunk Q, DevChunk R, D
         if (!__any(live)) break;
         if (live) {
             T cc[15];
-            const int cnt = gcc_gapless<P>(q, r, qpos, rpos, n, tfm, round > 0, thr, cc);
+            const int cnt = gcc_gapless<P, FG>(q, r, qpos, rpos, n, tfm, round > 0, thr, cc);
             ab += (round > 0 ? AB_EXT : AB_CC) * (unsigned long long)n;
             bool done = false;
             if (round > 0 && (cnt < 1 || cnt == n)) {
                 done = true;
             } else {
                 kabsch<T>(cc, (T)cnt, dinv, tfm);
-                const float sc = gscore_seed<P>(q, r, qpos, rpos, n, tfm, d0, d02, round > 0, round < 2, &thr);
+                const float sc = gscore_seed<P, FG>(q, r, qpos, rpos, n, tfm, d0, d02, round > 0, round < 2, &thr);
                 ab += AB_TM * (unsigned long long)n + AB_SLOT;
                 if (best < sc) best = sc;
                 if (++round == 3) done = true;
@@ -390,9 +404,9 @@ __global__ __launch_bounds__(RTHREADS) void k_findfrag(DevChunk Q, DevChunk R, D
             }
         }
     }
-    if (glane() == 0 && ab) atomicAdd(S.algbytes + KC_FINDFRAG, ab);
+    if (glane<FG>() == 0 && ab) atomicAdd(S.algbytes + KC_FINDFRAG, ab);
     SlotBest b;
-    group_best(mine, lds, b);
+    group_best<FG>(mine, lds, b);
     if (threadIdx.x == 0) {
         if (parts) parts[(size_t)blockIdx.x * gridDim.y + blockIdx.y] = b;
         else findfrag_epilogue(S, p, b);

@@ -18,9 +18,11 @@ namespace gta {
 #endif
 constexpr int GL = GTA_SLOT_LANES;      // lanes per slot group: a DPP row (16) or half row (8)
 constexpr int NGW = GTA_WAVE / GL;      // slot groups per wave
-static_assert(GL == 8 || GL == 16, "slot groups are a DPP row or half row");
+static_assert(GL == 8 || GL == 16, "the refinement kernels' groups are a DPP row or half row");
 
-__device__ __forceinline__ int glane() { return lane_id() & (GL - 1); }
+// Every group-level helper takes the group width G (1, 2, 4, 8 or 16 lanes) as
+// a template parameter, GL by default; a kernel may use its own width.
+template <int G = GL> __device__ __forceinline__ int glane() { return lane_id() & (G - 1); }
 
 // DPP row permutations used by the group reductions
 enum { DPP_XOR1 = 0xB1, DPP_XOR2 = 0x4E, DPP_MIRROR = 0x140, DPP_HMIRROR = 0x141 };
@@ -38,27 +40,27 @@ template <int CTRL> __device__ __forceinline__ double dpp_d(double v) {
     return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | (unsigned long long)lo);
 }
 
-// Sum over the GL lanes of the group.  Partners at each step hold disjoint
+// Sum over the G lanes of the group.  Partners at each step hold disjoint
 // partial sums and add them commutatively, so all lanes end bitwise equal.
-__device__ __forceinline__ float gsum(float v) {
-    v = v + dpp_f<DPP_XOR1>(v);
-    v = v + dpp_f<DPP_XOR2>(v);
-    v = v + dpp_f<DPP_HMIRROR>(v);
-    if (GL == 16) v = v + dpp_f<DPP_MIRROR>(v);
+template <int G = GL> __device__ __forceinline__ float gsum(float v) {
+    if (G >= 2) v = v + dpp_f<DPP_XOR1>(v);
+    if (G >= 4) v = v + dpp_f<DPP_XOR2>(v);
+    if (G >= 8) v = v + dpp_f<DPP_HMIRROR>(v);
+    if (G >= 16) v = v + dpp_f<DPP_MIRROR>(v);
     return v;
 }
-__device__ __forceinline__ double gsum(double v) {
-    v = v + dpp_d<DPP_XOR1>(v);
-    v = v + dpp_d<DPP_XOR2>(v);
-    v = v + dpp_d<DPP_HMIRROR>(v);
-    if (GL == 16) v = v + dpp_d<DPP_MIRROR>(v);
+template <int G = GL> __device__ __forceinline__ double gsum(double v) {
+    if (G >= 2) v = v + dpp_d<DPP_XOR1>(v);
+    if (G >= 4) v = v + dpp_d<DPP_XOR2>(v);
+    if (G >= 8) v = v + dpp_d<DPP_HMIRROR>(v);
+    if (G >= 16) v = v + dpp_d<DPP_MIRROR>(v);
     return v;
 }
-__device__ __forceinline__ int gsum(int v) {
-    v = v + dpp_i<DPP_XOR1>(v);
-    v = v + dpp_i<DPP_XOR2>(v);
-    v = v + dpp_i<DPP_HMIRROR>(v);
-    if (GL == 16) v = v + dpp_i<DPP_MIRROR>(v);
+template <int G = GL> __device__ __forceinline__ int gsum(int v) {
+    if (G >= 2) v = v + dpp_i<DPP_XOR1>(v);
+    if (G >= 4) v = v + dpp_i<DPP_XOR2>(v);
+    if (G >= 8) v = v + dpp_i<DPP_HMIRROR>(v);
+    if (G >= 16) v = v + dpp_i<DPP_MIRROR>(v);
     return v;
 }
 
@@ -69,11 +71,11 @@ template <int CTRL, class M> __device__ __forceinline__ void gmerge3_step(M& m) 
     m.put(y);
     m.put(z);
 }
-template <class M> __device__ __forceinline__ float gthird(M& m) {
-    gmerge3_step<DPP_XOR1>(m);
-    gmerge3_step<DPP_XOR2>(m);
-    gmerge3_step<DPP_HMIRROR>(m);
-    if (GL == 16) gmerge3_step<DPP_MIRROR>(m);
+template <int G = GL, class M> __device__ __forceinline__ float gthird(M& m) {
+    if (G >= 2) gmerge3_step<DPP_XOR1>(m);
+    if (G >= 4) gmerge3_step<DPP_XOR2>(m);
+    if (G >= 8) gmerge3_step<DPP_HMIRROR>(m);
+    if (G >= 16) gmerge3_step<DPP_MIRROR>(m);
     return m.c;
 }
 
@@ -91,9 +93,9 @@ struct GAccF32 {
         s[9] += qx; s[10] += qy; s[11] += qz;
         s[12] += rx; s[13] += ry; s[14] += rz;
     }
-    __device__ __forceinline__ void reduce(float* out) {
+    template <int G = GL> __device__ __forceinline__ void reduce(float* out) {
 #pragma unroll
-        for (int i = 0; i < 15; i++) out[i] = gsum(s[i]);
+        for (int i = 0; i < 15; i++) out[i] = gsum<G>(s[i]);
     }
 };
 // FP64 per lane: products of FP32 coordinates are exact in FP64 (24+24 < 53
@@ -115,9 +117,9 @@ struct GAccF64 {
         s[9] += x; s[10] += y; s[11] += z;
         s[12] += a; s[13] += b; s[14] += c;
     }
-    __device__ __forceinline__ void reduce(double* out) {
+    template <int G = GL> __device__ __forceinline__ void reduce(double* out) {
 #pragma unroll
-        for (int i = 0; i < 15; i++) out[i] = gsum(s[i]);
+        for (int i = 0; i < 15; i++) out[i] = gsum<G>(s[i]);
     }
 };
 template <int P> struct GScore;
@@ -125,7 +127,7 @@ template <> struct GScore<0> {
     float s;
     __device__ __forceinline__ void clear() { s = 0.f; }
     __device__ __forceinline__ void add(float v) { s += v; }
-    __device__ __forceinline__ float reduce() { return gsum(s); }
+    template <int G = GL> __device__ __forceinline__ float reduce() { return gsum<G>(s); }
 };
 template <> struct GScore<1> {
     float s, c;
@@ -135,7 +137,7 @@ template <> struct GScore<1> {
         c = __fsub_rn(__fsub_rn(t, s), y);
         s = t;
     }
-    __device__ __forceinline__ float reduce() { return (float)gsum((double)s - (double)c); }
+    template <int G = GL> __device__ __forceinline__ float reduce() { return (float)gsum<G>((double)s - (double)c); }
 };
 
 template <int P> struct GPrec;
@@ -144,8 +146,8 @@ template <> struct GPrec<1> { typedef double T; typedef GAccF64 Acc; };
 
 // Groups of this wave that need a slot take consecutive indices from the
 // workgroup's counter in LDS.  Called by the whole wave (convergent).
-__device__ __forceinline__ int fetch_slot(bool need, int* ctr) {
-    const unsigned long long lead = __ballot(need && glane() == 0);
+template <int G = GL> __device__ __forceinline__ int fetch_slot(bool need, int* ctr) {
+    const unsigned long long lead = __ballot(need && glane<G>() == 0);
     const int cnt = __popcll(lead);
     int base = 0;
     if (cnt) {
@@ -153,7 +155,7 @@ __device__ __forceinline__ int fetch_slot(bool need, int* ctr) {
         if (lane_id() == 0) b = atomicAdd(ctr, cnt);
         base = __shfl(b, 0, GTA_WAVE);
     }
-    const int g0 = lane_id() & ~(GL - 1);
+    const int g0 = lane_id() & ~(G - 1);
     const int rank = __popcll(lead & ((1ull << g0) - 1ull));
     return need ? base + rank : -1;
 }
