@@ -602,8 +602,10 @@ static int prepare_state(gta_ctx* c, DevState& S, size_t& alnsize) {
         }
         c->rsp = Sp;
     }
-    // batched form: per-pair bucket marks and the live-pair lists
-    if (c->opts.batched) {
+    // per-pair bucket marks and the live-pair lists (both forms list the pairs
+    // left after each filter; the batched form drains the lists with
+    // persistent grids, the plain form launches one workgroup per listed pair)
+    {
         c->bigmask = nullptr;
         if (c->dpb.any_big() || c->rsp.nbig > 0) {
             std::vector<unsigned char> m(np, 0);
@@ -677,34 +679,46 @@ static int run_stage(gta_ctx* c, double* seconds, double* cells) {
     } while (0)
     int maxaln = std::min(c->q.len.front(), c->r.len.front());
     RefineLaunch RL{dinv, 0, 0, 0, 0.f, std::min(maxaln, 1536)};
-    // batched form: after each filter the live pairs are listed on the device
-    // and the launches that follow are persistent grids draining those lists
+    // After each filter (the stage-1 prescreen, the -s threshold) the live
+    // pairs are listed on the device.  The batched form's launches that follow
+    // are persistent grids draining those lists (no host round trip: the
+    // sequence replays as a HIP graph); the plain form reads the list lengths
+    // back and launches one workgroup per listed pair.
     const bool batched = o.batched != 0;
     bool listed = false;
     int nqueue = 0;
-    int* lista = batched ? c->s_lista.as<int>() : nullptr;
-    int* listb = batched ? c->s_listb.as<int>() : nullptr;
+    int* lista = c->s_lista.as<int>();
+    int* listb = c->s_listb.as<int>();
     // every live pair, when the one-wave DP list leaves some out
-    int* listc = batched && c->dpb.any_big() ? c->s_listc.as<int>() : nullptr;
-    int* ltot = batched ? c->s_ltot.as<int>() : nullptr;
+    int* listc = c->dpb.any_big() ? c->s_listc.as<int>() : nullptr;
+    int* ltot = c->s_ltot.as<int>();
     int* heads = batched ? c->s_heads.as<int>() : nullptr;
     const int pgrid_dp = c->ncu * 32, pgrid_rf = c->ncu * 4;
+    DpBuckets dpb = c->dpb;         // plain form: one-wave list narrowed to live pairs
+    RefSplit rsp = c->rsp;
     auto queue = [&](const int* items, const int* count) -> WorkQ {
-        if (!listed || nqueue >= GTA_MAX_QUEUES) return WorkQ{};
+        if (!batched || !listed || nqueue >= GTA_MAX_QUEUES) return WorkQ{};
         return WorkQ{items, count, heads + nqueue++};
     };
     auto relist = [&]() -> hipError_t {
         listed = true;
-        return launch_live_lists(S, c->bigmask, c->s_lcnt.as<int>(), ltot, lista, listb, listc, st);
+        hipError_t e = launch_live_lists(S, c->bigmask, c->s_lcnt.as<int>(), ltot, lista, listb, listc, st);
+        if (e != hipSuccess || batched) return e;
+        int n[3];
+        if ((e = hipMemcpyAsync(n, ltot, sizeof n, hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
+        if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+        dpb.small = lista; dpb.nsmall = n[0];
+        rsp.small = listb; rsp.nsmall = n[1];
+        return hipSuccess;
     };
     auto dp = [&](int mode, int gap0, float gapopen, int usess, int dpscan) -> hipError_t {
         DpLaunch L{mode, gap0, gapopen, usess, usess ? 0.5f : 0.0f, dpscan};
-        return launch_dp(Q, R, S, L, c->dpb, queue(lista, ltot), pgrid_dp, st);
+        return launch_dp(Q, R, S, L, dpb, queue(lista, ltot), pgrid_dp, st);
     };
     auto refine = [&](int mode, int epi, int dpi, int maxdpi, float prescore) -> hipError_t {
         RefineLaunch L = RL;
         L.epi = epi; L.dpi = dpi; L.maxndpiters = maxdpi; L.prescore = prescore;
-        return launch_refine_dp(prec, mode, Q, R, S, G, L, c->rsp, queue(listb, ltot + 1), pgrid_rf, st);
+        return launch_refine_dp(prec, mode, Q, R, S, G, L, rsp, queue(listb, ltot + 1), pgrid_rf, st);
     };
     // DPRefine<GAP0,PRESCREEN> (MpStage1.cpp:455-510)
     auto dprefine = [&](int gap0, float prescore, int maxdpi) -> int {
@@ -733,7 +747,7 @@ static int run_stage(gta_ctx* c, double* seconds, double* cells) {
         GTA_TIMED(KC_REFINE_GAPLESS, launch_refine_gapless(Q, R, S, G, dinv, c->rsp, st));
         int rc;
         if ((rc = dprefine(1, o.prescore, 2)) != GTA_OK) return rc;
-        if (batched) HIPCHK(c, relist());
+        HIPCHK(c, relist());
         // stage 2, pass 1: SS-weighted DP with gap cost (MpStage2.h:60-80)
         GTA_TIMED(KC_DP, dp(DPM_GRAND_L, 0, -1.0f, 1, 1));
         GTA_TIMED(KC_REFINE_DP, refine(RM_DPREFINE_L, 0, 0, 0, 0.f));
@@ -746,7 +760,7 @@ static int run_stage(gta_ctx* c, double* seconds, double* cells) {
         if ((rc = dprefine(1, 0.f, o.convergence)) != GTA_OK) return rc;
         if (0.0f < o.scorethld) {
             GTA_TIMED(KC_MISC, launch_setlow(Q, R, S, o.scorethld, st));
-            if (batched) HIPCHK(c, relist());
+            HIPCHK(c, relist());
         }
         // final stage (MpStageFin.h:66-90)
         GTA_TIMED(KC_DP, dp(DPM_GRAND_L, 1, 0.0f, 0, 0));

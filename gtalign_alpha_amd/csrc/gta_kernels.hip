@@ -306,6 +306,34 @@ __device__ __forceinline__ float gscore_gapless(const Coords& q, const Coords& r
     return s;
 }
 
+// gscore_gapless (no threshold update) fused with the next iteration's
+// thresholded covariance over the same positions under the same transform
+// (gcc_gapless with threshold thrcc): bitwise the results of the two passes.
+template <int P>
+__device__ __forceinline__ float gscore_cc_gapless(const Coords& q, const Coords& r, int qp, int rp, int n,
+                                                   const float* tfm, float d02, float d82, float thrcc,
+                                                   typename GPrec<P>::T* cc, int* ncc) {
+    GScore<P> sc;
+    sc.clear();
+    typename GPrec<P>::Acc acc;
+    acc.clear();
+    int cnt = 0;
+    for (int k = glane(); k < n; k += GL) {
+        const int a = qp + k, b = rp + k;
+        const float qx = q.x[a], qy = q.y[a], qz = q.z[a], rx = r.x[b], ry = r.y[b], rz = r.z[b];
+        const float d = tfm_dist2(tfm, qx, qy, qz, rx, ry, rz);
+        if (d <= d82) sc.add(pair_score(d02, d));
+        if (!(thrcc < d)) {
+            acc.add(qx, qy, qz, rx, ry, rz);
+            cnt++;
+        }
+    }
+    const float s = sc.reduce();
+    acc.reduce(cc);
+    *ncc = gsum(cnt);
+    return s;
+}
+
 // ============================================================================
 // stage 1: best gapless diagonal (seed) per pair (FindGaplessAlignedFragment,
 // MpStage1.cpp:267-400).  Slots are seed diagonals; each runs up to three
@@ -432,6 +460,7 @@ __global__ __launch_bounds__(RTHREADS) void k_refine_gapless(DevChunk Q, DevChun
     __shared__ SlotBest lds[NGROUPS];
     __shared__ float tbest[NGROUPS][12];
     __shared__ typename GPrec<P>::T tlast[NGROUPS][16];
+    __shared__ typename GPrec<P>::T tnext[NGROUPS][16];   // next iteration's covariance from a fused pass
     __shared__ int ctr;
     const int p = plist ? plist[blockIdx.x] : (int)blockIdx.x;
     const int qi = pair_q(S, p), ri = pair_r(S, p);
@@ -464,6 +493,8 @@ __global__ __launch_bounds__(RTHREADS) void k_refine_gapless(DevChunk Q, DevChun
     int si = 0, cit = 0, fqp = 0, frp = 0, fl = 0;
     float tfm[12], thr = CP_LARGEDST, best = 0.f;
     T* last = tlast[grp];       // previous covariance record (15 sums, count)
+    T* next = tnext[grp];       // covariance of the next iteration, when fused
+    bool havenext = false;
     for (;;) {
         for (;;) {
             const int got = fetch_slot(need, &ctr);
@@ -500,7 +531,15 @@ __global__ __launch_bounds__(RTHREADS) void k_refine_gapless(DevChunk Q, DevChun
                 ncc = (T)gcc_gapless<P>(q, r, fqp, frp, fl, tfm, false, 0.f, cc);
                 ab += AB_CC * (unsigned long long)fl;
             } else {
-                ncc = (T)gcc_gapless<P>(q, r, qpos, rpos, nall, tfm, true, (cit < 2) ? thr : d02s1, cc);
+                if (havenext) {
+                    // computed by the previous iteration's fused score pass
+#pragma unroll
+                    for (int f = 0; f < 15; f++) cc[f] = next[f];
+                    ncc = next[15];
+                    havenext = false;
+                } else {
+                    ncc = (T)gcc_gapless<P>(q, r, qpos, rpos, nall, tfm, true, (cit < 2) ? thr : d02s1, cc);
+                }
                 ab += AB_EXT * (unsigned long long)nall;
                 if (cc_converged<T>(cc, ncc, last, last[15])) done = true;
             }
@@ -511,7 +550,21 @@ __global__ __launch_bounds__(RTHREADS) void k_refine_gapless(DevChunk Q, DevChun
                     last[15] = ncc;
                 }
                 kabsch<T>(cc, ncc, dinv, tfm);
-                const float sc = gscore_gapless<P>(q, r, qpos, rpos, nall, tfm, d0, d02, d82, cit < 1, &thr);
+                float sc;
+                if (cit >= 1 && cit + 1 < NMAXCONVIT + 2) {
+                    // the next iteration's covariance takes the fixed threshold
+                    // d02s1 under this transform: one pass for both
+                    T cn[15];
+                    int nn;
+                    sc = gscore_cc_gapless<P>(q, r, qpos, rpos, nall, tfm, d02, d82, d02s1, cn, &nn);
+                    if (glane() == 0) {
+                        for (int f = 0; f < 15; f++) next[f] = cn[f];
+                        next[15] = (T)nn;
+                    }
+                    havenext = true;
+                } else {
+                    sc = gscore_gapless<P>(q, r, qpos, rpos, nall, tfm, d0, d02, d82, cit < 1, &thr);
+                }
                 ab += AB_TM * (unsigned long long)nall + AB_SLOT;
                 if (best < sc) {
                     best = sc;
@@ -1169,6 +1222,36 @@ __device__ __forceinline__ float gscore_dp(const Aln& a, int L, const float* tfm
     return s;
 }
 
+// TM-score pass (as gscore_dp, no threshold update) fused with the next
+// iteration's thresholded covariance pass under the same transform (as
+// gcc_dp_ext with threshold thrcc): the positions, distances, accumulation
+// order and reductions are those of the two separate passes, so both results
+// are bitwise the same.  Returns the score; cc and *ncc receive the
+// covariance and its count.
+template <int P>
+__device__ __forceinline__ float gscore_cc_dp(const Aln& a, int L, const float* tfm, float d02, float d82, bool chk,
+                                              float thrcc, typename GPrec<P>::T* cc, int* ncc) {
+    GScore<P> sc;
+    sc.clear();
+    typename GPrec<P>::Acc acc;
+    acc.clear();
+    int cnt = 0;
+    for (int f = glane(); f < L; f += GL) {
+        const int idx = L - 1 - f;
+        const float qx = a.qx[idx], qy = a.qy[idx], qz = a.qz[idx], rx = a.rx[idx], ry = a.ry[idx], rz = a.rz[idx];
+        const float d = tfm_dist2(tfm, qx, qy, qz, rx, ry, rz);
+        if (!chk || d <= d82) sc.add(pair_score(d02, d));
+        if (!(thrcc < d)) {
+            acc.add(qx, qy, qz, rx, ry, rz);
+            cnt++;
+        }
+    }
+    const float s = sc.reduce();
+    acc.reduce(cc);
+    *ncc = gsum(cnt);
+    return s;
+}
+
 // Superposition refinement over the DP alignment, one workgroup per pair
 // (FragmentBasedDPAlignmentRefinement, FinalFragmentBasedDPAlignmentRefinement
 // Phase1/2, ProductionFragmentBasedDPAlignmentRefinementPhase1/2:
@@ -1223,6 +1306,7 @@ struct RefineLds {
     SlotBest slds[NGROUPS];
     float tbest[NGROUPS][12];
     typename GPrec<P>::T tlast[NGROUPS][16];
+    typename GPrec<P>::T tnext[NGROUPS][16];   // next iteration's covariance from a fused pass
     int ctr;
 };
 
@@ -1233,6 +1317,7 @@ __device__ void refine_pair(const DevChunk& Q, const DevChunk& R, const DevState
     SlotBest* slds = rl.slds;
     float (*tbest)[12] = rl.tbest;
     typename GPrec<P>::T (*tlast)[16] = rl.tlast;
+    typename GPrec<P>::T (*tnext)[16] = rl.tnext;
     int& ctr = rl.ctr;
     typedef typename GPrec<P>::T T;
     const int qi = pair_q(S, p), ri = pair_r(S, p);
@@ -1291,6 +1376,8 @@ __device__ void refine_pair(const DevChunk& Q, const DevChunk& R, const DevState
     int si = 0, cit = 0, fl = 0, sfragndx = 0, sfragpos = 0;
     float tfm[12], thr = CP_LARGEDST, best = 0.f;
     T* last = tlast[grp];       // previous covariance record (15 sums, count)
+    T* next = tnext[grp];       // covariance of the next iteration, when fused
+    bool havenext = false;
     for (;;) {
         for (;;) {
             const int got = fetch_slot(need, &ctr);
@@ -1349,7 +1436,15 @@ __device__ void refine_pair(const DevChunk& Q, const DevChunk& R, const DevState
                 ncc = (T)fl;
                 ab += AB_CC * (unsigned long long)fl;
             } else {
-                ncc = (T)gcc_dp_ext<P>(a, L, tfm, (cit < 2) ? thr : d02s1, cc);
+                if (havenext) {
+                    // computed by the previous iteration's fused score pass
+#pragma unroll
+                    for (int f = 0; f < 15; f++) cc[f] = next[f];
+                    ncc = next[15];
+                    havenext = false;
+                } else {
+                    ncc = (T)gcc_dp_ext<P>(a, L, tfm, (cit < 2) ? thr : d02s1, cc);
+                }
                 ab += AB_EXT * (unsigned long long)L;
                 if (cc_converged<T>(cc, ncc, last, last[15])) done = true;
             }
@@ -1360,7 +1455,21 @@ __device__ void refine_pair(const DevChunk& Q, const DevChunk& R, const DevState
                     last[15] = ncc;
                 }
                 kabsch<T>(cc, ncc, dinv, tfm);
-                const float sc = gscore_dp<P>(a, L, tfm, d0, d02, d82, chk, cit < 1, &thr);
+                float sc;
+                if (cit >= 1 && cit + 1 < NMAXCONVIT + 2) {
+                    // the next iteration's covariance takes the fixed threshold
+                    // d02s1 under this transform: one pass for both
+                    T cn[15];
+                    int nn;
+                    sc = gscore_cc_dp<P>(a, L, tfm, d02, d82, chk, d02s1, cn, &nn);
+                    if (glane() == 0) {
+                        for (int f = 0; f < 15; f++) next[f] = cn[f];
+                        next[15] = (T)nn;
+                    }
+                    havenext = true;
+                } else {
+                    sc = gscore_dp<P>(a, L, tfm, d0, d02, d82, chk, cit < 1, &thr);
+                }
                 ab += AB_TM * (unsigned long long)L + AB_SLOT;
                 if (best < sc) {
                     best = sc;
