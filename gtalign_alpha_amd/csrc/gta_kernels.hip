@@ -173,7 +173,7 @@ __global__ void k_init(DevChunk Q, DevChunk R, DevState S) {
 }
 
 // ============================================================================
-// gapless passes over structure coordinates (stage 1), 16-lane slot groups
+// gapless passes over structure coordinates (stage 1), in slot groups
 struct Coords {
     const float* x;
     const float* y;

@@ -1,20 +1,24 @@
-// gta_slots.h -- 16-lane slot groups for the superposition-refinement kernels.
+// gta_slots.h -- slot groups of a few lanes for the superposition-refinement kernels.
 //
 // A refinement kernel works on many independent slots per pair (a seed
 // diagonal, or a fragment length and position); each slot iterates
 // covariance pass -> 3x3 Kabsch solve -> TM-score pass.  The solve is a long
-// serial chain, so one wave runs four slots at once, one per 16-lane DPP row:
-// the passes stride positions over the 16 lanes of the row, the row reduces
-// with four DPP steps (every lane ends with the same, bitwise identical sum),
-// and the four rows' solves run side by side in the same instructions.
+// serial chain computed once per group, so one wave runs 64/G slots at once,
+// one per group of G lanes (a DPP row or part of one): the passes stride
+// positions over the group's lanes, the group reduces with log2(G) DPP steps
+// (every lane ends with the same, bitwise identical sum), and the groups'
+// solves run side by side in the same instructions.
 #pragma once
 
 #include "gta_device.h"
 
 namespace gta {
 
+// 8 lanes by default: measured on the 16 x 50,000 workload, DPRefine passes
+// 0.897 s with 16 lanes, 0.840 s with 8, 1.108 s with 4 (gapless refinement
+// 0.195 / 0.185 / 0.246 s)
 #ifndef GTA_SLOT_LANES
-#define GTA_SLOT_LANES 16
+#define GTA_SLOT_LANES 8
 #endif
 constexpr int GL = GTA_SLOT_LANES;      // lanes per slot group: a DPP row (16) or half row (8)
 constexpr int NGW = GTA_WAVE / GL;      // slot groups per wave
