@@ -146,7 +146,8 @@ class GtaBatch:
     def dp_under(self, tfms, gap0=True, gapopen=0.0, usess=False, dpscan=False):
         """The DP kernel alone under the given per-pair transforms (testing hook,
         include/gta/gtalign_amd_testing.h): returns per pair the matched
-        (query, reference) positions in backtrace walk order."""
+        (query, reference) positions in backtrace walk order.  usess=2 runs the
+        SS/residue fill of the --add-search-by-ss search instead."""
         nq, nr = self._q.nstrs, self._r.nstrs
         t = np.ascontiguousarray(tfms, np.float32).reshape(nq * nr * GTA_TFM_NFIELDS)
         n = np.zeros(nq * nr, np.int32)

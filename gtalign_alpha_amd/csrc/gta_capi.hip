@@ -153,6 +153,7 @@ void gta_default_options(gta_options* o) {
     o->referenced = 0;
     o->precision = GTA_PREC_FP32;
     o->batched = 0;
+    o->addsearchbyss = 0;
 }
 
 size_t gta_alignment_section_size(int nq, int nqposs, int nr, int nrposs) {
@@ -171,6 +172,7 @@ static int check_options(const gta_options* o) {
     if (o->refinement < 0 || o->refinement > GTA_REF_FULLA) return 0;
     if (o->precision != GTA_PREC_FP32 && o->precision != GTA_PREC_MIXED) return 0;
     if (o->batched != 0 && o->batched != 1) return 0;
+    if (o->addsearchbyss != 0 && o->addsearchbyss != 1) return 0;
     return 1;
 }
 
@@ -748,6 +750,16 @@ static int run_stage(gta_ctx* c, double* seconds, double* cells) {
         int rc;
         if ((rc = dprefine(1, o.prescore, 2)) != GTA_OK) return rc;
         HIPCHK(c, relist());
+        if (o.addsearchbyss) {
+            // superposition search from the alignment of secondary structure
+            // and residue similarity (MpStageSSRR.h:62-76, MpBatch.cpp:421-436):
+            // gap open -1, SS match weight 1, residue score weight 0.2
+            const DpLaunch L{DPM_IBEST_L, 0, -1.0f, 2, 1.0f, 0, 0.2f};
+            GTA_TIMED(KC_DP, launch_dp(Q, R, S, L, dpb, queue(lista, ltot), pgrid_dp, st));
+            GTA_TIMED(KC_REFINE_DP, refine(RM_DPREFINE_L, 0, 0, 0, 0.f));
+            if ((rc = dprefine(0, 0.f, 2)) != GTA_OK) return rc;
+            if ((rc = dprefine(1, 0.f, 2)) != GTA_OK) return rc;
+        }
         // stage 2, pass 1: SS-weighted DP with gap cost (MpStage2.h:60-80)
         GTA_TIMED(KC_DP, dp(DPM_GRAND_L, 0, -1.0f, 1, 1));
         GTA_TIMED(KC_REFINE_DP, refine(RM_DPREFINE_L, 0, 0, 0, 0.f));
@@ -873,7 +885,10 @@ int gta_test_dp(gta_ctx* c, const float* tfms, int gap0, float gapopen, int uses
     HIPCHK(c, launch_secstr(R, st));
     HIPCHK(c, launch_init(Q, R, S, st));
     HIPCHK(c, hipMemcpyAsync(S.tfm_grand, tfms, np * 12 * 4, hipMemcpyHostToDevice, st));
-    DpLaunch L{DPM_GRAND_L, gap0, gapopen, usess, usess ? 0.5f : 0.0f, dpscan};
+    if (usess < 0 || usess > 2) return fail(c, GTA_ERR_ARG, "usess must be 0, 1 or 2");
+    // kind 2: the SS/residue fill with the weights of the --add-search-by-ss search
+    const DpLaunch L{DPM_GRAND_L, gap0, gapopen, usess, usess == 2 ? 1.0f : usess ? 0.5f : 0.0f, dpscan,
+                     usess == 2 ? 0.2f : 0.0f};
     HIPCHK(c, launch_fill_i32(c->dpb.failflag, 0, 1, st));
     HIPCHK(c, launch_dp(Q, R, S, L, c->dpb, WorkQ{}, 0, st));
     HIPCHK(c, hipStreamSynchronize(st));

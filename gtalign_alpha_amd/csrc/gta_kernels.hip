@@ -30,6 +30,7 @@
 #include <vector>
 
 #include "gta_device.h"
+#include "gta_gonnet.h"
 #include "gta_kabsch.h"
 #include "gta_slots.h"
 
@@ -608,10 +609,39 @@ struct DpParams {
     int mode;
     int gap0;
     float gapopen;
-    int usess;
-    float ssw;
+    int usess;          // score kind (SC_*)
+    float ssw;          // weight of a secondary-structure match
     int dpscan;
+    float rrw;          // weight of the residue score (SC_SSRR)
 };
+
+// Cell score kinds of a DP fill: the TM-score-like distance score under the
+// pair's superposition (SC_TM), with a bonus for matching secondary
+// structure (SC_TMSS), or, with no superposition, secondary-structure match
+// plus weighted residue similarity (SC_SSRR: ExecDPSSwBtck128xKernel with
+// sequence scoring, MpDPHub.cpp:985-1230).
+enum { SC_TM = 0, SC_TMSS = 1, SC_SSRR = 2 };
+
+// SC_SSRR keeps gap extension free by storing the value of a cell reached
+// diagonally shifted by DPSS_INIT (DPSSDEFINITSCOREVAL, mpdpbase.h:16): a
+// neighbour below DPSS_INIT_cmp was reached diagonally, and leaving it
+// sideways opens a gap.  Cells outside the matrix hold DPSS_INIT.
+constexpr float DPSS_INIT = -1048576.0f;
+constexpr float DPSS_INIT_cmp = -524288.0f;
+
+// Residue scores of SC_SSRR in bits (_GONNET_SCORES_, alpha.h:36-60), with
+// residue codes 'A'..'Z' -> 0..25 and any other character -> 26 (a zero row
+// and column).  Uploaded from the host table with the length tables.
+constexpr int NGT = 27;
+__device__ float g_gonnet[NGT * NGT];
+
+__device__ __forceinline__ int res_code(char c) { return ('A' <= c && c <= 'Z') ? c - 'A' : NGT - 1; }
+
+// the residue scores times the weight, into LDS (the fills read them per cell)
+__device__ __forceinline__ void load_gtab(float* gtab, float rrw) {
+    for (int k = threadIdx.x; k < NGT * NGT; k += blockDim.x) gtab[k] = __fmul_rn(g_gonnet[k], rrw);
+    __syncthreads();
+}
 
 // Backtrace word of lane l in block (s, tb): the codes of row 64s+l at the
 // strip's steps 16tb..16tb+15, step t at bits 2(t&15); cell (i, j) is row i
@@ -697,21 +727,28 @@ constexpr int DPRING = 128;
 // previous strip) comes in the same blocks and reaches lane 0 with the
 // up-shift of the values; lane 63's values leave through a down-shift
 // conveyor into rowout, stored and published a block at a time.  No divergent
-// branches in the step loop.  GAP0 fixes the gap mode at compile time;
-// without BTCK no backtrace is written (score-only fills).  vend, when given,
-// receives each lane's value at the strip's last column (row 64s+l).
-// Returns false when a row wait gave up (Sync::wait).
-template <bool USESS, bool GAP0, bool BTCK = true, class Sync>
+// branches in the step loop.  SC is the cell score kind (SC_*); GAP0 fixes
+// the gap mode at compile time; without BTCK no backtrace is written
+// (score-only fills).  vend, when given, receives each lane's value at the
+// strip's last column (row 64s+l).  SC_SSRR ring entries hold {residue code
+// (int bits), SS letter, 0, row-above value} and read the weighted residue
+// scores from gtab.  Returns false when a row wait gave up (Sync::wait).
+template <int SC, bool GAP0, bool BTCK = true, class Sync>
 __device__ __forceinline__ bool dp_strip(const DevChunk& Q, const DevChunk& R, const DevState& S,
                                          const DpParams& prm, const DpPair& P, int s, const float* rowin,
                                          float* rowout, unsigned* bt, float4* ring, float* ssring, Sync& sync,
-                                         float* vend = nullptr) {
+                                         float* vend = nullptr, const float* gtab = nullptr) {
+    constexpr bool USESS = SC == SC_TMSS;
+    constexpr bool SSRR = SC == SC_SSRR;
+    // value of the cells outside the matrix
+    constexpr float OUT = SSRR ? DPSS_INIT : 0.0f;
     const int lane = lane_id();
     const int Lq = P.Lq, Lr = P.Lr;
     const float* Rx = R.x + P.rd;
     const float* Ry = R.y + P.rd;
     const float* Rz = R.z + P.rd;
     const char* Rs = R.ss + P.rd;
+    const char* Rr = R.rsd + P.rd;
     const float gapopen = prm.gapopen, ssw = prm.ssw, d02 = P.d02;
     const int nsteps = Lr + 63;
     const int nblkw = (nsteps + 15) >> 4;
@@ -719,22 +756,32 @@ __device__ __forceinline__ bool dp_strip(const DevChunk& Q, const DevChunk& R, c
     const bool rowok = i < Lq;
     float qx = 0.f, qy = 0.f, qz = 0.f;
     float qss = 0.f;
+    int qbase = 0;      // SC_SSRR: the lane's row of gtab
     if (rowok) {
-        qx = Q.x[P.qd + i]; qy = Q.y[P.qd + i]; qz = Q.z[P.qd + i];
-        tfm_point(P.tfm, qx, qy, qz);
-        if (USESS) qss = (float)Q.ss[P.qd + i];
+        if (SSRR) {
+            qss = (float)Q.ss[P.qd + i];
+            qbase = res_code(Q.rsd[P.qd + i]) * NGT;
+        } else {
+            qx = Q.x[P.qd + i]; qy = Q.y[P.qd + i]; qz = Q.z[P.qd + i];
+            tfm_point(P.tfm, qx, qy, qz);
+            if (USESS) qss = (float)Q.ss[P.qd + i];
+        }
     }
     const bool qok = rowok && qx < DEFCOORD_QRY_cmp;
     const bool haveup = rowin != nullptr;
     const bool putrow = rowout != nullptr;
     if (haveup && !sync.wait(s - 1, min(128, Lr))) return false;
-    // block 0 at entries 0..63, columns -64..-1 (entries 64..127) zero
+    // block 0 at entries 0..63, columns -64..-1 (entries 64..127) outside
     {
         const int c = lane;
         const bool ok = c < Lr;
-        ring[lane] = make_float4(ok ? Rx[c] : 0.f, ok ? Ry[c] : 0.f, ok ? Rz[c] : 0.f,
-                                 (ok && haveup) ? rowin[c] : 0.f);
-        ring[64 + lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (SSRR)
+            ring[lane] = make_float4(__int_as_float(ok ? res_code(Rr[c]) : 0), ok ? (float)Rs[c] : 0.f, 0.f,
+                                     (ok && haveup) ? rowin[c] : OUT);
+        else
+            ring[lane] = make_float4(ok ? Rx[c] : 0.f, ok ? Ry[c] : 0.f, ok ? Rz[c] : 0.f,
+                                     (ok && haveup) ? rowin[c] : OUT);
+        ring[64 + lane] = make_float4(0.f, 0.f, 0.f, OUT);
         if (USESS) {
             ssring[lane] = ok ? (float)Rs[c] : 0.f;
             ssring[64 + lane] = 0.f;
@@ -745,14 +792,18 @@ __device__ __forceinline__ bool dp_strip(const DevChunk& Q, const DevChunk& R, c
     {
         const int c2 = 64 + lane;
         const bool ok2 = c2 < Lr;
-        nbx = ok2 ? Rx[c2] : 0.f; nby = ok2 ? Ry[c2] : 0.f; nbz = ok2 ? Rz[c2] : 0.f;
+        if (SSRR) {
+            nbx = __int_as_float(ok2 ? res_code(Rr[c2]) : 0); nby = ok2 ? (float)Rs[c2] : 0.f; nbz = 0.f;
+        } else {
+            nbx = ok2 ? Rx[c2] : 0.f; nby = ok2 ? Ry[c2] : 0.f; nbz = ok2 ? Rz[c2] : 0.f;
+        }
         if (USESS) nbs = ok2 ? (float)Rs[c2] : 0.f;
-        nbu = (ok2 && haveup) ? rowin[c2] : 0.f;
+        nbu = (ok2 && haveup) ? rowin[c2] : OUT;
     }
     int ao = ((-lane) & (DPRING - 1)) << 4;  // byte offset of column t - lane's ring entry
     float4 cur = *(const float4*)((const char*)ring + ao);
     float curs = USESS ? ssring[ao >> 4] : 0.f;
-    float val = 0.f, upprev = 0.f, oblk = 0.f;
+    float val = OUT, upprev = OUT, oblk = 0.f;
     unsigned word = 0;
     int j = -lane;
     for (int t = 0; t < nsteps; t++) {
@@ -765,12 +816,16 @@ __device__ __forceinline__ bool dp_strip(const DevChunk& Q, const DevChunk& R, c
             if (USESS) ssring[e] = nbs;
             const int c2 = t + 65 + lane;
             const bool ok2 = c2 < Lr;
-            nbx = ok2 ? Rx[c2] : 0.f; nby = ok2 ? Ry[c2] : 0.f; nbz = ok2 ? Rz[c2] : 0.f;
+            if (SSRR) {
+                nbx = __int_as_float(ok2 ? res_code(Rr[c2]) : 0); nby = ok2 ? (float)Rs[c2] : 0.f;
+            } else {
+                nbx = ok2 ? Rx[c2] : 0.f; nby = ok2 ? Ry[c2] : 0.f; nbz = ok2 ? Rz[c2] : 0.f;
+            }
             if (USESS) nbs = ok2 ? (float)Rs[c2] : 0.f;
             if (haveup && t + 65 < Lr) {
                 if (!sync.wait(s - 1, min(t + 129, Lr))) return false;
             }
-            nbu = (ok2 && haveup) ? rowin[c2] : 0.f;
+            nbu = (ok2 && haveup) ? rowin[c2] : OUT;
         }
         ao = (ao + 16) & (DPRING * 16 - 1);
         cur = *(const float4*)((const char*)ring + ao);
@@ -780,18 +835,36 @@ __device__ __forceinline__ bool dp_strip(const DevChunk& Q, const DevChunk& R, c
         const float diag = upprev;
         upprev = up;
         const bool valid = rowok && (unsigned)j < (unsigned)Lr;
-        float sc = pair_score(d02, dist2(qx, qy, qz, rc.x, rc.y, rc.z));
-        sc = (qok && DEFCOORD_RFN_cmp < rc.x) ? sc : 0.0f;
-        if (USESS) sc = __fadd_rn(sc, (qss == rss) ? ssw : 0.0f);
-        float v = __fadd_rn(sc, GAP0 ? diag : fabsf(diag));
+        float v, nv;
         unsigned bk = BT_DIAG;
-        float w = val;
-        if (!GAP0 && w < 0.0f) w = __fsub_rn(gapopen, w);
-        if (v < w) { v = w; bk = BT_LEFT; }
-        w = up;
-        if (!GAP0 && w < 0.0f) w = __fsub_rn(gapopen, w);
-        if (v < w) { v = w; bk = BT_UP; }
-        const float nv = (GAP0 || bk != BT_DIAG) ? v : -v;
+        if (SSRR) {
+            // (float)(SS match) * wss + residue score * wrr, then the shifted
+            // recurrence (MpDPHub.cpp:1150-1185)
+            float sc = __fmul_rn(qss == rc.y ? 1.0f : 0.0f, ssw);
+            sc = __fadd_rn(sc, gtab[qbase + __float_as_int(rc.x)]);
+            float d = diag;
+            if (d < DPSS_INIT_cmp) d = __fsub_rn(d, DPSS_INIT);
+            v = __fadd_rn(sc, d);
+            float w = val;
+            if (w < DPSS_INIT_cmp) w = __fsub_rn(__fadd_rn(gapopen, w), DPSS_INIT);
+            if (v < w) { v = w; bk = BT_LEFT; }
+            w = up;
+            if (w < DPSS_INIT_cmp) w = __fsub_rn(__fadd_rn(gapopen, w), DPSS_INIT);
+            if (v < w) { v = w; bk = BT_UP; }
+            nv = (bk != BT_DIAG) ? v : __fadd_rn(v, DPSS_INIT);
+        } else {
+            float sc = pair_score(d02, dist2(qx, qy, qz, rc.x, rc.y, rc.z));
+            sc = (qok && DEFCOORD_RFN_cmp < rc.x) ? sc : 0.0f;
+            if (USESS) sc = __fadd_rn(sc, (qss == rss) ? ssw : 0.0f);
+            v = __fadd_rn(sc, GAP0 ? diag : fabsf(diag));
+            float w = val;
+            if (!GAP0 && w < 0.0f) w = __fsub_rn(gapopen, w);
+            if (v < w) { v = w; bk = BT_LEFT; }
+            w = up;
+            if (!GAP0 && w < 0.0f) w = __fsub_rn(gapopen, w);
+            if (v < w) { v = w; bk = BT_UP; }
+            nv = (GAP0 || bk != BT_DIAG) ? v : -v;
+        }
         // cells left of the strip's columns keep 0, cells past them the last value
         val = valid ? nv : val;
         const unsigned code = valid ? bk : (unsigned)BT_STOP;
@@ -897,9 +970,9 @@ __device__ __forceinline__ void dp_count(const DevState& S, const DpPair& P) {
 // DP fill with backtrace and the walk, one wave per pair (plist, when given,
 // maps blocks to pairs).  The strips run in order; each reads the row the
 // previous one wrote to the pair's row in HBM.
-template <bool USESS, bool GAP0>
+template <int SC, bool GAP0>
 __device__ void dp_pair(const DevChunk& Q, const DevChunk& R, const DevState& S, const DpParams& prm, int p,
-                        float4* ring, float* ssring) {
+                        float4* ring, float* ssring, const float* gtab) {
     DpPair P;
     if (!dp_setup(Q, R, S, prm, p, P)) return;
     float* row = S.rowbuf + (size_t)P.qi * (size_t)R.nposs + (size_t)P.rd;
@@ -907,8 +980,8 @@ __device__ void dp_pair(const DevChunk& Q, const DevChunk& R, const DevState& S,
     const int nstrips = (P.Lq + 63) >> 6;
     NoSync ns;
     for (int s = 0; s < nstrips; s++) {
-        dp_strip<USESS, GAP0>(Q, R, S, prm, P, s, s > 0 ? row : nullptr, s + 1 < nstrips ? row : nullptr, bt,
-                              ring, ssring, ns);
+        dp_strip<SC, GAP0>(Q, R, S, prm, P, s, s > 0 ? row : nullptr, s + 1 < nstrips ? row : nullptr, bt, ring,
+                           ssring, ns, nullptr, gtab);
         // the next strip reads this strip's row back
         __threadfence_block();
     }
@@ -920,17 +993,19 @@ __device__ void dp_pair(const DevChunk& Q, const DevChunk& R, const DevState& S,
 // One wave per pair: block b takes pair plist[b] (b without a list), or, with
 // a work queue, the waves of a persistent grid take queued pairs until the
 // queue is drained.
-template <bool USESS, bool GAP0>
+template <int SC, bool GAP0>
 __global__ __launch_bounds__(GTA_WAVE) void k_dp(DevChunk Q, DevChunk R, DevState S, DpParams prm,
                                                   const int* plist, WorkQ wq) {
     __shared__ float4 ring[DPRING];
-    __shared__ float ssring[USESS ? DPRING : 1];
+    __shared__ float ssring[SC == SC_TMSS ? DPRING : 1];
+    __shared__ float gtab[SC == SC_SSRR ? NGT * NGT : 1];
+    if (SC == SC_SSRR) load_gtab(gtab, prm.rrw);
     if (wq.items) {
         const int n = *wq.count;
         for (int it = wq_next_wave(wq); it < n; it = wq_next_wave(wq))
-            dp_pair<USESS, GAP0>(Q, R, S, prm, wq.items[it], ring, ssring);
+            dp_pair<SC, GAP0>(Q, R, S, prm, wq.items[it], ring, ssring, gtab);
     } else {
-        dp_pair<USESS, GAP0>(Q, R, S, prm, plist ? plist[blockIdx.x] : (int)blockIdx.x, ring, ssring);
+        dp_pair<SC, GAP0>(Q, R, S, prm, plist ? plist[blockIdx.x] : (int)blockIdx.x, ring, ssring, gtab);
     }
 }
 
@@ -962,7 +1037,7 @@ __global__ __launch_bounds__(GTA_WAVE) void k_dp_score(DevChunk Q, DevChunk R, D
     P.flags = __builtin_amdgcn_readfirstlane(S.flags[p]);
     const float d0 = GetD0(P.Lq, P.Lr);
     P.d02 = d0 * d0;
-    const DpParams prm{DPM_GRAND, GAP0 ? 1 : 0, gapopen, 0, 0.f, 0};
+    const DpParams prm{DPM_GRAND, GAP0 ? 1 : 0, gapopen, SC_TM, 0.f, 0, 0.f};
     float* row = S.rowbuf + (size_t)P.qi * (size_t)R.nposs + (size_t)P.rd;
     const int nstrips = (P.Lq + 63) >> 6;
     const int lastlane = (P.Lq - 1) & 63;
@@ -977,7 +1052,7 @@ __global__ __launch_bounds__(GTA_WAVE) void k_dp_score(DevChunk Q, DevChunk R, D
         for (int i = 0; i < 12; i++) P.tfm[i] = tfms[o * 12 + i];
         float vend = 0.f;
         for (int s = 0; s < nstrips; s++) {
-            dp_strip<false, GAP0, false>(Q, R, S, prm, P, s, s > 0 ? row : nullptr, s + 1 < nstrips ? row : nullptr,
+            dp_strip<SC_TM, GAP0, false>(Q, R, S, prm, P, s, s > 0 ? row : nullptr, s + 1 < nstrips ? row : nullptr,
                                          nullptr, ring, ssring, ns, &vend);
             __threadfence_block();
         }
@@ -1027,13 +1102,16 @@ struct LdsSync {
 // DP of one large pair per workgroup of W waves (pairs from plist; ring rows
 // of W x Lr floats per pair at ringoff[item]).  Same cells, codes and walk as
 // k_dp; only the strips run concurrently, pipelined down the query.
-template <bool USESS, bool GAP0, int W>
+template <int SC, bool GAP0, int W>
 __global__ __launch_bounds__(W * GTA_WAVE) void k_dp_mw(DevChunk Q, DevChunk R, DevState S, DpParams prm,
                                                          const int* plist, float* ring,
                                                          const unsigned long long* ringoff, int* failflag) {
+    constexpr bool USESS = SC == SC_TMSS;
     __shared__ int strip[W], done[W], fail;
     __shared__ float4 cring[W][DPRING];
     __shared__ float ssring[USESS ? W : 1][USESS ? DPRING : 1];
+    __shared__ float gtab[SC == SC_SSRR ? NGT * NGT : 1];
+    if (SC == SC_SSRR) load_gtab(gtab, prm.rrw);
     DpPair P;
     if (!dp_setup(Q, R, S, prm, plist[blockIdx.x], P)) return;
     const int w = threadIdx.x / GTA_WAVE;
@@ -1048,7 +1126,9 @@ __global__ __launch_bounds__(W * GTA_WAVE) void k_dp_mw(DevChunk Q, DevChunk R, 
         sy.begin(s);
         const float* rin = s > 0 ? rows + (size_t)((s - 1) % W) * P.Lr : nullptr;
         float* rout = s + 1 < nstrips ? rows + (size_t)(s % W) * P.Lr : nullptr;
-        if (!dp_strip<USESS, GAP0>(Q, R, S, prm, P, s, rin, rout, bt, cring[w], ssring[USESS ? w : 0], sy)) break;
+        if (!dp_strip<SC, GAP0>(Q, R, S, prm, P, s, rin, rout, bt, cring[w], ssring[USESS ? w : 0], sy, nullptr,
+                                gtab))
+            break;
         // the strip's row is complete
         sy.publish(s, P.Lr);
     }
@@ -1095,13 +1175,15 @@ struct GridSync {
 // strip before it, which a running workgroup already holds, so the lowest
 // unfinished strip never waits and the fill always drains.  Same cells and
 // codes as k_dp; k_dp_grid_walk then walks each pair.
-template <bool USESS, bool GAP0>
+template <int SC, bool GAP0>
 __global__ __launch_bounds__(GTA_WAVE) void k_dp_grid(DevChunk Q, DevChunk R, DevState S, DpParams prm,
                                                        const int* plist, int npairs, const int* strip0,
                                                        int* sync, float* rows, const unsigned long long* rowoff,
                                                        int* failflag) {
     __shared__ float4 cring[DPRING];
-    __shared__ float ssring[USESS ? DPRING : 1];
+    __shared__ float ssring[SC == SC_TMSS ? DPRING : 1];
+    __shared__ float gtab[SC == SC_SSRR ? NGT * NGT : 1];
+    if (SC == SC_SSRR) load_gtab(gtab, prm.rrw);
     const int total = strip0[npairs];
     int* next = sync + total;
     for (;;) {
@@ -1123,7 +1205,7 @@ __global__ __launch_bounds__(GTA_WAVE) void k_dp_grid(DevChunk Q, DevChunk R, De
         GridSync sy{sync + strip0[k], failflag};
         const float* rin = s > 0 ? prow + (size_t)(s - 1) * P.Lr : nullptr;
         float* rout = s + 1 < nstrips ? prow + (size_t)s * P.Lr : nullptr;
-        if (!dp_strip<USESS, GAP0>(Q, R, S, prm, P, s, rin, rout, bt, cring, ssring, sy)) break;
+        if (!dp_strip<SC, GAP0>(Q, R, S, prm, P, s, rin, rout, bt, cring, ssring, sy, nullptr, gtab)) break;
         sy.publish(s, P.Lr);
     }
 }
@@ -1881,24 +1963,42 @@ struct LengthTables {
 };
 }  // namespace
 
+// the residue score table in the device layout (codes 0..25 for 'A'..'Z', 26
+// for any other character)
+struct ResidueTable {
+    float g[NGT * NGT];
+    ResidueTable() {
+        float bits[GONNET_NA * GONNET_NA];
+        gonnet_bits(bits);
+        for (int a = 0; a < NGT; a++)
+            for (int b = 0; b < NGT; b++)
+                g[a * NGT + b] = (a < GONNET_NA && b < GONNET_NA) ? bits[a * GONNET_NA + b] : 0.0f;
+    }
+};
+
 hipError_t upload_length_tables() {
     static const LengthTables t;              // built once (thread-safe static)
+    static const ResidueTable rt;
     hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(g_cbrt_lm15), t.cb.data(), t.cb.size() * sizeof(float));
     if (e != hipSuccess) return e;
+    if ((e = hipMemcpyToSymbol(HIP_SYMBOL(g_gonnet), rt.g, sizeof rt.g)) != hipSuccess) return e;
     return hipMemcpyToSymbol(HIP_SYMBOL(g_pow03), t.pw.data(), t.pw.size() * sizeof(float));
 }
 
 hipError_t launch_dp(const DevChunk& Q, const DevChunk& R, const DevState& S, const DpLaunch& L,
                      const DpBuckets& B, const WorkQ& wq, int pgrid, hipStream_t st) {
-    DpParams prm{L.mode, L.gap0, L.gapopen, L.usess, L.ssw, L.dpscan};
-    // kernels specialised on (secondary-structure bonus, gap mode)
-#define GTA_DPV(KCALL)                                   \
-    do {                                                 \
-        if (L.usess) {                                   \
-            if (L.gap0) KCALL(true, true); else KCALL(true, false);   \
-        } else {                                         \
-            if (L.gap0) KCALL(false, true); else KCALL(false, false); \
-        }                                                \
+    DpParams prm{L.mode, L.gap0, L.gapopen, L.usess, L.ssw, L.dpscan, L.rrw};
+    // kernels specialised on (score kind, gap mode); the SS/residue fill
+    // always has a gap cost
+#define GTA_DPV(KCALL)                                                      \
+    do {                                                                    \
+        if (L.usess == SC_SSRR) {                                           \
+            KCALL(SC_SSRR, false);                                          \
+        } else if (L.usess == SC_TMSS) {                                    \
+            if (L.gap0) KCALL(SC_TMSS, true); else KCALL(SC_TMSS, false);   \
+        } else {                                                            \
+            if (L.gap0) KCALL(SC_TM, true); else KCALL(SC_TM, false);       \
+        }                                                                   \
     } while (0)
     const int nsmall = wq.items ? pgrid : (B.small ? B.nsmall : S.nq * S.nr);
     if (nsmall > 0) {
@@ -1920,7 +2020,14 @@ hipError_t launch_dp(const DevChunk& Q, const DevChunk& R, const DevState& S, co
 #undef GTA_KMW8
 #undef GTA_KMW16
 #undef GTA_MW
-    if (B.ngrid > 0) {
+    if (B.ngrid > 0 && L.usess == SC_SSRR) {
+        // The SS/residue fill does not use the device-wide pipeline: its
+        // grid-form run stalled on the GPU with no cause found yet.  Those
+        // pairs run as 4-wave workgroups instead, on their grid rows (a pair
+        // holds nstrips >= min(4, nstrips) rows of Lr floats there).
+        hipLaunchKernelGGL((k_dp_mw<SC_SSRR, false, 4>), dim3(B.ngrid), dim3(4 * GTA_WAVE), 0, st, Q, R, S, prm,
+                           B.grid, B.ring, B.growoff, B.failflag);
+    } else if (B.ngrid > 0) {
         // progress counters and the item counter start at zero for every fill
         hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for((long long)B.gtotal + 1, 256)), dim3(256), 0, st, B.gsync, 0,
                            (size_t)B.gtotal + 1);

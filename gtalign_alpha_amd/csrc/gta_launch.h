@@ -13,9 +13,10 @@ struct DpLaunch {
     int mode;          // DPM_*_L
     int gap0;
     float gapopen;
-    int usess;
+    int usess;         // cell score: 0 distance, 1 distance + SS bonus, 2 SS match + residue similarity
     float ssw;
     int dpscan;
+    float rrw;         // residue-score weight of kind 2
 };
 
 // Pairs of a chunk split by DP size: the one-wave kernel takes `small`

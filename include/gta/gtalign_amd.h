@@ -94,6 +94,9 @@ typedef struct gta_options {
                                 1 batched (after each filter the live pairs are listed on the device
                                 and persistent grids drain those lists; the launch sequence is
                                 captured once as a HIP graph and replayed) -- default 0 */
+    int addsearchbyss;       /* --add-search-by-ss (default 0): after stage 1, an extra superposition
+                                search from an alignment of secondary structure and residue similarity
+                                (MpStageSSRR.h:62-76) */
 } gta_options;
 
 /* results of one gta_process call; host memory owned by the context and

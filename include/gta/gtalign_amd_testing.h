@@ -17,8 +17,12 @@ extern "C" {
 /* DP fill + backtrace (the kernel behind ExecDPTFMSSwBtck128xKernel and
  * BtckToMatched128xKernel, MpDPHub.cpp:687-984,1500-1610) for every pair of
  * the resident chunks under the caller's per-pair transforms tfms[pair*12].
- * d0 is the search d0 (dpscan=0) or the DP-scan d0 (dpscan=1); usess adds
- * 0.5 for equal secondary-structure letters.  nmatched[pair] receives the
+ * d0 is the search d0 (dpscan=0) or the DP-scan d0 (dpscan=1); usess=1 adds
+ * 0.5 for equal secondary-structure letters.  usess=2 runs instead the fill
+ * of the --add-search-by-ss search (ExecDPSSwBtck128xKernel<true>,
+ * MpDPHub.cpp:985-1230, as MpStageSSRR::Align calls it): score 1 for equal
+ * SS letters plus 0.2 x the residue score, gap open gapopen, no transform
+ * used, gap0 ignored.  nmatched[pair] receives the
  * number of matched pairs; qpos/rpos receive the matched positions in walk
  * order at offset q*nrefposs + dst[r] for pair (q, r) (arrays of
  * nqrys*nrefposs ints). */

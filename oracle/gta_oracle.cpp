@@ -36,6 +36,7 @@
 #include <string>
 
 #include "gta/gtalign_amd.h"
+#include "gta_gonnet.h"
 
 namespace {
 
@@ -1011,6 +1012,65 @@ void dp_refine(Ctx& c, bool gap0, bool prescreen, int maxndpiters, float prescor
     s.flags &= ~CONV_SCOREDP;   // InitScoresKernel<INITOPT_CONVFLAG_SCOREDP>
 }
 
+// ---- search by secondary structure and residue similarity -------------------
+// (MpStageSSRR.h:62-112; --add-search-by-ss)
+constexpr float DPSS_INIT = -1048576.0f;      // DPSSDEFINITSCOREVAL (mpdpbase.h:16)
+constexpr float DPSS_INIT_cmp = -524288.0f;   // DPSSDEFINITSCOREVAL_cmp (mpdpbase.h:18)
+
+// pairwise residue score in bits (_GONNET_SCORES_::get, alpha.h:54-59)
+float gonnet_get(char a, char b) {
+    static float bits[gta::GONNET_NA * gta::GONNET_NA];
+    static bool ready = false;
+    if (!ready) { gta::gonnet_bits(bits); ready = true; }
+    if ('A' <= a && a <= 'Z' && 'A' <= b && b <= 'Z') return bits[(a - 'A') * gta::GONNET_NA + (b - 'A')];
+    return 0.0f;
+}
+
+// ExecDPSSwBtck128xKernel<USESEQSCORING=true> (MpDPHub.cpp:985-1230): the score
+// of a cell is the SS match weight plus the scaled Gonnet score; diagonal
+// results are stored shifted by DPSS_INIT so that a gap opened from them pays
+// gapopen while a continued gap (value above DPSS_INIT_cmp) does not; cells
+// outside the matrix hold DPSS_INIT (a shifted 0).
+void dp_fill_ssrr(const Str& q, const Str& r, float gapopen, float wss, float wrr, std::vector<uint8_t>& bt,
+                  double& cells) {
+    const int Lq = q.len, Lr = r.len;
+    bt.assign((size_t)Lq * Lr, 0);
+    std::vector<float> prev(Lr + 1, DPSS_INIT), cur(Lr + 1, DPSS_INIT);
+    for (int i = 0; i < Lq; i++) {
+        cur[0] = DPSS_INIT;
+        for (int j = 0; j < Lr; j++) {
+            float v = (float)(q.ss[i] == r.ss[j]) * wss;
+            v += gonnet_get(q.rsd[i], r.rsd[j]) * wrr;
+            float w = prev[j];                       // diagonal
+            if (w < DPSS_INIT_cmp) w -= DPSS_INIT;
+            v += w;
+            int bk = BT_DIAG;
+            w = cur[j];                              // left
+            if (w < DPSS_INIT_cmp) w = gapopen + w - DPSS_INIT;
+            if (v < w) { v = w; bk = BT_LEFT; }
+            w = prev[j + 1];                         // up
+            if (w < DPSS_INIT_cmp) w = gapopen + w - DPSS_INIT;
+            if (v < w) { v = w; bk = BT_UP; }
+            cur[j + 1] = (bk != BT_DIAG) ? v : v + DPSS_INIT;
+            bt[(size_t)i * Lr + j] = (uint8_t)bk;
+        }
+        std::swap(prev, cur);
+    }
+    cells += (double)Lq * Lr;
+}
+
+// MpStageSSRR::Align (MpStageSSRR.h:83-112): the SS/residue alignment of slot
+// 0 (gap open -1, weights 1 and 0.2), walked as BtckToMatched128xKernel does
+void ssrr_align(Ctx& c) {
+    PairState& s = *c.st;
+    if (s.flags & (CONV_LOWTMSC | CONV_SCOREDP)) { s.nalnposs = 0; return; }
+    std::vector<uint8_t> bt;
+    dp_fill_ssrr(c.q, c.r, -1.0f, 1.0f, 0.2f, bt, c.cells);
+    std::vector<int> qp, rp;
+    backtrace(bt, c.q.len, c.r.len, qp, rp);
+    store_matched(c, qp, rp);
+}
+
 // ---- stage 2 (MpStage2.h:60-131) -------------------------------------------
 void dp_align_grand(Ctx& c, float gapopen, bool gap0, bool usess, bool dpscan, bool constrained) {
     PairState& s = *c.st;
@@ -1028,6 +1088,14 @@ void dp_align_grand(Ctx& c, float gapopen, bool gap0, bool usess, bool dpscan, b
         if (s.flags & CONV_SCOREDP) { s.nalnposs = 0; return; }
         store_matched(c, qp, rp);
     }
+}
+
+// MpStageSSRR::RunSpecialized<true> (MpStageSSRR.h:62-76)
+void stage_ssrr(Ctx& c) {
+    ssrr_align(c);
+    refine_frag_dp(c);
+    dp_refine(c, false, false, 2, c.o->prescore);
+    dp_refine(c, true, false, 2, c.o->prescore);
 }
 
 void stage2(Ctx& c, bool gap0, bool usess, bool dpscan, int maxndpiters, bool checklow, float scorethld) {
@@ -1243,7 +1311,8 @@ void init_pair(Ctx& c) {
 }
 
 // the whole stage for one pair (MpBatch.cpp:283-485 with frg3 and the
-// sequence-similarity filter off, their defaults for this stage)
+// sequence-similarity filter off, their defaults for this stage; the search
+// by secondary structure and residue similarity when requested)
 void process_pair(Ctx& c) {
     const gta_options& o = *c.o;
     init_pair(c);
@@ -1251,6 +1320,9 @@ void process_pair(Ctx& c) {
     find_frag(c);
     refine_frag_init(c);
     dp_refine(c, true, true, 2, o.prescore);
+    // superposition search by secondary structure and residue similarity
+    // (MpBatch.cpp:421-436)
+    if (o.addsearchbyss) stage_ssrr(c);
     // stage 2, two passes (MpBatch.cpp:477-483)
     stage2(c, false, true, true, 2, false, o.scorethld);
     stage2(c, true, false, false, o.convergence, true, o.scorethld);
@@ -1365,6 +1437,26 @@ int gta_oracle_dp(const float* qx, const float* qy, const float* qz, const char*
     for (size_t k = 0; k < qp.size(); k++) { qpos[k] = qp[k]; rpos[k] = rp[k]; }
     return (int)qp.size();
 }
+
+// The SS/residue DP fill (ExecDPSSwBtck128xKernel<true>) and its walk for
+// one pair given SS letters and residue codes.
+int gta_oracle_dp_ssrr(const char* qss, const char* qrsd, int Lq, const char* rss, const char* rrsd, int Lr,
+                       float gapopen, float wss, float wrr, unsigned char* bt_out, int* qpos, int* rpos) {
+    Str q, r;
+    q.len = Lq; q.x = q.y = q.z = nullptr; q.ss = qss; q.rsd = qrsd;
+    r.len = Lr; r.x = r.y = r.z = nullptr; r.ss = rss; r.rsd = rrsd;
+    std::vector<uint8_t> bt;
+    double cells = 0;
+    dp_fill_ssrr(q, r, gapopen, wss, wrr, bt, cells);
+    if (bt_out) std::memcpy(bt_out, bt.data(), bt.size());
+    std::vector<int> qp, rp;
+    backtrace(bt, Lq, Lr, qp, rp);
+    for (size_t k = 0; k < qp.size(); k++) { qpos[k] = qp[k]; rpos[k] = rp[k]; }
+    return (int)qp.size();
+}
+
+// The residue score table (26 x 26, bits) the SS/residue fill uses.
+void gta_oracle_gonnet(float* out) { gta::gonnet_bits(out); }
 
 // Score-only DP fill (ExecDPScore128xKernel<GAP0>, MpDPHub.cpp:389-647) under
 // transform tfm with the search d0 (GetD02): the bottom-right cell's value.

@@ -38,6 +38,10 @@ def lib():
         _orc.gta_oracle_dp_score.argtypes = ([C.c_void_p] * 3 + [C.c_int] + [C.c_void_p] * 3 +
                                              [C.c_int, C.c_void_p, C.c_float, C.c_int])
         _orc.gta_oracle_dp_score.restype = C.c_float
+        _orc.gta_oracle_dp_ssrr.argtypes = ([C.c_char_p, C.c_char_p, C.c_int] * 2 + [C.c_float] * 3 +
+                                            [C.c_void_p] * 3)
+        _orc.gta_oracle_dp_ssrr.restype = C.c_int
+        _orc.gta_oracle_gonnet.argtypes = [C.c_void_p]
         _orc.gta_oracle_kabsch.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
         _orc.gta_oracle_rmsd.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
         _orc.gta_oracle_rmsd.restype = C.c_float
@@ -128,6 +132,48 @@ def dp(qxyz, rxyz, tfm, d02, gapopen=0.0, gap0=True, qss=None, rss=None, ssw=0.0
                             rs.ctypes.data, Lr, t.ctypes.data, d02, gapopen, int(gap0), int(usess),
                             ssw, bt.ctypes.data, qp.ctypes.data, rp.ctypes.data)
     return bt, np.stack([qp[:n], rp[:n]], axis=1)
+
+
+def _ssrr_call(fn, qss, qrsd, rss, rrsd, gapopen, wss, wrr):
+    Lq, Lr = len(qss), len(rss)
+    assert len(qrsd) == Lq and len(rrsd) == Lr
+    bt = np.zeros((Lq, Lr), np.uint8)
+    qpos = np.zeros(Lq + Lr, np.int32)
+    rpos = np.zeros(Lq + Lr, np.int32)
+    n = fn(qss, qrsd, Lq, rss, rrsd, Lr, gapopen, wss, wrr, bt.ctypes.data, qpos.ctypes.data,
+           rpos.ctypes.data)
+    if n < 0:
+        raise RuntimeError("DP failed (%d)" % n)
+    return bt, np.stack([qpos[:n], rpos[:n]], 1)
+
+
+def dp_ssrr(qss, qrsd, rss, rrsd, gapopen=-1.0, wss=1.0, wrr=0.2):
+    """The SS/residue DP of the --add-search-by-ss search for one pair (bytes
+    of SS letters and residue codes); returns (btck[Lq,Lr], matched pairs in
+    walk order)."""
+    return _ssrr_call(lib().gta_oracle_dp_ssrr, qss, qrsd, rss, rrsd, gapopen, wss, wrr)
+
+
+def ref_dp_ssrr(qss, qrsd, rss, rrsd, gapopen=-1.0, wss=1.0, wrr=0.2):
+    """The same on the reference's own code (oracle/_ref)."""
+    f = ref_lib().gta_ref_dp_ssrr
+    f.argtypes = [C.c_char_p, C.c_char_p, C.c_int] * 2 + [C.c_float] * 3 + [C.c_void_p] * 3
+    f.restype = C.c_int
+    return _ssrr_call(f, qss, qrsd, rss, rrsd, gapopen, wss, wrr)
+
+
+def gonnet():
+    """The residue score table (26 x 26, bits) of the oracle."""
+    t = np.zeros(26 * 26, np.float32)
+    lib().gta_oracle_gonnet(t.ctypes.data)
+    return t.reshape(26, 26)
+
+
+def ref_gonnet():
+    """The reference's own table (GONNET_SCORES)."""
+    t = np.zeros(26 * 26, np.float32)
+    ref_lib().gta_ref_gonnet(C.c_void_p(t.ctypes.data))
+    return t.reshape(26, 26)
 
 
 def dp_score(qxyz, rxyz, tfm, gap0=True, gapopen=0.0):

@@ -27,6 +27,8 @@
 #include "libmymp/mpss/MpSecStr.h"
 #include "libmymp/mpstage1/MpStage1.h"
 #include "libmymp/mpstage1/MpStage2.h"
+#include "libmymp/mpstage1/MpStageSSRR.h"
+#include "libutil/alpha.h"
 #include "libmymp/mpstage1/MpStageFin.h"
 
 #include "gta/gtalign_amd.h"
@@ -119,6 +121,15 @@ static int run_subchunk(const gta_chunk_view* q, const gta_chunk_view* r, const 
     MpSecStr(tmpdpdiagbuffers, qb, qe, rb, re, nqystrs, ndbCstrs, nqyposs, ndbCposs, qystr1len,
              dbstr1len, qystrnlen, dbstrnlen, dbxpad).Run();
     stg1.Run();
+    if (o->addsearchbyss) {
+        // MpBatch.cpp:421-436
+        MpStageSSRR stgSSRR(2, maxnsteps, minfraglen, o->prescore, stepinit, qb, qe, rb, re,
+                            nqystrs, ndbCstrs, nqyposs, ndbCposs, qystr1len, dbstr1len, qystrnlen, dbstrnlen,
+                            dbxpad, scores, tmpdpdiagbuffers, tmpdpbotbuffer, tmpdpalnpossbuffer, maxcoordsbuf,
+                            btckdata, wrkmemory, wrkmemoryccd, wrkmemorytm, wrkmemorytmibest, auxwrkmemory,
+                            wrkmemory2, alndatamemory, tfmmemory, globvarsbuf);
+        stgSSRR.RunSpecialized<true>(2);
+    }
     MpStage2 stg2(2, maxnsteps, minfraglen, o->prescore, stepinit, qb, qe, rb, re,
                   nqystrs, ndbCstrs, nqyposs, ndbCposs, qystr1len, dbstr1len, qystrnlen, dbstrnlen,
                   dbxpad, scores, tmpdpdiagbuffers, tmpdpbotbuffer, tmpdpalnpossbuffer, maxcoordsbuf,
@@ -259,19 +270,17 @@ int gta_ref_process_block(const gta_chunk_view* q, const gta_chunk_view* r, cons
     }
 }
 
-// The reference's DP fill + backtrace (MpDPHub::ExecDPwBtck128xKernel and
-// BtckToMatched128xKernel, MpDPHub.cpp:73-388,1500-1610) for one pair under a
-// given transform; bt receives the Lq x Lr backtrace codes and qpos/rpos the
-// matched positions in walk order.  Returns the number of matched pairs.
-int gta_ref_dp(const float* qx, const float* qy, const float* qz, int Lq, const float* rx,
-               const float* ry, const float* rz, int Lr, const float* tfm, int gap0, float gapopen,
-               unsigned char* bt, int* qpos, int* rpos) {
-    try {
-        std::vector<int> ql{Lq}, rl{Lr}, zq(Lq, 0), zr(Lr, 0);
-        std::vector<unsigned> d0{0};
-        std::vector<char> aq(Lq, 'A'), ar(Lr, 'A');
-        gta_chunk_view q{1, Lq, ql.data(), d0.data(), nullptr, qx, qy, qz, zq.data(), nullptr, aq.data(), nullptr};
-        gta_chunk_view r{1, Lr, rl.data(), d0.data(), nullptr, rx, ry, rz, zr.data(), nullptr, ar.data(), nullptr};
+}  // extern "C"
+
+// One DP fill + BtckToMatched of one pair on the reference's code: the
+// superposition fill ExecDPwBtck128xKernel (ssrr 0) or the fill from SS and
+// residue similarity ExecDPSSwBtck128xKernel<true> (ssrr 1, as
+// MpStageSSRR::Align calls it, MpStageSSRR.h:94-120).
+static int ref_dp_pair(const gta_chunk_view& q, const gta_chunk_view& r, int ssrr, const float* tfm, int gap0,
+                       float gapopen, float wss, float wrr, unsigned char* bt, int* qpos, int* rpos) {
+    const int Lq = q.nposs, Lr = r.nposs;
+    const float *qx = q.x, *rx = r.x;
+    {
         ChunkFields qf, rf;
         fill(&q, qf);
         fill(&r, rf);
@@ -310,7 +319,10 @@ int gta_ref_dp(const float* qx, const float* qy, const float* qz, int Lq, const 
                     (float*)sec(MpGlobalMemory::ddsBegOfWrkMemoryTM), wrkmemorytmibest, auxwrkmemory,
                     (float*)sec(MpGlobalMemory::ddsBegOfWrkMemory2), alndatamemory, tfmmemory,
                     (uint*)sec(MpGlobalMemory::ddsBegOfGlobVars));
-        if (gap0)
+        if (ssrr)
+            dph.ExecDPSSwBtck128xKernel<true>(gapopen, wss, wrr, qf.beg, rf.beg, auxwrkmemory, tmpdpdiagbuffers,
+                                              tmpdpbotbuffer, btckdata);
+        else if (gap0)
             dph.ExecDPwBtck128xKernel<false, false, true, D02IND_SEARCH, false>(
                 gapopen, 0, qf.beg, rf.beg, wrkmemorytmibest, auxwrkmemory, tmpdpdiagbuffers,
                 tmpdpbotbuffer, btckdata);
@@ -336,6 +348,47 @@ int gta_ref_dp(const float* qx, const float* qy, const float* qz, int Lq, const 
             rpos[k] = b;
         }
         return n;
+    }
+}
+
+extern "C" {
+
+// The reference's DP fill + backtrace (MpDPHub::ExecDPwBtck128xKernel and
+// BtckToMatched128xKernel, MpDPHub.cpp:73-388,1500-1610) for one pair under a
+// given transform; bt receives the Lq x Lr backtrace codes and qpos/rpos the
+// matched positions in walk order.  Returns the number of matched pairs.
+int gta_ref_dp(const float* qx, const float* qy, const float* qz, int Lq, const float* rx,
+               const float* ry, const float* rz, int Lr, const float* tfm, int gap0, float gapopen,
+               unsigned char* bt, int* qpos, int* rpos) {
+    try {
+        std::vector<int> ql{Lq}, rl{Lr}, zq(Lq, 0), zr(Lr, 0);
+        std::vector<unsigned> d0{0};
+        std::vector<char> aq(Lq, 'A'), ar(Lr, 'A');
+        gta_chunk_view q{1, Lq, ql.data(), d0.data(), nullptr, qx, qy, qz, zq.data(), nullptr, aq.data(), nullptr};
+        gta_chunk_view r{1, Lr, rl.data(), d0.data(), nullptr, rx, ry, rz, zr.data(), nullptr, ar.data(), nullptr};
+        return ref_dp_pair(q, r, 0, tfm, gap0, gapopen, 0.f, 0.f, bt, qpos, rpos);
+    } catch (...) {
+        return -1;
+    }
+}
+
+// The SS/residue fill for one pair given its SS letters and residue codes
+// (gap open gapopen, weights wss and wrr); matched positions in walk order.
+int gta_ref_dp_ssrr(const char* qss, const char* qrsd, int Lq, const char* rss, const char* rrsd, int Lr,
+                    float gapopen, float wss, float wrr, unsigned char* bt, int* qpos, int* rpos) {
+    try {
+        std::vector<int> ql{Lq}, rl{Lr}, zq(Lq, 0), zr(Lr, 0);
+        std::vector<unsigned> d0{0};
+        // distinct x coordinates identify the matched positions
+        std::vector<float> qx(Lq), rx(Lr), qz(Lq, 0.f), rz(Lr, 0.f);
+        for (int i = 0; i < Lq; i++) qx[i] = (float)i;
+        for (int j = 0; j < Lr; j++) rx[j] = (float)j;
+        const float tfm[12] = {1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0};
+        gta_chunk_view q{1, Lq, ql.data(), d0.data(), nullptr, qx.data(), qz.data(), qz.data(), zq.data(),
+                         nullptr, qrsd, qss};
+        gta_chunk_view r{1, Lr, rl.data(), d0.data(), nullptr, rx.data(), rz.data(), rz.data(), zr.data(),
+                         nullptr, rrsd, rss};
+        return ref_dp_pair(q, r, 1, tfm, 0, gapopen, wss, wrr, bt, qpos, rpos);
     } catch (...) {
         return -1;
     }
@@ -408,6 +461,10 @@ int gta_ref_dpscore(const float* qx, const float* qy, const float* qz, int Lq, c
         return -1;
     }
 }
+
+// The reference's pairwise residue scores (GONNET_SCORES, alpha.cpp:198-206):
+// 26 x 26 floats.
+void gta_ref_gonnet(float* out) { std::memcpy(out, GONNET_SCORES.data_, sizeof(GONNET_SCORES.data_)); }
 
 }  // extern "C"
 

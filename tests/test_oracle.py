@@ -166,6 +166,78 @@ def test_dp_matches_reference_cell_by_cell(have_ref, gap0, gapopen):
         assert np.array_equal(po, pr), (k, Lq, Lr)
 
 
+# ---- the search by secondary structure and residue similarity -------------------------
+
+def test_residue_scores_equal_reference_table(have_ref):
+    """The Gonnet table in bits (the HIP library uploads the same table)
+    equals the reference's GONNET_SCORES entry for entry."""
+    if not have_ref:
+        pytest.skip("reference CPU build (oracle/_ref) not present")
+    assert np.array_equal(oracle.gonnet(), oracle.ref_gonnet())
+
+
+def test_residue_scores_known_values():
+    """Entries fixed by the published matrix: log2(10^(G/10)) for G the
+    Gonnet entry (A-A 2.4, W-W 14.2, C-C 11.5, W-G -4.0), symmetric, and 0
+    for letters outside the 20 amino acids."""
+    g = oracle.gonnet()
+    idx = lambda a: ord(a) - ord("A")
+    for a, b, v in [("A", "A", 2.4), ("W", "W", 14.2), ("C", "C", 11.5), ("W", "G", -4.0), ("L", "I", 2.8)]:
+        assert abs(g[idx(a), idx(b)] - v / 10.0 * np.log2(10.0)) < 1e-5, (a, b)
+    assert np.array_equal(g, g.T)
+    for x in "BJOUXZ":
+        assert not g[idx(x)].any() and not g[:, idx(x)].any()
+
+
+def test_dp_ssrr_matches_reference_cell_by_cell(have_ref):
+    """The SS/residue DP fill (ExecDPSSwBtck128xKernel<true>) and its walk:
+    backtrace codes and matched positions identical to the reference's own
+    code, on random SS strings and residue codes that include non-standard
+    letters, lower case and gaps (which score 0)."""
+    if not have_ref:
+        pytest.skip("reference CPU build (oracle/_ref) not present")
+    rng = np.random.default_rng(21)
+    letters = np.frombuffer(b"ACDEFGHIKLMNPQRSTVWYXBZUaq-", np.uint8)
+    ssl = np.frombuffer(b"HE ", np.uint8)
+    for k in range(12):
+        Lq, Lr = (int(v) for v in rng.integers(1, 260, 2))
+        qss, rss = bytes(rng.choice(ssl, Lq)), bytes(rng.choice(ssl, Lr))
+        qr, rr = bytes(rng.choice(letters, Lq)), bytes(rng.choice(letters, Lr))
+        bo, po = oracle.dp_ssrr(qss, qr, rss, rr)
+        br, pr = oracle.ref_dp_ssrr(qss, qr, rss, rr)
+        assert np.array_equal(bo, br), (k, Lq, Lr)
+        assert np.array_equal(po, pr), (k, Lq, Lr)
+
+
+def test_dp_ssrr_identical_sequences_align_diagonally():
+    """Equal SS and residue strings: every residue pairs with itself (each
+    diagonal step scores >= 1 + 0.2 x a positive self-score, any gap costs 1)."""
+    rng = np.random.default_rng(4)
+    s = bytes(rng.choice(np.frombuffer(b"ACDEFGHIKLMNPQRSTVWY", np.uint8), 90))
+    ss = bytes(rng.choice(np.frombuffer(b"HE ", np.uint8), 90))
+    _, pos = oracle.dp_ssrr(ss, s, ss, s)
+    assert np.array_equal(pos[:, 0], np.arange(89, -1, -1))
+    assert np.array_equal(pos[:, 1], pos[:, 0])
+
+
+@pytest.mark.parametrize("cfg,scale,qsel", [(0, 0.25, None), (1, 0.004, [0, 9, 15])],
+                         ids=["c0_1x64", "c1_3x200"])
+def test_stage_search_by_ss_matches_reference(have_ref, cfg, scale, qsel):
+    """--add-search-by-ss: the oracle's whole stage equals the reference's CPU
+    path (with MpStageSSRR between stages 1 and 2) bit for bit."""
+    if not have_ref:
+        pytest.skip("reference CPU build (oracle/_ref) not present")
+    qc, rc, _ = synth.config(cfg, scale=scale)
+    if qsel:
+        qc = qc.subset(qsel)
+    opts = default_options(addsearchbyss=1)
+    o = oracle.process_block(qc, rc, opts)
+    r = oracle.ref_process_block(qc, rc, opts, nthreads=1)
+    assert np.array_equal(o.alns, r.alns)
+    assert np.array_equal(o.alndata, r.alndata)
+    assert np.array_equal(o.tfms, r.tfms)
+
+
 # ---- the whole stage ------------------------------------------------------------------
 
 def test_self_alignment_scores_one():
