@@ -38,6 +38,7 @@ class Options(C.Structure):
         ("nodeletions", C.c_int), ("referenced", C.c_int), ("precision", C.c_int),
         ("batched", C.c_int),
         ("addsearchbyss", C.c_int),
+        ("qgroup", C.c_int),
     ]
 
 
@@ -54,7 +55,7 @@ def default_options(**kw):
     """Reference defaults (CLOptions.cpp:35-100) with keyword overrides."""
     o = Options(prescore=0.4, scorethld=0.5, convergence=18, depth=DEPTH_MEDIUM,
                 refinement=REF_ONE, symmetric=0, nodeletions=0, referenced=0,
-                precision=PREC_FP32, batched=0, addsearchbyss=0)
+                precision=PREC_FP32, batched=0, addsearchbyss=0, qgroup=0)
     for k, v in kw.items():
         if not hasattr(o, k):
             raise KeyError(k)

@@ -154,6 +154,7 @@ void gta_default_options(gta_options* o) {
     o->precision = GTA_PREC_FP32;
     o->batched = 0;
     o->addsearchbyss = 0;
+    o->qgroup = 0;
 }
 
 size_t gta_alignment_section_size(int nq, int nqposs, int nr, int nrposs) {
@@ -173,6 +174,7 @@ static int check_options(const gta_options* o) {
     if (o->precision != GTA_PREC_FP32 && o->precision != GTA_PREC_MIXED) return 0;
     if (o->batched != 0 && o->batched != 1) return 0;
     if (o->addsearchbyss != 0 && o->addsearchbyss != 1) return 0;
+    if (o->qgroup < 0) return 0;
     return 1;
 }
 
@@ -749,6 +751,8 @@ static int run_stage(gta_ctx* c, double* seconds, double* cells) {
         GTA_TIMED(KC_REFINE_GAPLESS, launch_refine_gapless(Q, R, S, G, dinv, c->rsp, st));
         int rc;
         if ((rc = dprefine(1, o.prescore, 2)) != GTA_OK) return rc;
+        // references kept for a block of queries continue with all its queries
+        if (0.0f < o.prescore) GTA_TIMED(KC_MISC, launch_keep_refs(Q, R, S, o.qgroup, st));
         HIPCHK(c, relist());
         if (o.addsearchbyss) {
             // superposition search from the alignment of secondary structure

@@ -173,6 +173,24 @@ __global__ void k_init(DevChunk Q, DevChunk R, DevState S) {
     S.out[p * 15 + 8] = 9999.9f;
 }
 
+// The reference filter after the stage-1 prescreen (MpReform.h:117-200,
+// MpReform.cpp:28-124; CuBatch.cu:558-563): per block of `qgroup` consecutive
+// queries (0: all queries), a reference flagged low-scoring for every query
+// of the block leaves it (its pairs keep the flag), and every pair of a
+// reference that stays has its flags reset to the molecule-type check.  One
+// thread per (block, reference).
+__global__ void k_keep_refs(DevChunk Q, DevChunk R, DevState S, int qgroup) {
+    const int nb = (S.nq + qgroup - 1) / qgroup;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)nb * S.nr) return;
+    const int b = (int)(t / S.nr), ri = (int)(t % S.nr);
+    const int q0 = b * qgroup, q1 = min(S.nq, q0 + qgroup);
+    bool stays = false;
+    for (int qi = q0; qi < q1; qi++) stays |= !(S.flags[qi * S.nr + ri] & CONV_LOWTMSC);
+    if (!stays) return;
+    for (int qi = q0; qi < q1; qi++) S.flags[qi * S.nr + ri] = (Q.na[qi] == R.na[ri]) ? 0 : CONV_LOWTMSC;
+}
+
 // ============================================================================
 // gapless passes over structure coordinates (stage 1), in slot groups
 struct Coords {
@@ -1883,6 +1901,13 @@ static inline int blocks_for(long long n, int t) { return (int)((n + t - 1) / t)
 
 hipError_t launch_secstr(const DevChunk& C, hipStream_t st) {
     hipLaunchKernelGGL(k_secstr, dim3(blocks_for(C.nposs, 256)), dim3(256), 0, st, C);
+    return hipGetLastError();
+}
+hipError_t launch_keep_refs(const DevChunk& Q, const DevChunk& R, const DevState& S, int qgroup,
+                            hipStream_t st) {
+    const int g = (qgroup > 0 && qgroup < S.nq) ? qgroup : S.nq;
+    const long long n = (long long)((S.nq + g - 1) / g) * S.nr;
+    if (n > 0) hipLaunchKernelGGL(k_keep_refs, dim3(blocks_for(n, 256)), dim3(256), 0, st, Q, R, S, g);
     return hipGetLastError();
 }
 hipError_t launch_init(const DevChunk& Q, const DevChunk& R, const DevState& S, hipStream_t st) {

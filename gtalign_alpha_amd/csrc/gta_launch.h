@@ -67,6 +67,9 @@ struct RefineLaunch {
 
 hipError_t launch_secstr(const DevChunk& C, hipStream_t st);
 hipError_t launch_init(const DevChunk& Q, const DevChunk& R, const DevState& S, hipStream_t st);
+// the reference filter after the stage-1 prescreen, per block of qgroup queries (0: all)
+hipError_t launch_keep_refs(const DevChunk& Q, const DevChunk& R, const DevState& S, int qgroup,
+                            hipStream_t st);
 // the superposition searches run in the reference's FP32 arithmetic in both
 // precision forms (their result is a choice of superposition, which the final
 // stage refines in the precision of the form)

@@ -97,6 +97,13 @@ typedef struct gta_options {
     int addsearchbyss;       /* --add-search-by-ss (default 0): after stage 1, an extra superposition
                                 search from an alignment of secondary structure and residue similarity
                                 (MpStageSSRR.h:62-76) */
+    int qgroup;              /* queries per block of the reference filter after stage 1 (default 0):
+                                with --pre-score > 0 the reference drops from a block the references
+                                that every query of the block flags low-scoring, and the pairs of the
+                                references it keeps all continue, their flags reset (CuBatch.cu:558-563,
+                                MpReform.cpp:28-124).  0: the whole query chunk is one block, as one
+                                CuBatch::ProcessBlock call; k: blocks of k consecutive queries, as the
+                                reference's scheduler forms them (--dev-queries-per-chunk, default 2) */
 } gta_options;
 
 /* results of one gta_process call; host memory owned by the context and

@@ -1310,16 +1310,22 @@ void init_pair(Ctx& c) {
     s.flags = (c.q.na == c.r.na) ? 0 : CONV_LOWTMSC;
 }
 
-// the whole stage for one pair (MpBatch.cpp:283-485 with frg3 and the
+// The stage for one pair (MpBatch.cpp:283-485 with frg3 and the
 // sequence-similarity filter off, their defaults for this stage; the search
-// by secondary structure and residue similarity when requested)
-void process_pair(Ctx& c) {
+// by secondary structure and residue similarity when requested), in two
+// parts around the reference filter of the block (process_block).
+// Stage 1 (MpStage1.h:78-85).
+void process_pair_stage1(Ctx& c) {
     const gta_options& o = *c.o;
     init_pair(c);
-    // stage 1 (MpStage1.h:78-85)
     find_frag(c);
     refine_frag_init(c);
     dp_refine(c, true, true, 2, o.prescore);
+}
+
+// everything after the filter
+void process_pair_rest(Ctx& c) {
+    const gta_options& o = *c.o;
     // superposition search by secondary structure and residue similarity
     // (MpBatch.cpp:421-436)
     if (o.addsearchbyss) stage_ssrr(c);
@@ -1373,10 +1379,12 @@ size_t gta_oracle_alignment_section_size(int nq, int nqposs, int nr, int nrposs)
 
 // The whole stage on CPU.  alndata: nq*nr*15, tfms: nq*nr*12, alns: the
 // alignment section (gta_oracle_alignment_section_size bytes), ss_q/ss_r:
-// optional outputs of the SS pass.  Returns the number of DP cells updated.
+// optional outputs of the SS pass, pass: optional nq*nr bytes, 1 for the
+// pairs whose reference stays in its block after the stage-1 filter.
+// Returns the number of DP cells updated.
 double gta_oracle_process_block(const gta_chunk_view* q, const gta_chunk_view* r,
                                 const gta_options* o, float* alndata, float* tfms, char* alns,
-                                char* ss_q, char* ss_r) {
+                                char* ss_q, char* ss_r, unsigned char* pass) {
     std::vector<char> qss, rss;
     calc_secstr(q, qss);
     calc_secstr(r, rss);
@@ -1387,16 +1395,51 @@ double gta_oracle_process_block(const gta_chunk_view* q, const gta_chunk_view* r
     const size_t alnsize = gta_oracle_alignment_section_size(nq, q->nposs, nr, r->nposs);
     if (alns) std::memset(alns, 0, alnsize);
     double cells = 0.0;
+    std::vector<PairState> states((size_t)nq * nr);
+    auto ctx = [&](int qi, int ri) {
+        Ctx c;
+        c.o = o; c.g = g;
+        c.q = get_str(q, qss, qi);
+        c.r = get_str(r, rss, ri);
+        c.st = &states[(size_t)qi * nr + ri];
+        return c;
+    };
+    for (int qi = 0; qi < nq; qi++)
+        for (int ri = 0; ri < nr; ri++) {
+            Ctx c = ctx(qi, ri);
+            process_pair_stage1(c);
+            cells += c.cells;
+        }
+    // the reference filter after the stage-1 prescreen (MpBatch.cpp:371-378;
+    // MpReform.h:117-200, MpReform.cpp:104-114): per block of queries, a
+    // reference flagged low-scoring for every query leaves the block; for the
+    // references that stay, every pair's flags are reset to the molecule-type
+    // check alone
+    std::vector<unsigned char> keep((size_t)nq * nr, 1);
+    if (0.0f < o->prescore) {
+        const int bq = o->qgroup > 0 ? o->qgroup : nq;
+        for (int q0 = 0; q0 < nq; q0 += bq) {
+            const int q1 = std::min(nq, q0 + bq);
+            for (int ri = 0; ri < nr; ri++) {
+                bool stays = false;
+                for (int qi = q0; qi < q1; qi++)
+                    if (!(states[(size_t)qi * nr + ri].flags & CONV_LOWTMSC)) stays = true;
+                for (int qi = q0; qi < q1; qi++) {
+                    PairState& st = states[(size_t)qi * nr + ri];
+                    keep[(size_t)qi * nr + ri] = stays;
+                    if (stays) st.flags = (get_str(q, qss, qi).na == get_str(r, rss, ri).na) ? 0 : CONV_LOWTMSC;
+                }
+            }
+        }
+    }
+    if (pass) std::memcpy(pass, keep.data(), keep.size());
     for (int qi = 0; qi < nq; qi++) {
         for (int ri = 0; ri < nr; ri++) {
-            PairState st;
-            Ctx c;
-            c.o = o; c.g = g;
-            c.q = get_str(q, qss, qi);
-            c.r = get_str(r, rss, ri);
-            c.st = &st;
-            process_pair(c);
+            Ctx c = ctx(qi, ri);
+            c.cells = 0;
+            process_pair_rest(c);
             cells += c.cells;
+            PairState& st = *c.st;
             const size_t p = (size_t)qi * nr + ri;
             std::memcpy(alndata + p * GTA_ALN_NFIELDS, st.out, sizeof(st.out));
             std::memcpy(tfms + p * GTA_TFM_NFIELDS, st.tfm_grand, 12 * sizeof(float));

@@ -29,7 +29,7 @@ def lib():
         _orc = C.CDLL(ORACLE_LIB)
         _orc.gta_oracle_process_block.argtypes = [C.POINTER(ChunkView), C.POINTER(ChunkView),
                                                   C.POINTER(Options), C.c_void_p, C.c_void_p,
-                                                  C.c_void_p, C.c_void_p, C.c_void_p]
+                                                  C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         _orc.gta_oracle_process_block.restype = C.c_double
         _orc.gta_oracle_dp.argtypes = ([C.c_void_p] * 4 + [C.c_int] + [C.c_void_p] * 4 +
                                        [C.c_int, C.c_void_p, C.c_float, C.c_float, C.c_int, C.c_int,
@@ -63,12 +63,23 @@ def ref_lib():
                                                C.POINTER(Options), C.c_int, C.c_void_p, C.c_void_p,
                                                C.c_void_p, C.c_size_t, C.POINTER(C.c_double)]
         _ref.gta_ref_process_block.restype = C.c_int
+        _ref.gta_ref_last_pass.argtypes = [C.c_void_p, C.c_size_t]
+        _ref.gta_ref_last_pass.restype = C.c_int
+        _ref.gta_ref_set_move_slots.argtypes = [C.c_int]
+        _ref.gta_ref_set_move_slots.restype = C.c_int
     return _ref
 
 
 class BlockResult:
-    def __init__(self, nq, nr, alndata, tfms, alns, seconds=0.0, cells=0.0, ss_q=None, ss_r=None):
+    """Result sections of a block; `passed[q, r]` is True for the pairs whose
+    reference stayed in its block after the stage-1 filter (the reference
+    reports nothing for the others)."""
+
+    def __init__(self, nq, nr, alndata, tfms, alns, seconds=0.0, cells=0.0, ss_q=None, ss_r=None,
+                 passed=None):
         self.nq, self.nr = nq, nr
+        self.passed = (np.ones((nq, nr), bool) if passed is None
+                       else np.asarray(passed, bool).reshape(nq, nr))
         self.alndata = alndata.reshape(nq, nr, GTA_ALN_NFIELDS)
         self.tfms = tfms.reshape(nq, nr, GTA_TFM_NFIELDS)
         self.alns = alns
@@ -91,13 +102,14 @@ def process_block(q, r, opts=None):
     aln, tfm, alns = _buffers(q, r)
     ssq = np.zeros(q.nposs, dtype=np.uint8)
     ssr = np.zeros(r.nposs, dtype=np.uint8)
+    passed = np.zeros(q.nstrs * r.nstrs, dtype=np.uint8)
     import time
     t0 = time.perf_counter()
     cells = lib().gta_oracle_process_block(C.byref(q.view()), C.byref(r.view()), C.byref(opts),
                                            aln.ctypes.data, tfm.ctypes.data, alns.ctypes.data,
-                                           ssq.ctypes.data, ssr.ctypes.data)
+                                           ssq.ctypes.data, ssr.ctypes.data, passed.ctypes.data)
     dt = time.perf_counter() - t0
-    return BlockResult(q.nstrs, r.nstrs, aln, tfm, alns, dt, cells, ssq, ssr)
+    return BlockResult(q.nstrs, r.nstrs, aln, tfm, alns, dt, cells, ssq, ssr, passed)
 
 
 def ref_process_block(q, r, opts=None, nthreads=1):
@@ -110,7 +122,38 @@ def ref_process_block(q, r, opts=None, nthreads=1):
                                          alns.ctypes.data, alns.nbytes, C.byref(secs))
     if rc != 0:
         raise RuntimeError("reference CPU path failed (%d)" % rc)
-    return BlockResult(q.nstrs, r.nstrs, aln, tfm, alns, secs.value)
+    passed = np.zeros(q.nstrs * r.nstrs, dtype=np.uint8)
+    if ref_lib().gta_ref_last_pass(passed.ctypes.data, passed.size) != 0:
+        raise RuntimeError("reference CPU path: no filter record")
+    return BlockResult(q.nstrs, r.nstrs, aln, tfm, alns, secs.value, passed=passed)
+
+
+def ref_set_move_slots(on):
+    """Diagnostic switch of the reference driver: when on, packing a block
+    after stage 1 also moves every kept pair's per-slot transforms and
+    records to its new index (the reference moves only the per-pair records
+    and output transform).  Returns the previous setting."""
+    return ref_lib().gta_ref_set_move_slots(int(on))
+
+
+def kept_first(q, r, opts=None):
+    """The references of r reordered so that, after the reference's own
+    filter after stage 1 (one block: all queries of q), the references it
+    keeps come before those it drops: kept references longer than a length
+    split and dropped ones not longer, in descending length, the split chosen
+    to keep the most aligned references.  Kept references then keep their
+    indices when the reference packs the block, so each kept pair's results
+    are the reference's for that pair alone (packing moves the per-pair
+    records but not every per-slot buffer, MpReform.cpp:57-124)."""
+    res = ref_process_block(q, r, opts, nthreads=1)
+    kept = res.passed.any(axis=0)
+    aligned = (res.passed & (res.alndata[..., 8] < 9999)).any(axis=0)
+
+    def pick(x):
+        return [i for i in range(r.nstrs) if (kept[i] and r.len[i] > x) or (not kept[i] and r.len[i] <= x)]
+
+    x = max(sorted(set(r.len.tolist())), key=lambda v: (sum(aligned[i] for i in pick(v)), len(pick(v))))
+    return r.subset(sorted(pick(x), key=lambda i: -r.len[i]))
 
 
 def dp(qxyz, rxyz, tfm, d02, gapopen=0.0, gap0=True, qss=None, rss=None, ssw=0.0):

@@ -7,7 +7,8 @@ processes.  bench.py's CPU baseline uses this.
 
     python -m oracle.ref_cli IN.npz OUT.npz NTHREADS
 IN holds q_len/q_x/q_y/q_z/q_rsd and r_* arrays; OUT receives alndata, tfms,
-alns and the wall-clock seconds of the call.
+alns, the per-pair record of the stage-1 filter (passed) and the wall-clock
+seconds of the call.
 """
 import os
 import sys
@@ -41,7 +42,7 @@ def main():
     t0 = time.perf_counter()
     res = oracle.ref_process_block(q, r, nthreads=nthreads)
     dt = time.perf_counter() - t0
-    np.savez(dst, alndata=res.alndata, tfms=res.tfms, alns=res.alns, seconds=dt)
+    np.savez(dst, alndata=res.alndata, tfms=res.tfms, alns=res.alns, seconds=dt, passed=res.passed)
 
 
 if __name__ == "__main__":

@@ -6,10 +6,10 @@
 // baseline; it is never part of the product.  The sequence below follows
 // MpBatch::ProcessBlock (src/libmymp/mpbatch/MpBatch.cpp:283-485) with the
 // sequence-similarity pre-filter, the frg3 search (--no-detailed-search) and
-// the finalizer thread left out.  Reference compaction after stage 1
-// (FilteroutReferences, MpBatch.cpp:359-367) is replaced by leaving the
-// flagged pairs in the chunk: every later kernel skips pairs carrying the
-// low-score flag, so the surviving pairs see the same computation.
+// the finalizer thread left out.  After stage 1 the reference's own
+// compaction runs (MpReform, PMBatchStrData::FilterStructs): references that
+// no query of the block keeps leave it and report nothing
+// (gta_ref_last_pass tells which pairs stayed).
 
 #include <omp.h>
 #include <chrono>
@@ -28,6 +28,7 @@
 #include "libmymp/mpstage1/MpStage1.h"
 #include "libmymp/mpstage1/MpStage2.h"
 #include "libmymp/mpstage1/MpStageSSRR.h"
+#include "libmymp/mpfilter/MpReform.h"
 #include "libutil/alpha.h"
 #include "libmymp/mpstage1/MpStageFin.h"
 
@@ -71,9 +72,35 @@ void fill(const gta_chunk_view* c, ChunkFields& f) {
 
 extern "C" {
 
+// per pair of the last gta_ref_process_block call: 1 when the pair's
+// reference stayed in its block after the stage-1 filter
+static std::vector<unsigned char> g_pass;
+// diagnostic: when set, packing a block also moves each kept pair's per-slot
+// transforms and per-slot records (slots 1..) to its new index, which the
+// reference's packing does not do (MpReform.cpp:57-124 moves slot 0 of the
+// records and the output transform only)
+static int g_move_slots = 0;
+
+// Moves rows [(q*maxnsteps + si) * n + r] of `width` floats from the
+// unpacked layout (n = ndb) to the packed one (n = ndb2, index newndx[r]).
+static void move_slot_rows(float* buf, size_t nq, size_t maxnsteps, size_t ndb, size_t ndb2,
+                           const std::vector<int>& newndx, size_t width, size_t si0) {
+    std::vector<float> old(buf, buf + nq * maxnsteps * ndb * width);
+    for (size_t q = 0; q < nq; q++)
+        for (size_t si = si0; si < maxnsteps; si++)
+            for (size_t r = 0; r < ndb; r++) {
+                if (newndx[r] < 0) continue;
+                std::memcpy(buf + ((q * maxnsteps + si) * ndb2 + newndx[r]) * width,
+                            old.data() + ((q * maxnsteps + si) * ndb + r) * width, width * sizeof(float));
+            }
+}
+
 // Runs one reference sub-chunk (which the reference's memory layout must hold).
+// pass[ql * nstrs + rl] receives 1 for the pairs of references that stay in
+// the block after the stage-1 filter; the result sections of the others are
+// left untouched (the reference reports nothing for them).
 static int run_subchunk(const gta_chunk_view* q, const gta_chunk_view* r, const gta_options* o,
-                        MpGlobalMemory* gmem, float* alndata, float* tfms, char* alns) {
+                        MpGlobalMemory* gmem, float* alndata, float* tfms, char* alns, unsigned char* pass) {
     ChunkFields qf, rf;
     fill(q, qf);
     fill(r, rf);
@@ -121,32 +148,100 @@ static int run_subchunk(const gta_chunk_view* q, const gta_chunk_view* r, const 
     MpSecStr(tmpdpdiagbuffers, qb, qe, rb, re, nqystrs, ndbCstrs, nqyposs, ndbCposs, qystr1len,
              dbstr1len, qystrnlen, dbstrnlen, dbxpad).Run();
     stg1.Run();
+
+    // MpBatch::FilteroutReferences after the stage-1 prescreen
+    // (MpBatch.cpp:104-193,371-378): references flagged low-scoring for every
+    // query of the block leave it, the others are packed and their pairs'
+    // flags reset (MpReform.cpp:28-124)
+    size_t ndbCposs2 = ndbCposs, ndbCstrs2 = ndbCstrs, dbstr1len2 = dbstr1len, dbstrnlen2 = dbstrnlen;
+    size_t dbxpad2 = dbxpad;
+    std::vector<int> newndx(ndbCstrs);
+    for (size_t i = 0; i < ndbCstrs; i++) newndx[i] = (int)i;
+    if (0.0f < o->prescore) {
+        std::vector<uint> filterdata(ndbCstrs * nTFilterData, 0);
+        std::vector<const char*> desc(ndbCstrs, "");
+        MpReform refm(maxnsteps, qb, qe, rb, re, nullptr, nullptr, nullptr, nullptr, nqystrs, ndbCstrs, nqyposs,
+                      ndbCposs, tmpdpdiagbuffers, auxwrkmemory, tfmmemory, globvarsbuf, filterdata.data());
+        refm.MakeDbCandidateList();
+        for (size_t i = 0; i < ndbCstrs; i++) {
+            const uint nd = filterdata[ndbCstrs * fdNewReferenceIndex + i];
+            newndx[i] = nd ? (int)nd - 1 : -1;
+        }
+        PMBatchStrData::FilterStructs(desc.data(), rb, re, filterdata.data());
+        ndbCposs2 = PMBatchStrData::GetNoPosits(rb, re);
+        ndbCstrs2 = PMBatchStrData::GetNoStructs(rb, re);
+        if (ndbCstrs2) {
+            dbstr1len2 = PMBatchStrData::GetLengthAt(rb, 0);
+            dbstrnlen2 = PMBatchStrData::GetLengthAt(rb, ndbCstrs2 - 1);
+        }
+        dbxpad2 = ALIGN_UP(ndbCposs2, CUL2CLINESIZE) - ndbCposs2;
+        if (ndbCstrs2 && ndbCposs2 && g_move_slots) {
+            move_slot_rows(wrkmemorytmibest, nqystrs, maxnsteps, ndbCstrs, ndbCstrs2, newndx, nTTranformMatrix, 0);
+            move_slot_rows(wrkmemorytm, nqystrs, maxnsteps, ndbCstrs, ndbCstrs2, newndx, nTTranformMatrix, 0);
+            // per-slot records, field-major: [(q*maxnsteps + si) * NF + f] * n + r
+            std::vector<float> old(auxwrkmemory, auxwrkmemory + nqystrs * maxnsteps * nTAuxWorkingMemoryVars * ndbCstrs);
+            for (size_t q = 0; q < nqystrs; q++)
+                for (size_t si = 1; si < maxnsteps; si++)
+                    for (int f = 0; f < nTAuxWorkingMemoryVars; f++)
+                        for (size_t r = 0; r < ndbCstrs; r++)
+                            if (newndx[r] >= 0)
+                                auxwrkmemory[((q * maxnsteps + si) * nTAuxWorkingMemoryVars + f) * ndbCstrs2 + newndx[r]] =
+                                    old[((q * maxnsteps + si) * nTAuxWorkingMemoryVars + f) * ndbCstrs + r];
+        }
+        if (ndbCstrs2 && ndbCposs2) refm.SelectAndReformat((int)ndbCstrs2, (int)gmem->GetCurrentMaxDbPos());
+    }
+    for (size_t ql = 0; ql < nqystrs; ql++)
+        for (size_t i = 0; i < ndbCstrs; i++) pass[ql * ndbCstrs + i] = newndx[i] >= 0;
+    if (!ndbCstrs2 || !ndbCposs2) return 0;
+
     if (o->addsearchbyss) {
         // MpBatch.cpp:421-436
         MpStageSSRR stgSSRR(2, maxnsteps, minfraglen, o->prescore, stepinit, qb, qe, rb, re,
-                            nqystrs, ndbCstrs, nqyposs, ndbCposs, qystr1len, dbstr1len, qystrnlen, dbstrnlen,
-                            dbxpad, scores, tmpdpdiagbuffers, tmpdpbotbuffer, tmpdpalnpossbuffer, maxcoordsbuf,
+                            nqystrs, ndbCstrs2, nqyposs, ndbCposs2, qystr1len, dbstr1len2, qystrnlen, dbstrnlen2,
+                            dbxpad2, scores, tmpdpdiagbuffers, tmpdpbotbuffer, tmpdpalnpossbuffer, maxcoordsbuf,
                             btckdata, wrkmemory, wrkmemoryccd, wrkmemorytm, wrkmemorytmibest, auxwrkmemory,
                             wrkmemory2, alndatamemory, tfmmemory, globvarsbuf);
         stgSSRR.RunSpecialized<true>(2);
     }
     MpStage2 stg2(2, maxnsteps, minfraglen, o->prescore, stepinit, qb, qe, rb, re,
-                  nqystrs, ndbCstrs, nqyposs, ndbCposs, qystr1len, dbstr1len, qystrnlen, dbstrnlen,
-                  dbxpad, scores, tmpdpdiagbuffers, tmpdpbotbuffer, tmpdpalnpossbuffer, maxcoordsbuf,
+                  nqystrs, ndbCstrs2, nqyposs, ndbCposs2, qystr1len, dbstr1len2, qystrnlen, dbstrnlen2,
+                  dbxpad2, scores, tmpdpdiagbuffers, tmpdpbotbuffer, tmpdpalnpossbuffer, maxcoordsbuf,
                   btckdata, wrkmemory, wrkmemoryccd, wrkmemorytm, wrkmemorytmibest, auxwrkmemory,
                   wrkmemory2, alndatamemory, tfmmemory, globvarsbuf);
     stg2.RunSpecialized<false, true, D02IND_DPSCAN>(2, false, o->scorethld);
     stg2.RunSpecialized<true, false, D02IND_SEARCH>(o->convergence, true, o->scorethld);
+    const size_t szalns2 = gmem->GetSizeOfDP2Alns(nqystrs, nqyposs, ndbCposs2, ndbCstrs2);
+    std::memset(alnsmemory, 0, szalns2);
     MpStageFin(EQUIVALENCE_DISTANCE * EQUIVALENCE_DISTANCE, maxnsteps, minfraglen, qb, qe, rb, re,
-               nqystrs, ndbCstrs, nqyposs, ndbCposs, qystr1len, dbstr1len, qystrnlen, dbstrnlen,
-               dbxpad, scores, tmpdpdiagbuffers, tmpdpbotbuffer, tmpdpalnpossbuffer, maxcoordsbuf,
+               nqystrs, ndbCstrs2, nqyposs, ndbCposs2, qystr1len, dbstr1len2, qystrnlen, dbstrnlen2,
+               dbxpad2, scores, tmpdpdiagbuffers, tmpdpbotbuffer, tmpdpalnpossbuffer, maxcoordsbuf,
                btckdata, wrkmemory, wrkmemoryccd, wrkmemorytm, wrkmemorytmibest, auxwrkmemory,
                wrkmemory2, alndatamemory, tfmmemory, alnsmemory, globvarsbuf).Run();
 
-    const size_t np = nqystrs * ndbCstrs;
-    std::memcpy(alndata, alndatamemory, np * nTDP2OutputAlnData * sizeof(float));
-    std::memcpy(tfms, tfmmemory, np * nTTranformMatrix * sizeof(float));
-    std::memcpy(alns, alnsmemory, szalns);
+    // results of the packed block back at the references' places in the
+    // caller's sub-chunk (alignment rows: production_match2aln.cu:130-133)
+    std::vector<size_t> dst2(ndbCstrs2);
+    for (size_t i = 0; i < ndbCstrs2; i++) dst2[i] = PMBatchStrData::GetAddressAt(rb, i);
+    for (size_t ql = 0; ql < nqystrs; ql++) {
+        const size_t qlen = q->len[ql];
+        for (size_t i = 0; i < ndbCstrs; i++) {
+            const int k = newndx[i];
+            if (k < 0) continue;
+            std::memcpy(alndata + (ql * ndbCstrs + i) * nTDP2OutputAlnData,
+                        alndatamemory + (ql * ndbCstrs2 + k) * nTDP2OutputAlnData, nTDP2OutputAlnData * sizeof(float));
+            std::memcpy(tfms + (ql * ndbCstrs + i) * nTTranformMatrix,
+                        tfmmemory + (ql * ndbCstrs2 + k) * nTTranformMatrix, nTTranformMatrix * sizeof(float));
+            const size_t seg = qlen + r->len[i] + 1;
+            const size_t sofff = (ql * ndbCposs2 + ndbCstrs2 * (q->dst[ql] + ql)) * 5;
+            const size_t slen = ndbCposs2 + ndbCstrs2 * (qlen + 1);
+            const size_t sbeg = dst2[k] + (size_t)k * (qlen + 1);
+            const size_t dofff = (ql * ndbCposs + ndbCstrs * (q->dst[ql] + ql)) * 5;
+            const size_t dlen = ndbCposs + ndbCstrs * (qlen + 1);
+            const size_t dbeg = (size_t)r->dst[i] + i * (qlen + 1);
+            for (int f = 0; f < 5; f++)
+                std::memcpy(alns + dofff + dbeg + dlen * f, alnsmemory + sofff + sbeg + slen * f, seg);
+        }
+    }
     return 0;
 }
 
@@ -180,6 +275,7 @@ int gta_ref_process_block(const gta_chunk_view* q, const gta_chunk_view* r, cons
         std::vector<float> sa, st;
         std::vector<char> sl;
         if (alns) std::memset(alns, 0, alnsize);
+        g_pass.assign((size_t)nq * nr, 0);
         double total = 0.0;
         for (int q0 = 0; q0 < nq; q0 += qgroup) {
             const int q1 = std::min(nq, q0 + qgroup);
@@ -233,13 +329,15 @@ int gta_ref_process_block(const gta_chunk_view* q, const gta_chunk_view* r, cons
                 sl.assign(gmem->GetSizeOfDP2Alns((size_t)nqs, (size_t)qs.nposs, (size_t)sub.nposs,
                                                  (size_t)sub.nstrs), 0);
                 auto t0 = std::chrono::steady_clock::now();
-                int rc = run_subchunk(&qs, &sub, o, gmem, sa.data(), st.data(), sl.data());
+                std::vector<unsigned char> sp(nps, 0);
+                int rc = run_subchunk(&qs, &sub, o, gmem, sa.data(), st.data(), sl.data(), sp.data());
                 total += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
                 if (rc) return rc;
                 for (int qi = q0; qi < q1; qi++) {
                     const int ql = qi - q0;
                     for (int ri = r0; ri < r1; ri++) {
                         const int rl = ri - r0;
+                        g_pass[(size_t)qi * nr + ri] = sp[(size_t)ql * sub.nstrs + rl];
                         std::memcpy(alndata + ((size_t)qi * nr + ri) * nTDP2OutputAlnData,
                                     sa.data() + ((size_t)ql * sub.nstrs + rl) * nTDP2OutputAlnData,
                                     nTDP2OutputAlnData * sizeof(float));
@@ -268,6 +366,22 @@ int gta_ref_process_block(const gta_chunk_view* q, const gta_chunk_view* r, cons
     } catch (...) {
         return -1;
     }
+}
+
+// Diagnostic switch (see g_move_slots); returns the previous setting.
+int gta_ref_set_move_slots(int on) {
+    const int prev = g_move_slots;
+    g_move_slots = on ? 1 : 0;
+    return prev;
+}
+
+// Per pair of the last gta_ref_process_block call (nq*nr bytes): 1 when the
+// pair's reference stayed in its block after the stage-1 filter, 0 when the
+// reference left it (no results are reported for that pair).
+int gta_ref_last_pass(unsigned char* out, size_t n) {
+    if (n != g_pass.size()) return -1;
+    std::memcpy(out, g_pass.data(), n);
+    return 0;
 }
 
 }  // extern "C"

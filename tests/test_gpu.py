@@ -108,17 +108,21 @@ def test_stage_matches_oracle(case, prec):
 
 @pytest.mark.parametrize("path", GOLDEN, ids=[os.path.basename(p) for p in GOLDEN])
 def test_stage_matches_reference_golden(path):
-    """Against the reference's own CPU outputs stored as fixtures."""
+    """Against the reference's own CPU outputs stored as fixtures (pairs of
+    references the stage-1 filter drops are not reported by the reference;
+    here they carry no alignment)."""
     from tests.test_golden import load
     q, r, d = load(path)
     g = run_gpu(q, r)
+    passed = d["passed"].astype(bool)
     assert np.array_equal(g.alns, d["alns"])
     for f in INT_FIELDS:
-        assert np.array_equal(g.alndata[..., f], d["alndata"][..., f]), f
-    assert np.abs(g.alndata[..., 9:11] - d["alndata"][..., 9:11]).max() < 1e-4
-    assert np.abs(g.alndata[..., 8] - d["alndata"][..., 8]).max() < 1e-3
-    aligned = d["alndata"][..., 8] < 9999
+        assert np.array_equal(g.alndata[..., f][passed], d["alndata"][..., f][passed]), f
+    assert np.abs(g.alndata[..., 9:11] - d["alndata"][..., 9:11])[passed].max() < 1e-4
+    assert np.abs(g.alndata[..., 8] - d["alndata"][..., 8])[passed].max() < 1e-3
+    aligned = passed & (d["alndata"][..., 8] < 9999)
     assert np.abs(g.tfms - d["tfms"])[aligned].max() < 1e-3
+    assert (g.alndata[~passed][..., 8] >= 9999).all()
 
 
 def check_fp64(g, q, r, prec):
@@ -427,6 +431,22 @@ def test_dp_grid_bit_exact(lq, usess):
                     d02 = float(oracle.d0(int(q.len[i]), int(r.len[j]))["d0"]) ** 2
                     _, want = oracle.dp(q.coords(i), r.coords(j), tf[i, j], d02, gapopen, gap0)
                     assert np.array_equal(got[0][i, j], want), (gap0, i, j, int(q.len[i]), int(r.len[j]))
+
+
+@pytest.mark.parametrize("qgroup", [0, 1, 2], ids=["one-block", "per-query", "blocks-of-2"])
+def test_reference_filter_blocks(qgroup):
+    """The reference filter after the stage-1 prescreen: a reference that one
+    query of a block keeps continues with every query of the block (flags
+    reset), one that no query keeps is dropped (no alignment).  Equal to the
+    oracle for each block size, and the block size changes the outcome."""
+    qc, rc, _ = synth.config(1, scale=0.002)
+    q = qc.subset([0, 1, 2, 3])
+    o = oracle.process_block(q, rc, default_options(qgroup=qgroup))
+    g = run_gpu(q, rc, qgroup=qgroup)
+    assert_same(o, g, q, rc)
+    if qgroup == 0:
+        own = oracle.process_block(q, rc, default_options(qgroup=1))
+        assert (o.passed & ~own.passed).any()
 
 
 @pytest.mark.parametrize("grid", [2, 1], ids=["W-waves", "grid"])
