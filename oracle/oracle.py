@@ -142,9 +142,9 @@ def kept_first(q, r, opts=None):
     keeps come before those it drops: kept references longer than a length
     split and dropped ones not longer, in descending length, the split chosen
     to keep the most aligned references.  Kept references then keep their
-    indices when the reference packs the block, so each kept pair's results
-    are the reference's for that pair alone (packing moves the per-pair
-    records but not every per-slot buffer, MpReform.cpp:57-124)."""
+    indices when the reference packs the block, which leaves fewer per-slot
+    buffers behind at a kept pair's index (packing moves the per-pair records
+    but not the per-slot buffers, MpReform.cpp:57-124)."""
     res = ref_process_block(q, r, opts, nthreads=1)
     kept = res.passed.any(axis=0)
     aligned = (res.passed & (res.alndata[..., 8] < 9999)).any(axis=0)
