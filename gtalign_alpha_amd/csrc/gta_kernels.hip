@@ -1254,16 +1254,33 @@ struct RefineParams {
     float prescore;        // > 0 with prescreen at the last iteration
 };
 
+// The aligned pairs' coordinates, read one position at a time by the passes:
+// six planes in HBM (Aln), or staged into LDS position-major, the six floats
+// of position i at 6i (AlnL: three 8-byte LDS loads from one 32-bit address).
 struct Aln {
     const float* qx; const float* qy; const float* qz;
     const float* rx; const float* ry; const float* rz;
+    __device__ __forceinline__ void load(int i, float& x0, float& y0, float& z0, float& x1, float& y1,
+                                         float& z1) const {
+        x0 = qx[i]; y0 = qy[i]; z0 = qz[i]; x1 = rx[i]; y1 = ry[i]; z1 = rz[i];
+    }
+};
+typedef float lds_f2 __attribute__((ext_vector_type(2)));
+typedef const __attribute__((address_space(3))) lds_f2* lds_cf2ptr;
+struct AlnL {
+    lds_cf2ptr p;
+    __device__ __forceinline__ void load(int i, float& x0, float& y0, float& z0, float& x1, float& y1,
+                                         float& z1) const {
+        const lds_f2 a = p[3 * i], b = p[3 * i + 1], c = p[3 * i + 2];
+        x0 = a.x; y0 = a.y; z0 = b.x; x1 = b.y; y1 = c.x; z1 = c.y;
+    }
 };
 
 // forward alignment position f is stored at L-1-f
 // covariance of the fragment [fragpos, fragpos+fl) of the alignment (with the
 // sums of squares when sq is given, for the RMSD)
-template <int P>
-__device__ __forceinline__ void gcc_dp_frag(const Aln& a, int L, int fragpos, int fl,
+template <int P, class A>
+__device__ __forceinline__ void gcc_dp_frag(const A& a, int L, int fragpos, int fl,
                                             typename GPrec<P>::T* cc, typename GPrec<P>::T* sq) {
     int ql = L, rl = L, qp = fragpos, rp = fragpos;
     UpdateLengths(ql, rl, qp, rp, fl);
@@ -1273,7 +1290,8 @@ __device__ __forceinline__ void gcc_dp_frag(const Aln& a, int L, int fragpos, in
     T s2[6] = {0, 0, 0, 0, 0, 0};
     for (int k = glane(); k < fl; k += GL) {
         const int idx = L - 1 - (rp + k);
-        const float qx = a.qx[idx], qy = a.qy[idx], qz = a.qz[idx], rx = a.rx[idx], ry = a.ry[idx], rz = a.rz[idx];
+        float qx, qy, qz, rx, ry, rz;
+        a.load(idx, qx, qy, qz, rx, ry, rz);
         acc.add(qx, qy, qz, rx, ry, rz);
         if (sq) {
             // squares formed in T: exact for T = double
@@ -1288,15 +1306,16 @@ __device__ __forceinline__ void gcc_dp_frag(const Aln& a, int L, int fragpos, in
     }
 }
 // covariance of the aligned pairs within thr under tprev; returns the count
-template <int P>
-__device__ __forceinline__ int gcc_dp_ext(const Aln& a, int L, const float* tprev, float thr,
+template <int P, class A>
+__device__ __forceinline__ int gcc_dp_ext(const A& a, int L, const float* tprev, float thr,
                                           typename GPrec<P>::T* cc) {
     typename GPrec<P>::Acc acc;
     acc.clear();
     int cnt = 0;
     for (int f = glane(); f < L; f += GL) {
         const int idx = L - 1 - f;
-        const float qx = a.qx[idx], qy = a.qy[idx], qz = a.qz[idx], rx = a.rx[idx], ry = a.ry[idx], rz = a.rz[idx];
+        float qx, qy, qz, rx, ry, rz;
+        a.load(idx, qx, qy, qz, rx, ry, rz);
         if (thr < tfm_dist2(tprev, qx, qy, qz, rx, ry, rz)) continue;
         acc.add(qx, qy, qz, rx, ry, rz);
         cnt++;
@@ -1304,8 +1323,8 @@ __device__ __forceinline__ int gcc_dp_ext(const Aln& a, int L, const float* tpre
     acc.reduce(cc);
     return gsum(cnt);
 }
-template <int P>
-__device__ __forceinline__ float gscore_dp(const Aln& a, int L, const float* tfm, float d0, float d02, float d82,
+template <int P, class A>
+__device__ __forceinline__ float gscore_dp(const A& a, int L, const float* tfm, float d0, float d02, float d82,
                                            bool chk, bool want_thr, float* thr) {
     GScore<P> sc;
     sc.clear();
@@ -1313,7 +1332,9 @@ __device__ __forceinline__ float gscore_dp(const Aln& a, int L, const float* tfm
     m3.init();
     for (int f = glane(); f < L; f += GL) {
         const int idx = L - 1 - f;
-        const float d = tfm_dist2(tfm, a.qx[idx], a.qy[idx], a.qz[idx], a.rx[idx], a.ry[idx], a.rz[idx]);
+        float qx, qy, qz, rx, ry, rz;
+        a.load(idx, qx, qy, qz, rx, ry, rz);
+        const float d = tfm_dist2(tfm, qx, qy, qz, rx, ry, rz);
         if (!chk || d <= d82) sc.add(pair_score(d02, d));
         m3.put(d);
     }
@@ -1328,8 +1349,8 @@ __device__ __forceinline__ float gscore_dp(const Aln& a, int L, const float* tfm
 // order and reductions are those of the two separate passes, so both results
 // are bitwise the same.  Returns the score; cc and *ncc receive the
 // covariance and its count.
-template <int P>
-__device__ __forceinline__ float gscore_cc_dp(const Aln& a, int L, const float* tfm, float d02, float d82, bool chk,
+template <int P, class A>
+__device__ __forceinline__ float gscore_cc_dp(const A& a, int L, const float* tfm, float d02, float d82, bool chk,
                                               float thrcc, typename GPrec<P>::T* cc, int* ncc) {
     GScore<P> sc;
     sc.clear();
@@ -1338,7 +1359,8 @@ __device__ __forceinline__ float gscore_cc_dp(const Aln& a, int L, const float* 
     int cnt = 0;
     for (int f = glane(); f < L; f += GL) {
         const int idx = L - 1 - f;
-        const float qx = a.qx[idx], qy = a.qy[idx], qz = a.qz[idx], rx = a.rx[idx], ry = a.ry[idx], rz = a.rz[idx];
+        float qx, qy, qz, rx, ry, rz;
+        a.load(idx, qx, qy, qz, rx, ry, rz);
         const float d = tfm_dist2(tfm, qx, qy, qz, rx, ry, rz);
         if (!chk || d <= d82) sc.add(pair_score(d02, d));
         if (!(thrcc < d)) {
@@ -1432,19 +1454,15 @@ __device__ void refine_pair(const DevChunk& Q, const DevChunk& R, const DevState
 
     const int L = skip ? 0 : S.nalnposs[p];
     const size_t base = (size_t)qi * (size_t)R.nposs + (size_t)R.dst[ri];
-    Aln a;
     const float* g = S.aln + base;
     const size_t pl = S.alnplane;
-    if (!skip && L <= ldscap) {
+    const bool staged = !skip && L <= ldscap;
+    if (staged) {
         for (int k = threadIdx.x; k < L; k += RTHREADS) {
-            lds_rf[k] = g[k]; lds_rf[ldscap + k] = g[pl + k]; lds_rf[2 * ldscap + k] = g[2 * pl + k];
-            lds_rf[3 * ldscap + k] = g[3 * pl + k]; lds_rf[4 * ldscap + k] = g[4 * pl + k];
-            lds_rf[5 * ldscap + k] = g[5 * pl + k];
+            float* d = lds_rf + 6 * k;
+            d[0] = g[k]; d[1] = g[pl + k]; d[2] = g[2 * pl + k];
+            d[3] = g[3 * pl + k]; d[4] = g[4 * pl + k]; d[5] = g[5 * pl + k];
         }
-        a = Aln{lds_rf, lds_rf + ldscap, lds_rf + 2 * ldscap, lds_rf + 3 * ldscap, lds_rf + 4 * ldscap,
-                lds_rf + 5 * ldscap};
-    } else {
-        a = Aln{g, g + pl, g + 2 * pl, g + 3 * pl, g + 4 * pl, g + 5 * pl};
     }
     __syncthreads();
 
@@ -1478,6 +1496,8 @@ __device__ void refine_pair(const DevChunk& Q, const DevChunk& R, const DevState
     T* last = tlast[grp];       // previous covariance record (15 sums, count)
     T* next = tnext[grp];       // covariance of the next iteration, when fused
     bool havenext = false;
+    // the slot loop, instantiated for planes in LDS and for planes in HBM
+    auto slots = [&](const auto& a) {
     for (;;) {
         for (;;) {
             const int got = fetch_slot(need, &ctr);
@@ -1593,6 +1613,12 @@ __device__ void refine_pair(const DevChunk& Q, const DevChunk& R, const DevState
                 need = true;
             }
         }
+    }
+    };
+    if (staged) {
+        slots(AlnL{(lds_cf2ptr)lds_rf});
+    } else {
+        slots(Aln{g, g + pl, g + 2 * pl, g + 3 * pl, g + 4 * pl, g + 5 * pl});
     }
     if (glane() == 0 && ab) atomicAdd(S.algbytes + (MODE == RM_DPREFINE ? KC_REFINE_DP : KC_REFINE_FIN), ab);
     SlotBest b;
