@@ -193,10 +193,20 @@ __global__ void k_keep_refs(DevChunk Q, DevChunk R, DevState S, int qgroup) {
 
 // ============================================================================
 // gapless passes over structure coordinates (stage 1), in slot groups
+// One structure's coordinate planes from a workgroup-uniform base: position
+// i is read at a 32-bit byte offset from the base (kept in scalar registers),
+// which compiles to a global load with a scalar base and a 32-bit vector
+// offset instead of per-lane 64-bit address arithmetic.
 struct Coords {
     const float* x;
     const float* y;
     const float* z;
+    __device__ __forceinline__ static float at(const float* b, int i) {
+        return *(const float*)((const char*)b + ((unsigned)i << 2));
+    }
+    __device__ __forceinline__ void load(int i, float& px, float& py, float& pz) const {
+        px = at(x, i); py = at(y, i); pz = at(z, i);
+    }
 };
 
 __device__ __forceinline__ void copy12(float* d, const float* s) {
@@ -257,7 +267,9 @@ __device__ __forceinline__ int gcc_gapless(const Coords& q, const Coords& r, int
     int cnt = 0;
     for (int k = glane<G>(); k < n; k += G) {
         const int a = qp + k, b = rp + k;
-        const float qx = q.x[a], qy = q.y[a], qz = q.z[a], rx = r.x[b], ry = r.y[b], rz = r.z[b];
+        float qx, qy, qz, rx, ry, rz;
+        q.load(a, qx, qy, qz);
+        r.load(b, rx, ry, rz);
         if (usethr && thr < tfm_dist2(tfm, qx, qy, qz, rx, ry, rz)) continue;
         acc.add(qx, qy, qz, rx, ry, rz);
         cnt++;
@@ -278,7 +290,10 @@ __device__ __forceinline__ float gscore_seed(const Coords& q, const Coords& r, i
     m3.init();
     for (int k = glane<G>(); k < n; k += G) {
         const int a = qp + k, b = rp + k;
-        const float d = tfm_dist2(tfm, q.x[a], q.y[a], q.z[a], r.x[b], r.y[b], r.z[b]);
+        float qx, qy, qz, rx, ry, rz;
+        q.load(a, qx, qy, qz);
+        r.load(b, rx, ry, rz);
+        const float d = tfm_dist2(tfm, qx, qy, qz, rx, ry, rz);
         sc.add(pair_score(d02, d));
         m3.put(d);
     }
@@ -316,7 +331,10 @@ __device__ __forceinline__ float gscore_gapless(const Coords& q, const Coords& r
     m3.init();
     for (int k = glane(); k < n; k += GL) {
         const int a = qp + k, b = rp + k;
-        const float d = tfm_dist2(tfm, q.x[a], q.y[a], q.z[a], r.x[b], r.y[b], r.z[b]);
+        float qx, qy, qz, rx, ry, rz;
+        q.load(a, qx, qy, qz);
+        r.load(b, rx, ry, rz);
+        const float d = tfm_dist2(tfm, qx, qy, qz, rx, ry, rz);
         if (d <= d82) sc.add(pair_score(d02, d));
         m3.put(d);
     }
@@ -339,7 +357,9 @@ __device__ __forceinline__ float gscore_cc_gapless(const Coords& q, const Coords
     int cnt = 0;
     for (int k = glane(); k < n; k += GL) {
         const int a = qp + k, b = rp + k;
-        const float qx = q.x[a], qy = q.y[a], qz = q.z[a], rx = r.x[b], ry = r.y[b], rz = r.z[b];
+        float qx, qy, qz, rx, ry, rz;
+        q.load(a, qx, qy, qz);
+        r.load(b, rx, ry, rz);
         const float d = tfm_dist2(tfm, qx, qy, qz, rx, ry, rz);
         if (d <= d82) sc.add(pair_score(d02, d));
         if (!(thrcc < d)) {
@@ -378,7 +398,8 @@ __global__ __launch_bounds__(RTHREADS) void k_findfrag(DevChunk Q, DevChunk R, D
     const int p = plist ? plist[blockIdx.x] : (int)blockIdx.x;
     const int qi = pair_q(S, p), ri = pair_r(S, p);
     const int qlen = Q.len[qi], rlen = R.len[ri];
-    const int qd = Q.dst[qi], rd = R.dst[ri];
+    // workgroup-uniform structure offsets (scalar registers)
+    const int qd = __builtin_amdgcn_readfirstlane(Q.dst[qi]), rd = __builtin_amdgcn_readfirstlane(R.dst[ri]);
     if (S.flags[p] & CONV_LOWTMSC) {
         if (threadIdx.x == 0 && !parts) S.best0[p] = 0.f;
         return;
@@ -484,7 +505,8 @@ __global__ __launch_bounds__(RTHREADS) void k_refine_gapless(DevChunk Q, DevChun
     const int p = plist ? plist[blockIdx.x] : (int)blockIdx.x;
     const int qi = pair_q(S, p), ri = pair_r(S, p);
     const int qlen = Q.len[qi], rlen = R.len[ri];
-    const int qd = Q.dst[qi], rd = R.dst[ri];
+    // workgroup-uniform structure offsets (scalar registers)
+    const int qd = __builtin_amdgcn_readfirstlane(Q.dst[qi]), rd = __builtin_amdgcn_readfirstlane(R.dst[ri]);
     if (S.flags[p] & CONV_LOWTMSC) {
         if (threadIdx.x == 0 && !parts) S.best0[p] = 0.f;
         return;
