@@ -75,6 +75,8 @@ def lib():
         L.gta_test_set_dp_grid.restype = C.c_int
         L.gta_test_set_split.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
         L.gta_test_set_split.restype = C.c_int
+        L.gta_test_pair_score.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
+        L.gta_test_pair_score.restype = C.c_int
         _lib = L
     return _lib
 
@@ -142,6 +144,15 @@ class GtaBatch:
         n = lib().gta_kernel_stats(self._h, arr, 16)
         return [{"name": arr[i].name.decode(), "launches": arr[i].launches,
                  "seconds": arr[i].seconds, "alg_bytes": arr[i].alg_bytes} for i in range(n)]
+
+    def pair_score(self, d02, d2):
+        """The device's pair score d02 / (d02 + d2) for arrays of operands
+        (testing hook, include/gta/gtalign_amd_testing.h)."""
+        a = np.ascontiguousarray(d02, np.float32)
+        b = np.ascontiguousarray(d2, np.float32)
+        out = np.empty_like(a)
+        self._check(lib().gta_test_pair_score(self._h, a.ctypes.data, b.ctypes.data, a.size, out.ctypes.data))
+        return out
 
     def dp_under(self, tfms, gap0=True, gapopen=0.0, usess=False, dpscan=False):
         """The DP kernel alone under the given per-pair transforms (testing hook,

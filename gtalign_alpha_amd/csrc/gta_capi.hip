@@ -987,6 +987,25 @@ int gta_test_set_dp_grid(gta_ctx* c, int mode) {
     return GTA_OK;
 }
 
+int gta_test_pair_score(gta_ctx* c, const float* d02, const float* d2, int n, float* out) {
+    if (!c || !d02 || !d2 || !out || n < 0) return fail(c, GTA_ERR_ARG, "invalid argument");
+    HIPCHK(c, hipSetDevice(c->device));
+    DevBuf b;
+    const size_t sz = (size_t)n * sizeof(float);
+    hipError_t e = b.ensure(3 * sz);
+    if (e != hipSuccess) return fail(c, GTA_ERR_HIP, hipGetErrorString(e));
+    float* a = b.as<float>();
+    int rc = GTA_OK;
+    if ((e = hipMemcpy(a, d02, sz, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(a + n, d2, sz, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = launch_pair_score(a, a + n, n, a + 2 * (size_t)n, c->stream)) != hipSuccess ||
+        (e = hipStreamSynchronize(c->stream)) != hipSuccess ||
+        (e = hipMemcpy(out, a + 2 * (size_t)n, sz, hipMemcpyDeviceToHost)) != hipSuccess)
+        rc = fail(c, GTA_ERR_HIP, hipGetErrorString(e));
+    b.release();
+    return rc;
+}
+
 int gta_test_set_split(gta_ctx* c, int min_len, int seeds_per_part, int slots_per_part) {
     if (!c || min_len < 1 || seeds_per_part < 1 || slots_per_part < 1)
         return fail(c, GTA_ERR_ARG, "invalid split parameters");

@@ -1951,6 +1951,15 @@ hipError_t launch_secstr(const DevChunk& C, hipStream_t st) {
     hipLaunchKernelGGL(k_secstr, dim3(blocks_for(C.nposs, 256)), dim3(256), 0, st, C);
     return hipGetLastError();
 }
+// pair_score over caller-given operands (tests of the division)
+__global__ void k_pair_score(const float* d02, const float* d2, int n, float* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = pair_score(d02[i], d2[i]);
+}
+hipError_t launch_pair_score(const float* d02, const float* d2, int n, float* out, hipStream_t st) {
+    if (n > 0) hipLaunchKernelGGL(k_pair_score, dim3(blocks_for(n, 256)), dim3(256), 0, st, d02, d2, n, out);
+    return hipGetLastError();
+}
 hipError_t launch_keep_refs(const DevChunk& Q, const DevChunk& R, const DevState& S, int qgroup,
                             hipStream_t st) {
     const int g = (qgroup > 0 && qgroup < S.nq) ? qgroup : S.nq;

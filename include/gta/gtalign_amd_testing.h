@@ -68,6 +68,12 @@ int gta_test_dp_forms(const gta_ctx* ctx, int* counts, double* tune_seconds);
  * gta_process. */
 int gta_test_set_split(gta_ctx* ctx, int min_len, int seeds_per_part, int slots_per_part);
 
+/* The pair score d0^2 / (d0^2 + d^2) of the DP cells and TM-score passes
+ * (scoringbase.h:61-65) for n caller-given operand pairs: out[i] receives
+ * the device's value for d02[i], d2[i] (its division must round as IEEE
+ * division does). */
+int gta_test_pair_score(gta_ctx* ctx, const float* d02, const float* d2, int n, float* out);
+
 #ifdef __cplusplus
 }
 #endif

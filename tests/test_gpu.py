@@ -346,6 +346,44 @@ def test_stage_search_by_ss_matches_oracle(form):
         assert not np.array_equal(base.alndata, o.alndata)
 
 
+def test_pair_score_division_is_ieee():
+    """The DP cells and TM-score passes divide with a shortened
+    Newton-Raphson sequence; over the operand ranges of the stage (d0^2 of
+    every length, search, DP-scan and final d0, distances from 0 to 1e11 A^2,
+    including exact powers of two and their neighbours) it must round as IEEE
+    division does: bit for bit equal to numpy's float32 d02 / (d02 + d2)."""
+    rng = np.random.default_rng(5)
+    d0 = []
+    for L in range(1, 65536):
+        d = oracle.d0(L, L)
+        d0 += [float(d["d0"]), float(d["d0"]) + 1.5]
+    d02 = np.square(np.array(d0, np.float32)).astype(np.float32)
+    d02 = np.unique(np.concatenate([d02, np.float32([0.9, 1.0, 2.0 ** 11, 2400.0])]))
+    n = 1 << 22
+    a = rng.choice(d02, n).astype(np.float32)
+    e = rng.uniform(-30.0, 36.5, n)
+    b = np.exp2(e).astype(np.float32)
+    b[: n // 8] = rng.uniform(0.0, 64.0, n // 8).astype(np.float32)
+    k = np.arange(-24, 37, dtype=np.float64)
+    pw = np.exp2(k).astype(np.float32)
+    special = np.concatenate([pw, np.nextafter(pw, np.float32(0)), np.nextafter(pw, np.float32(np.inf)),
+                              np.float32([0.0])]).astype(np.float32)
+    b[n // 8: n // 8 + special.size * 64] = np.tile(special, 64)
+    # denominators that are exact powers of two and their neighbours
+    den_targets = np.tile(np.concatenate([pw, np.nextafter(pw, np.float32(np.inf))]), 32)
+    m = den_targets.size
+    a2 = rng.choice(d02[d02 < 256], m).astype(np.float32)
+    ok = den_targets > a2
+    b2 = (den_targets[ok] - a2[ok]).astype(np.float32)
+    a = np.concatenate([a, a2[ok]])
+    b = np.concatenate([b, b2])
+    with GtaBatch(0, default_options(), max_qrys=1, max_qry_poss=8, max_refs=1, max_ref_poss=8) as bt:
+        got = bt.pair_score(a, b)
+    want = (a / (a + b).astype(np.float32)).astype(np.float32)
+    bad = np.flatnonzero(got.view(np.uint32) != want.view(np.uint32))
+    assert bad.size == 0, (bad.size, a[bad[:3]], b[bad[:3]], got[bad[:3]], want[bad[:3]])
+
+
 def test_dp_kernel_long_reference():
     """The DP with its boundary row in HBM (reference > 3276 residues)."""
     q, r, tf = _dp_workload(9, 2, 2, 100, 200)
