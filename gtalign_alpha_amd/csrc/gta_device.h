@@ -164,24 +164,29 @@ __device__ __forceinline__ int wave_sum_i(int v) {
 }
 
 // three smallest values of a multiset, kept sorted per lane
+// (a <= b <= c: the lowest three of {a, b, c, d} by a min/max network, the
+// same triple as sorted insertion, without branches)
 struct Min3 {
     float a, b, c;
     __device__ void init() { a = b = c = CP_LARGEDST; }
-    __device__ void put(float d) {
-        if (d < a) { c = b; b = a; a = d; }
-        else if (d < b) { c = b; b = d; }
-        else if (d < c) c = d;
+    __device__ __forceinline__ void put(float d) {
+        const float ha = fmaxf(a, d), hb = fmaxf(b, d);
+        c = fminf(c, hb);
+        b = fminf(b, ha);
+        a = fminf(a, d);
     }
 };
-// three smallest distinct values
+// three smallest distinct values (a value already held changes nothing: it
+// enters as c, which leaves the triple as it is)
 struct Min3d {
     float a, b, c;
     __device__ void init() { a = b = c = CP_LARGEDST; }
-    __device__ void put(float d) {
-        if (d == a || d == b || d == c) return;
-        if (d < a) { c = b; b = a; a = d; }
-        else if (d < b) { c = b; b = d; }
-        else if (d < c) c = d;
+    __device__ __forceinline__ void put(float d) {
+        if (d == a || d == b || d == c) d = c;
+        const float ha = fmaxf(a, d), hb = fmaxf(b, d);
+        c = fminf(c, hb);
+        b = fminf(b, ha);
+        a = fminf(a, d);
     }
 };
 

@@ -288,6 +288,8 @@ __device__ __forceinline__ float gscore_seed(const Coords& q, const Coords& r, i
     sc.clear();
     Min3d m3;
     m3.init();
+    // distances are tracked only when some group of the wave wants a threshold
+    const bool track = __any(want_thr);
     for (int k = glane<G>(); k < n; k += G) {
         const int a = qp + k, b = rp + k;
         float qx, qy, qz, rx, ry, rz;
@@ -295,7 +297,7 @@ __device__ __forceinline__ float gscore_seed(const Coords& q, const Coords& r, i
         r.load(b, rx, ry, rz);
         const float d = tfm_dist2(tfm, qx, qy, qz, rx, ry, rz);
         sc.add(pair_score(d02, d));
-        m3.put(d);
+        if (track) m3.put(d);
     }
     const float s = sc.template reduce<G>();
     if (want_thr) {
@@ -329,6 +331,7 @@ __device__ __forceinline__ float gscore_gapless(const Coords& q, const Coords& r
     sc.clear();
     Min3 m3;
     m3.init();
+    const bool track = __any(want_thr);
     for (int k = glane(); k < n; k += GL) {
         const int a = qp + k, b = rp + k;
         float qx, qy, qz, rx, ry, rz;
@@ -336,7 +339,7 @@ __device__ __forceinline__ float gscore_gapless(const Coords& q, const Coords& r
         r.load(b, rx, ry, rz);
         const float d = tfm_dist2(tfm, qx, qy, qz, rx, ry, rz);
         if (d <= d82) sc.add(pair_score(d02, d));
-        m3.put(d);
+        if (track) m3.put(d);
     }
     const float s = sc.reduce();
     if (want_thr) *thr = refined_threshold(gthird(m3), d0, n);
@@ -1352,13 +1355,14 @@ __device__ __forceinline__ float gscore_dp(const A& a, int L, const float* tfm, 
     sc.clear();
     Min3 m3;
     m3.init();
+    const bool track = __any(want_thr);
     for (int f = glane(); f < L; f += GL) {
         const int idx = L - 1 - f;
         float qx, qy, qz, rx, ry, rz;
         a.load(idx, qx, qy, qz, rx, ry, rz);
         const float d = tfm_dist2(tfm, qx, qy, qz, rx, ry, rz);
         if (!chk || d <= d82) sc.add(pair_score(d02, d));
-        m3.put(d);
+        if (track) m3.put(d);
     }
     const float s = sc.reduce();
     if (want_thr) *thr = refined_threshold(gthird(m3), d0, L);
