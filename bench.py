@@ -70,12 +70,12 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def time_form(qc, rc, prec, steps, warmup, profile, batched=0, tune=None):
+def time_form(qc, rc, prec, steps, warmup, profile, batched=0, tune=None, **opt):
     """Times `steps` calls after `warmup` untimed ones.  With tune (a dict),
     DP launch-form autotuning is on with a fresh cache file: the tuning runs
     inside the first warmup call, and tune receives the seconds it took and
     the pairs per chosen form."""
-    opts = default_options(precision=prec, batched=batched, qgroup=QGROUP)
+    opts = default_options(precision=prec, batched=batched, qgroup=QGROUP, **opt)
     with GtaBatch(0, opts, max_qrys=qc.nstrs, max_qry_poss=qc.nposs, max_refs=rc.nstrs,
                   max_ref_poss=rc.nposs) as b:
         b.TransferQueryPMDataToDevice(qc)
@@ -386,6 +386,9 @@ def main():
     ap.add_argument("--scale", type=float, default=1.0,
                     help="fraction of the 50,000 reference chains (default: the full workload)")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--search-by-ss", action="store_true",
+                    help="also time the same workload with --add-search-by-ss (off by default so that the "
+                         "kernel-trace averages of the default run cover the default options only)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-stress", action="store_true", help="skip the precision-stress section")
     ap.add_argument("--no-batched", action="store_true",
@@ -427,6 +430,15 @@ def main():
         for st in stats_m:
             if st["launches"]:
                 log("  %-16s launches %4d  %8.3f s" % (st["name"], st["launches"], st["seconds"]))
+
+    ssrr = None
+    if args.search_by_ss:
+        log("plain FP32 form with --add-search-by-ss")
+        wall_s, _, cells_s, stats_s = time_form(qc, rc, PREC_FP32, args.steps, args.warmup, True, addsearchbyss=1)
+        ssrr = {"pairs_per_s": round(npairs * args.steps / wall_s, 1),
+                "ms_per_step": round(1e3 * wall_s / args.steps, 2),
+                "vs_default_options": round(wall / wall_s, 3),
+                "kernel_seconds": {st["name"]: round(st["seconds"], 4) for st in stats_s if st["launches"]}}
 
     for st in stats:
         if st["launches"]:
@@ -491,6 +503,7 @@ def main():
         "dp_gcups": round(cells / dev / 1e9, 2),
         "dp_kernel_gcups": round(cells / args.steps / kern["dp"]["seconds"] / 1e9, 2) if "dp" in kern else None,
         "mixed_precision_pairs_per_s": round(mixed, 1) if mixed else None,
+        "search_by_ss": ssrr,
         "pcie_inclusive": pcie,
         "kernels": kern,
         "roofline": roofline,
