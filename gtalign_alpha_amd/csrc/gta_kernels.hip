@@ -524,6 +524,11 @@ __global__ __launch_bounds__(RTHREADS) void k_refine_gapless(DevChunk Q, DevChun
     const int nall = anyslot ? min(qlen - qpos, rlen - rpos) : 0;
     int lo, hi;
     part_range(0, G.nlocsteps, blockIdx.y, gridDim.y, lo, hi);
+    // slots at fragment positions from nall + SFRAGSTEP on fail the slot test
+    // below for every fragment length >= 1: not fetched at all
+    const int vslots = ((nall + SFRAGSTEP - 1) / SFRAGSTEP + 1) * NMAXSUBFRAGS;
+    lo = min(lo, vslots);
+    hi = min(hi, vslots);
     if (threadIdx.x == 0) ctr = lo;
     __syncthreads();
 
@@ -1507,6 +1512,13 @@ __device__ void refine_pair(const DevChunk& Q, const DevChunk& R, const DevState
     const int base_pos = phase2 ? S.sfragpos[p] : 0;
     int slo, shi;
     part_range(0, nslots, blockIdx.y, gridDim.y, slo, shi);
+    if (!phase2) {
+        // slots at fragment positions from L + SFRAGSTEP on fail the slot
+        // test below for every fragment length >= 1: not fetched at all
+        const int vslots = ((L + SFRAGSTEP - 1) / SFRAGSTEP + 1) * NMAXSUBFRAGS;
+        slo = min(slo, vslots);
+        shi = min(shi, vslots);
+    }
     if (threadIdx.x == 0) ctr = slo;
     __syncthreads();
 
