@@ -854,7 +854,8 @@ __device__ __forceinline__ bool dp_strip(const DevChunk& Q, const DevChunk& R, c
     float val = OUT, upprev = OUT, oblk = 0.f;
     unsigned word = 0;
     int j = -lane;
-    for (int t = 0; t < nsteps; t++) {
+    // one step (anti-diagonal); false when a row wait gave up
+    auto step = [&](const int t) -> bool {
         const float4 rc = cur;
         const float rss = curs;
         if ((t & 63) == 63) {
@@ -933,7 +934,16 @@ __device__ __forceinline__ bool dp_strip(const DevChunk& Q, const DevChunk& R, c
                 word = 0;
             }
         }
+        return true;
+    };
+    // two steps per iteration, so that the prefetched ring entry and the
+    // carried values alternate registers instead of being copied every step
+    int t = 0;
+    for (; t + 1 < nsteps; t += 2) {
+        if (!step(t)) return false;
+        if (!step(t + 1)) return false;
     }
+    if (t < nsteps && !step(t)) return false;
     if (vend) *vend = val;
     return true;
 }
