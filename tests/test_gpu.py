@@ -487,6 +487,25 @@ def test_reference_filter_blocks(qgroup):
         assert (o.passed & ~own.passed).any()
 
 
+def test_filter_blocks_and_search_by_ss_batched_mixed():
+    """The options together: blocks of 2 queries for the filter after stage
+    1, --add-search-by-ss, the batched form (graph replays) in mixed
+    precision; the FP32 plain form with the same options equals the oracle,
+    and the batched mixed form stays within the mixed form's bar of it."""
+    qc, rc, _ = synth.config(1, scale=0.002)
+    q = qc.subset([0, 3, 7, 11])
+    kw = dict(qgroup=2, addsearchbyss=1)
+    o = oracle.process_block(q, rc, default_options(**kw))
+    assert_same(o, run_gpu(q, rc, **kw), q, rc)
+    with GtaBatch(0, default_options(precision=PREC_MIXED, batched=1, **kw), max_qrys=q.nstrs,
+                  max_qry_poss=q.nposs, max_refs=rc.nstrs, max_ref_poss=rc.nposs) as b:
+        b.TransferQueryPMDataToDevice(q)
+        b.TransferCPMDataToDevice(rc)
+        runs = [b.ProcessBlock() for _ in range(2)]
+    assert np.array_equal(runs[0].alndata, runs[1].alndata)
+    assert_close_mixed(o, runs[1], q, rc)
+
+
 @pytest.mark.parametrize("grid", [2, 1], ids=["W-waves", "grid"])
 def test_stage_with_multiwave_dp(grid):
     """The whole stage with every pair of >= 2 strips and >= 256 reference
