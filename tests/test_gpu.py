@@ -726,3 +726,33 @@ def test_structure_length_limit():
         with pytest.raises(GtaError) as e:
             b.TransferCPMDataToDevice(c)
     assert e.value.code == _code("GTA_ERR_LENGTH")
+
+
+def test_stage_on_chunks_read_from_pdb_files(tmp_path):
+    """Chains written as PDB files, read back by the PDB reader into query and
+    reference chunks (residue numbers and chain words included), then the
+    stage with --add-search-by-ss vs the oracle on the same chunks."""
+    from gtalign_alpha_amd import pdb
+    qc, rc, _ = synth.config(1, scale=0.002)
+    rng = np.random.default_rng(11)
+
+    def write(c, idx, tag):
+        paths = []
+        for i in idx:
+            xyz = c.coords(i)
+            three = {v: k for k, v in pdb._RESIDUE_CODES.items() if not k.startswith("D")}
+            lines = ["MODEL        1"]
+            for k, (res, p) in enumerate(zip(c.residues(i), xyz)):
+                lines.append("ATOM  %5d  CA  %3s %1s%4d    %8.3f%8.3f%8.3f  1.00  0.00" %
+                             (k + 1, three.get(res, "UNK"), "A", k + 1, *p))
+            path = tmp_path / ("%s%d.pdb" % (tag, i))
+            path.write_text("\n".join(lines + ["TER", "ENDMDL", "END"]) + "\n")
+            paths.append(str(path))
+        return paths
+
+    q, kq = pdb.read_chunk(write(qc, [0, 5, 9], "q"))
+    r, kr = pdb.read_chunk(write(rc, rng.choice(rc.nstrs, 40, replace=False), "r"))
+    assert len(kq) == 3 and len(kr) >= 30
+    kw = dict(addsearchbyss=1)
+    o = oracle.process_block(q, r, default_options(**kw))
+    assert_same(o, run_gpu(q, r, **kw), q, r)
