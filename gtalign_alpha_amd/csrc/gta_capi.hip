@@ -48,18 +48,18 @@ struct ChunkDev {
     std::vector<int> len;
     std::vector<unsigned> dst;
     std::vector<int> na;
-    DevBuf b_len, b_dst, b_na, b_x, b_y, b_z, b_rsd, b_ss;
+    DevBuf b_len, b_dst, b_na, b_x, b_y, b_z, b_rsd, b_ss, b_napos;
     DevChunk view() const {
         DevChunk c;
         c.nstrs = nstrs; c.nposs = nposs;
         c.len = b_len.as<int>(); c.dst = b_dst.as<unsigned>(); c.na = b_na.as<int>();
         c.x = b_x.as<float>(); c.y = b_y.as<float>(); c.z = b_z.as<float>();
-        c.rsd = b_rsd.as<char>(); c.ss = b_ss.as<char>();
+        c.rsd = b_rsd.as<char>(); c.ss = b_ss.as<char>(); c.napos = b_napos.as<int>();
         return c;
     }
     void release() {
         b_len.release(); b_dst.release(); b_na.release(); b_x.release(); b_y.release();
-        b_z.release(); b_rsd.release(); b_ss.release();
+        b_z.release(); b_rsd.release(); b_ss.release(); b_napos.release();
     }
 };
 
@@ -186,8 +186,6 @@ int gta_create(int device, const gta_options* opts, int max_qrys, int max_qry_po
     gta_default_options(&o);
     if (opts) o = *opts;
     if (!check_options(&o)) return fail(nullptr, GTA_ERR_ARG, "option out of range");
-    if (o.refinement == GTA_REF_FULLA)
-        return fail(nullptr, GTA_ERR_ARG, "refinement by full search over all fragment lengths is not supported");
     if (max_qrys < 1 || max_qry_poss < 1 || max_refs < 1 || max_ref_poss < 1)
         return fail(nullptr, GTA_ERR_ARG, "invalid capacity");
     if (max_qrys > 100) return fail(nullptr, GTA_ERR_TOOMANY, "more than 100 queries per chunk");
@@ -252,6 +250,14 @@ static int upload_chunk(gta_ctx* c, ChunkDev& d, const gta_chunk_view* v, int ma
         tot += v->len[i];
     }
     if (tot != v->nposs) return fail(c, GTA_ERR_ARG, "nposs does not match the lengths");
+    // the pair score's division is exact only for operands far below the
+    // float exponent limit: coordinates must be finite and |x| <= GTA_MAX_COORD
+    for (int i = 0; i < v->nposs; i++) {
+        const float a[3] = {v->x[i], v->y[i], v->z[i]};
+        for (float f : a)
+            if (!(fabsf(f) <= GTA_MAX_COORD))
+                return fail(c, GTA_ERR_ARG, "coordinate not finite or above GTA_MAX_COORD in magnitude");
+    }
     (void)hipSetDevice(c->device);
     d.nstrs = v->nstrs; d.nposs = v->nposs;
     d.len.assign(v->len, v->len + v->nstrs);
@@ -268,6 +274,7 @@ static int upload_chunk(gta_ctx* c, ChunkDev& d, const gta_chunk_view* v, int ma
     HIPCHK(c, d.b_z.ensure(np * 4));
     HIPCHK(c, d.b_rsd.ensure(np));
     HIPCHK(c, d.b_ss.ensure(np));
+    HIPCHK(c, d.b_napos.ensure(np * 4));
     HIPCHK(c, hipMemcpy(d.b_len.p, d.len.data(), ns * 4, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(d.b_dst.p, d.dst.data(), ns * 4, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(d.b_na.p, d.na.data(), ns * 4, hipMemcpyHostToDevice));
@@ -275,6 +282,9 @@ static int upload_chunk(gta_ctx* c, ChunkDev& d, const gta_chunk_view* v, int ma
     HIPCHK(c, hipMemcpy(d.b_y.p, v->y, np * 4, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(d.b_z.p, v->z, np * 4, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(d.b_rsd.p, v->rsd, np, hipMemcpyHostToDevice));
+    // the stage assigns every position's SS itself (k_secstr), as the
+    // reference does (MpSecStr::Run, MpSecStr.h:50-56); blank until then
+    HIPCHK(c, hipMemset(d.b_ss.p, ' ', np));
     return GTA_OK;
 }
 
@@ -591,7 +601,8 @@ static int prepare_state(gta_ctx* c, DevState& S, size_t& alnsize) {
             const Geom G = make_geom(c);
             const int maxparts = std::max({(G.nseeds + c->split_seeds - 1) / c->split_seeds,
                                            (G.nlocsteps + c->split_slots - 1) / c->split_slots,
-                                           (G.maxnfragfcts + c->split_slots - 1) / c->split_slots, 1});
+                                           (G.maxnfragfcts * NMAXSUBFRAGS + c->split_slots - 1) / c->split_slots,
+                                           1});
             HIPCHK(c, c->s_rsmall.ensure(std::max<size_t>(small.size(), 1) * 4));
             HIPCHK(c, c->s_rbig.ensure(big.size() * 4));
             HIPCHK(c, c->s_parts.ensure(big.size() * (size_t)maxparts * part_record_bytes()));
@@ -781,16 +792,22 @@ static int run_stage(gta_ctx* c, double* seconds, double* cells) {
         // final stage (MpStageFin.h:66-90)
         GTA_TIMED(KC_DP, dp(DPM_GRAND_L, 1, 0.0f, 0, 0));
         GTA_TIMED(KC_REFINE_FIN, refine(RM_FIN1_L, 0, 0, 0, 0.f));
+        // MpStageFin::Refine (MpStageFin.h:337-392)
         if (!(o.depth == GTA_DEPTH_SHALLOW || o.refinement == GTA_REF_COARSE))
-            GTA_TIMED(KC_REFINE_FIN, refine(RM_FIN2_L, 0, 0, 0, 0.f));
+            GTA_TIMED(KC_REFINE_FIN, refine(o.refinement == GTA_REF_FULLA ? RM_FIN2F_L : RM_FIN2_L, 0, 0, 0, 0.f));
         GTA_TIMED(KC_DP, dp(DPM_CONSTRAINED_L, 1, 0.0f, 0, 0));
         GTA_TIMED(KC_PROD_ALN, launch_prod_aln(Q, R, S, c->s_alns.as<char>(), o.nodeletions,
                                                listc ? queue(listc, ltot + 2) : queue(lista, ltot),
                                                pgrid_dp, st));
         GTA_TIMED(KC_REFINE_FIN, refine(RM_PROD1_L, 0, 0, 0, 0.f));
         if (prec) GTA_TIMED(KC_REFINE_FIN, launch_rmsd_fp64(Q, R, S, st));
+        // MpStageFin::ProduceOutputScores (MpStageFin.h:398-460)
         if (o.refinement == GTA_REF_ONE || o.refinement == GTA_REF_COARSE)
             GTA_TIMED(KC_REFINE_FIN, refine(RM_PROD2_L, 0, 0, 0, 0.f));
+        else if (o.refinement == GTA_REF_LOG)
+            GTA_TIMED(KC_REFINE_FIN, refine(RM_PROD2L_L, 0, 0, 0, 0.f));
+        else
+            GTA_TIMED(KC_REFINE_FIN, refine(RM_PROD2F_L, 0, 0, 0, 0.f));
         if (o.referenced) GTA_TIMED(KC_MISC, launch_revert(S, st));
         return GTA_OK;
     };
@@ -941,6 +958,7 @@ int gta_test_set_dp_buckets(gta_ctx* c, long long min_cells, int min_strips) {
     if (!c || min_cells < 0 || min_strips < 1) return fail(c, GTA_ERR_ARG, "invalid DP bucket thresholds");
     c->mw_min_cells = min_cells;
     c->mw_min_strips = min_strips;
+    c->gvalid = false;   // the captured graph holds the old launch forms and buffers
     return GTA_OK;
 }
 
@@ -984,6 +1002,7 @@ int gta_test_dp_forms(const gta_ctx* c, int* counts, double* tune_seconds) {
 int gta_test_set_dp_grid(gta_ctx* c, int mode) {
     if (!c || mode < 0 || mode > 2) return fail(c, GTA_ERR_ARG, "invalid DP grid mode");
     c->dp_grid = mode;
+    c->gvalid = false;   // the captured graph holds the old launch forms and buffers
     return GTA_OK;
 }
 
@@ -1012,6 +1031,7 @@ int gta_test_set_split(gta_ctx* c, int min_len, int seeds_per_part, int slots_pe
     c->split_min_len = min_len;
     c->split_seeds = seeds_per_part;
     c->split_slots = slots_per_part;
+    c->gvalid = false;   // the captured graph holds the old split lists and buffers
     return GTA_OK;
 }
 

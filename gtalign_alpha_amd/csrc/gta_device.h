@@ -45,6 +45,7 @@ struct DevChunk {
     const float* z;
     const char* rsd;
     char* ss;
+    int* napos;             // per position: nucleic-acid pairing partner (scratch of k_secstr)
 };
 
 struct Geom {

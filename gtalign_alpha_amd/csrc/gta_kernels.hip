@@ -132,7 +132,29 @@ __global__ void k_secstr(DevChunk C) {
     const int L = C.len[s];
     const int D = (int)C.dst[s];
     const int j = g - D;
-    if (C.na[s]) return;
+    if (C.na[s]) {
+        // nucleic acid: the pairing partner of residue j (CalcDistances_NASS,
+        // sskna.cu:135-270; CPU twin MpSecStr.cpp:253-360; C3' bounds
+        // 12.5-15 A, PM2DVectorFields.h:178-206): the complementary residue x
+        // (|x - j| > 2) whose distance deviates least from 13.75 A, within
+        // 1.25 A, the lowest x on exact ties; k_secstr_na marks the pairs
+        const char a = C.rsd[D + j];
+        const float xj = C.x[D + j], yj = C.y[D + j], zj = C.z[D + j];
+        const float avg = 13.75f, dlt = 1.25f;
+        float best = 9999.9f;
+        int bx = -1;
+        for (int x = 0; x < L; x++) {
+            const char b = C.rsd[D + x];
+            const bool pr = ((a == 'T' || a == 'U') && b == 'A') || (a == 'A' && (b == 'T' || b == 'U')) ||
+                            (a == 'G' && (b == 'C' || b == 'U')) || ((a == 'C' || a == 'U') && b == 'G');
+            if (!pr || abs(x - j) <= 2) continue;
+            const float dev = fabsf(sqrtf(dist2(xj, yj, zj, C.x[D + x], C.y[D + x], C.z[D + x])) - avg);
+            if (dlt < dev) continue;
+            if (dev < best) { best = dev; bx = x; }
+        }
+        C.napos[D + j] = (bx >= 0 && best < 1.5f) ? bx : -1;
+        return;
+    }
     char v = ' ';
     if (2 <= j && j < L - 2) {
         auto d = [&](int a, int b) {
@@ -154,6 +176,26 @@ __global__ void k_secstr(DevChunk C) {
         }
     }
     C.ss[D + j] = v;
+}
+
+// Nucleic-acid SS from the pairing partners (MpSecStr.cpp:148-200): mutual
+// pairs j < x become '(' at j and ')' at x, every other position '.'.
+__global__ void k_secstr_na(DevChunk C) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= C.nposs) return;
+    int lo = 0, hi = C.nstrs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if ((int)C.dst[mid] <= g) lo = mid; else hi = mid - 1;
+    }
+    const int s = lo;
+    if (!C.na[s]) return;
+    const int D = (int)C.dst[s], L = C.len[s], j = g - D;
+    const int p = C.napos[g];
+    char v = '.';
+    if (0 <= p && p < L && j < p && C.napos[D + p] == j) v = '(';
+    else if (0 <= p && p < L && p < j && C.napos[D + p] == j) v = ')';
+    C.ss[g] = v;
 }
 
 // ============================================================================
@@ -1283,7 +1325,17 @@ __global__ __launch_bounds__(GTA_WAVE) void k_dp_grid_walk(DevChunk Q, DevChunk 
 
 // ============================================================================
 // refinement over the DP alignment (reverse-ordered matched pairs)
-enum { RM_DPREFINE = 0, RM_FIN1 = 1, RM_FIN2 = 2, RM_PROD1 = 3, RM_PROD2 = 4 };
+// Refinement modes: DPRefine, the final stage's phases 1/2 and the production
+// phases 1/2 (MpStageFin.h:337-460).  Phase 2 around phase 1's fragment:
+// FIN2/PROD2 one fragment length at 2x128 positions (--refinement ONE/COARSE),
+// FIN2F/PROD2F every fragment length at those positions (the full searches of
+// --refinement FULLA and FULL/FULLA), PROD2L the sequential log search of
+// --refinement LOG.
+enum { RM_DPREFINE = 0, RM_FIN1 = 1, RM_FIN2 = 2, RM_PROD1 = 3, RM_PROD2 = 4, RM_FIN2F = 5, RM_PROD2F = 6,
+       RM_PROD2L = 7 };
+// the log search's steps: 32, 16, ..., 1 positions forward then backward,
+// fragment lengths 1..5 at each (MpStageFin.cpp:1636-1663)
+constexpr int LOG_NSTEPS = 6 * 2 * (NMAXSUBFRAGS - 1);
 
 struct RefineParams {
     int dinv_on;
@@ -1426,7 +1478,7 @@ __device__ void refine_dp_epilogue(const DevChunk& Q, const DevChunk& R, const D
                                    const RefineParams& prm, int p, const SlotBest& b) {
     const int qi = pair_q(S, p), ri = pair_r(S, p);
     const int qlen = Q.len[qi], rlen = R.len[ri];
-    const bool prod = (MODE == RM_PROD1 || MODE == RM_PROD2);
+    const bool prod = (MODE == RM_PROD1 || MODE == RM_PROD2 || MODE == RM_PROD2F);
     const bool have = b.best > 0.f;
     float* tg = S.tfm_grand + (size_t)p * 12;
     if (!prod) {
@@ -1435,7 +1487,7 @@ __device__ void refine_dp_epilogue(const DevChunk& Q, const DevChunk& R, const D
         if (MODE == RM_FIN1) { S.sfragndx[p] = have ? b.ndx : 0; S.sfragpos[p] = have ? b.pos : 0; }
         if (have && S.grand[p] < b.best) { S.grand[p] = b.best; copy12(tg, b.tfm); }
     } else {
-        const float grand = (MODE == RM_PROD2) ? S.grand[p] : 0.f;
+        const float grand = (MODE == RM_PROD2 || MODE == RM_PROD2F) ? S.grand[p] : 0.f;
         if (have && grand < b.best) {
             S.grand[p] = b.best;
             if (MODE == RM_PROD1) { S.sfragndx[p] = b.ndx; S.sfragpos[p] = b.pos; }
@@ -1507,7 +1559,7 @@ __device__ void refine_pair(const DevChunk& Q, const DevChunk& R, const DevState
     }
     __syncthreads();
 
-    const bool prod = (MODE == RM_PROD1 || MODE == RM_PROD2);
+    const bool prod = (MODE == RM_PROD1 || MODE == RM_PROD2 || MODE == RM_PROD2F || MODE == RM_PROD2L);
     const int rna = R.na[ri];
     const float d0 = prod ? GetD0fin(qlen, rlen, rna) : GetD0(qlen, rlen);
     const float d02 = d0 * d0, d82 = GetD82(qlen, rlen);
@@ -1517,12 +1569,21 @@ __device__ void refine_pair(const DevChunk& Q, const DevChunk& R, const DevState
     const bool chk = !prod;
     const float bestinit = (MODE == RM_DPREFINE) ? 0.f : -1.f;
     const bool phase2 = (MODE == RM_FIN2 || MODE == RM_PROD2);
-    const int nslots = skip ? 0 : (phase2 ? G.maxnfragfcts : G.nlocsteps);
-    const int base_ndx = phase2 ? S.sfragndx[p] : 0;
-    const int base_pos = phase2 ? S.sfragpos[p] : 0;
+    const bool full2 = (MODE == RM_FIN2F || MODE == RM_PROD2F);
+    const bool logm = (MODE == RM_PROD2L);
+    const int nslots = skip ? 0
+                            : (phase2 ? G.maxnfragfcts
+                                      : full2 ? G.maxnfragfcts * NMAXSUBFRAGS : logm ? LOG_NSTEPS : G.nlocsteps);
+    const int base_ndx = (phase2 || full2) ? S.sfragndx[p] : 0;
+    const int base_pos = (phase2 || full2 || logm) ? S.sfragpos[p] : 0;
+    // log search: the running best (from the pair's grand best) and the
+    // position the search has moved to, carried from step to step
+    const float bestorg = logm ? S.grand[p] : 0.f;
+    float lbest = bestorg, lbestprev = 0.f;
+    int lpos = base_pos, lcenter = base_pos;
     int slo, shi;
     part_range(0, nslots, blockIdx.y, gridDim.y, slo, shi);
-    if (!phase2) {
+    if (!phase2 && !full2 && !logm) {
         // slots at fragment positions from L + SFRAGSTEP on fail the slot
         // test below for every fragment length >= 1: not fetched at all
         const int vslots = ((L + SFRAGSTEP - 1) / SFRAGSTEP + 1) * NMAXSUBFRAGS;
@@ -1538,7 +1599,8 @@ __device__ void refine_pair(const DevChunk& Q, const DevChunk& R, const DevState
     float* tfmBest = tbest[grp];
     if (glane() == 0) mine.clear();
     unsigned long long ab = 0;
-    bool need = slo < shi, live = false;
+    // the log search's steps depend on each other: one slot group takes them in order
+    bool need = slo < shi && (!logm || grp == 0), live = false;
     int si = 0, cit = 0, fl = 0, sfragndx = 0, sfragpos = 0;
     float tfm[12], thr = CP_LARGEDST, best = 0.f;
     T* last = tlast[grp];       // previous covariance record (15 sums, count)
@@ -1560,6 +1622,20 @@ __device__ void refine_pair(const DevChunk& Q, const DevChunk& R, const DevState
                         if (sfragndx == 0) sfragndx++;
                         sfragpos = base_pos + (si - (G.maxnfragfcts >> 1));
                         step = 1;
+                    } else if (full2) {
+                        const int sfragfct = si / NMAXSUBFRAGS;
+                        sfragndx = si - sfragfct * NMAXSUBFRAGS;
+                        sfragpos = base_pos + (sfragfct - (G.maxnfragfcts >> 1));
+                        step = 1;
+                    } else if (logm) {
+                        const int sgnstep = ((si / (NMAXSUBFRAGS - 1)) & 1) ? -(32 >> (si / (2 * (NMAXSUBFRAGS - 1))))
+                                                                          : (32 >> (si / (2 * (NMAXSUBFRAGS - 1))));
+                        sfragndx = 1 + si % (NMAXSUBFRAGS - 1);
+                        // the step's position is taken from the search's position
+                        // when the step begins, before its fragment lengths run
+                        if (sfragndx == 1) lcenter = lpos;
+                        sfragpos = lcenter + sgnstep;
+                        step = max(0, sgnstep);
                     } else {
                         const int sfragfct = si / NMAXSUBFRAGS;
                         sfragndx = si - sfragfct * NMAXSUBFRAGS;
@@ -1570,10 +1646,11 @@ __device__ void refine_pair(const DevChunk& Q, const DevChunk& R, const DevState
                     if (fl >= 1 && !(sfragpos < 0 || L + step <= sfragpos + fl)) {
                         cit = 0;
                         thr = CP_LARGEDST;
-                        best = bestinit;
+                        best = logm ? lbest : bestinit;
+                        lbestprev = lbest;
 #pragma unroll
                         for (int i = 0; i < 12; i++) tfm[i] = 0.f;
-                        if (glane() == 0)
+                        if (glane() == 0 && !logm)
                             for (int i = 0; i < 12; i++) tfmBest[i] = (i == 0 || i == 4 || i == 8) ? 1.f : 0.f;
                         need = false;
                         live = true;
@@ -1645,7 +1722,13 @@ __device__ void refine_pair(const DevChunk& Q, const DevChunk& R, const DevState
                 }
                 if (++cit == NMAXCONVIT + 2) done = true;
             }
-            if (done) {
+            if (done && logm) {
+                // the search moves to this position when it raised the best
+                if (lbestprev < best) lpos = sfragpos;
+                lbest = best;
+                live = false;
+                need = true;
+            } else if (done) {
                 float gbest = best;
                 if (prod && qlen != rlen) {
                     float tb[12];
@@ -1662,6 +1745,26 @@ __device__ void refine_pair(const DevChunk& Q, const DevChunk& R, const DevState
             }
         }
     }
+    if (logm) {
+        // a result above the grand best goes directly to the output record and
+        // transform (SaveBestQRScoresAndTM_Phase2_logsearch_Complete,
+        // MpStageFin.h:777-801); every group computes, group 0 writes
+        float tb[12];
+        copy12(tb, tfmBest);
+        float gbest = lbest;
+        if (qlen != rlen) {
+            gbest = gscore_dp<P>(a, L, tb, g0, g02, d82, false, false, nullptr);
+            ab += AB_TM * (unsigned long long)L;
+        }
+        if (grp == 0 && glane() == 0 && bestorg < lbest) {
+            float sq = lbest, sr = gbest;
+            if (rlen < qlen) { float t = sq; sq = sr; sr = t; }
+            float* o = S.out + (size_t)p * 15;
+            o[9] = sq / (float)qlen;
+            o[10] = sr / (float)rlen;
+            copy12(S.tfm_grand + (size_t)p * 12, tb);
+        }
+    }
     };
     if (staged) {
         slots(AlnL{(lds_cf2ptr)lds_rf});
@@ -1669,6 +1772,7 @@ __device__ void refine_pair(const DevChunk& Q, const DevChunk& R, const DevState
         slots(Aln{g, g + pl, g + 2 * pl, g + 3 * pl, g + 4 * pl, g + 5 * pl});
     }
     if (glane() == 0 && ab) atomicAdd(S.algbytes + (MODE == RM_DPREFINE ? KC_REFINE_DP : KC_REFINE_FIN), ab);
+    if (logm) return;
     SlotBest b;
     group_best_lds(slds, b);
     if (threadIdx.x == 0) {
@@ -1975,6 +2079,7 @@ static inline int blocks_for(long long n, int t) { return (int)((n + t - 1) / t)
 
 hipError_t launch_secstr(const DevChunk& C, hipStream_t st) {
     hipLaunchKernelGGL(k_secstr, dim3(blocks_for(C.nposs, 256)), dim3(256), 0, st, C);
+    hipLaunchKernelGGL(k_secstr_na, dim3(blocks_for(C.nposs, 256)), dim3(256), 0, st, C);
     return hipGetLastError();
 }
 // pair_score over caller-given operands (tests of the division)
@@ -2169,7 +2274,11 @@ hipError_t launch_refine_dp(int prec, int mode, const DevChunk& Q, const DevChun
     const size_t lds = (size_t)L.ldscap * 6 * sizeof(float);
     const int nsmall = wq.items ? pgrid : (Sp.small ? Sp.nsmall : S.nq * S.nr);
     const bool phase2 = (mode == RM_FIN2 || mode == RM_PROD2);
-    const int np = Sp.nbig > 0 ? nparts_for(phase2 ? G.maxnfragfcts : G.nlocsteps, Sp.slots_per_part) : 1;
+    const bool full2 = (mode == RM_FIN2F || mode == RM_PROD2F);
+    const int np = Sp.nbig > 0 ? nparts_for(phase2 ? G.maxnfragfcts
+                                                   : full2 ? G.maxnfragfcts * NMAXSUBFRAGS : G.nlocsteps,
+                                            Sp.slots_per_part)
+                               : 1;
     SlotBest* parts = (SlotBest*)Sp.partbuf;
 #define GTA_RL(PP, MM)                                                                                        \
     do {                                                                                                      \
@@ -2183,16 +2292,30 @@ hipError_t launch_refine_dp(int prec, int mode, const DevChunk& Q, const DevChun
                                prm, Sp.big, Sp.nbig, np, parts);                                              \
         }                                                                                                     \
     } while (0)
+    // the log search's steps run in order within one workgroup: no split
+#define GTA_RL_LOG(PP)                                                                                         \
+    do {                                                                                                       \
+        if (nsmall > 0)                                                                                        \
+            hipLaunchKernelGGL((k_refine_dp<PP, RM_PROD2L>), dim3(nsmall), dim3(RTHREADS), lds, st, Q, R, S, G, \
+                               prm, L.ldscap, Sp.small, (SlotBest*)nullptr, wq);                               \
+        if (Sp.nbig > 0)                                                                                       \
+            hipLaunchKernelGGL((k_refine_dp<PP, RM_PROD2L>), dim3(Sp.nbig), dim3(RTHREADS), lds, st, Q, R, S, G,\
+                               prm, L.ldscap, Sp.big, (SlotBest*)nullptr, WorkQ{});                            \
+    } while (0)
 #define GTA_RLM(PP)                                      \
     switch (mode) {                                      \
         case RM_DPREFINE: GTA_RL(0, RM_DPREFINE); break;  \
         case RM_FIN1: GTA_RL(PP, RM_FIN1); break;         \
         case RM_FIN2: GTA_RL(PP, RM_FIN2); break;         \
+        case RM_FIN2F: GTA_RL(PP, RM_FIN2F); break;       \
         case RM_PROD1: GTA_RL(PP, RM_PROD1); break;       \
+        case RM_PROD2F: GTA_RL(PP, RM_PROD2F); break;     \
+        case RM_PROD2L: GTA_RL_LOG(PP); break;            \
         default: GTA_RL(PP, RM_PROD2); break;             \
     }
     if (prec) { GTA_RLM(1) } else { GTA_RLM(0) }
 #undef GTA_RLM
+#undef GTA_RL_LOG
 #undef GTA_RL
     return hipGetLastError();
 }

@@ -7,7 +7,8 @@
 namespace gta {
 
 enum { DPM_IBEST_L = 0, DPM_GRAND_L = 1, DPM_CONSTRAINED_L = 2 };
-enum { RM_DPREFINE_L = 0, RM_FIN1_L = 1, RM_FIN2_L = 2, RM_PROD1_L = 3, RM_PROD2_L = 4 };
+enum { RM_DPREFINE_L = 0, RM_FIN1_L = 1, RM_FIN2_L = 2, RM_PROD1_L = 3, RM_PROD2_L = 4, RM_FIN2F_L = 5,
+       RM_PROD2F_L = 6, RM_PROD2L_L = 7 };
 
 struct DpLaunch {
     int mode;          // DPM_*_L

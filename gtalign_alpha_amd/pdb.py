@@ -23,8 +23,17 @@ order (PM2DVectorFields.h:40-41), negated for a protein residue; the first
 residue's word carries the structure's type (PMBatchStrData.h:368-412,
 476-477).  A structure shorter than min_length (--dev-min-length, default 20,
 CLOptions.cpp:91) or with no selected atom is skipped, as
-PMBatchStrData::FinalizeCrntStructure does (PMBatchStrData.h:439-451); one
-longer than 65,535 residues is abandoned.
+PMBatchStrData::FinalizeCrntStructure does (PMBatchStrData.h:439-451).
+PMBatchStrData::AddOneResidue (PMBatchStrData.h:368-386) abandons a structure
+when a residue arrives while more than max_length residues (when max_length
+> 0) or more than 65,535 are already stored, so it keeps at most
+max_length + 1 (at most 65,536).  The reference reads queries with no limit
+(TaskScheduler.cpp:483-488) and references with --dev-max-length (default
+4000, CLOptions.cpp:90; TaskScheduler.cpp:490-494): `read_chunk` takes the
+limit as an argument, DEV_MAX_LENGTH for a reference chunk and None (no limit)
+for a query chunk.  The stage library accepts structures of up to 65,535
+residues (the 16-bit packed alignment coordinates of gproc.h:207-227), so a
+65,536-residue structure is read but rejected at upload.
 
 Lines shorter than 80 columns are read as if padded with blanks to 80; the
 reference reads such lines' missing columns from the bytes that follow.
@@ -36,9 +45,11 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from .chunk import Chunk, MAX_STRUCTURE_LENGTH
+from .chunk import Chunk
 
 DEFAULT_MIN_LENGTH = 20
+DEV_MAX_LENGTH = 4000        # --dev-max-length default, for reference chunks
+_BUFFER_LIMIT = 65535        # PMBSmaxonestructurelength (PMBatchStrData.h:34)
 PROTEIN_ATOM = " CA "
 NUCLEIC_ATOM = " C3'"
 
@@ -110,11 +121,12 @@ def _lines(source):
         return f.read().decode("latin-1").splitlines()
 
 
-def read_pdb(source, hetatm=False, min_length=DEFAULT_MIN_LENGTH,
+def read_pdb(source, hetatm=False, min_length=DEFAULT_MIN_LENGTH, max_length=None,
              protein_atom=PROTEIN_ATOM, nucleic_atom=NUCLEIC_ATOM):
     """The file's structure (path, or PDB text containing a newline), or None
-    when no atom is selected, the chain is shorter than min_length or longer
-    than the maximum structure length."""
+    when no atom is selected, the chain is shorter than min_length, or a
+    residue arrives while more than max_length (None or <= 0: no limit) or
+    more than 65,535 residues are stored."""
     if len(protein_atom) != 4 or len(nucleic_atom) != 4:
         raise ValueError("atom names are 4 columns wide")
     model = ""
@@ -148,7 +160,7 @@ def read_pdb(source, hetatm=False, min_length=DEFAULT_MIN_LENGTH,
             chord += 1
         restype = 1 if (line[17] == " " and line[18] in " D") else -1
         moltype += restype
-        if len(xyz) >= MAX_STRUCTURE_LENGTH:
+        if (max_length is not None and 0 < max_length < len(xyz)) or _BUFFER_LIMIT < len(xyz):
             return None
         word = ((ord(insc) & 0xff) << 16) | ((ord(chid) & 0xff) << 8) | (chord & 0xff)
         xyz.append((x, y, z))
@@ -174,12 +186,15 @@ def chunk_from_structures(structs):
     return c
 
 
-def read_chunk(sources, **kw):
+def read_chunk(sources, max_length=None, **kw):
     """Chunk of the structures of the given PDB files (skipped files are left
-    out); returns (chunk, indices of the sources that gave a structure)."""
+    out); returns (chunk, indices of the sources that gave a structure).
+    max_length: None for a query chunk (the reference reads queries with no
+    limit), DEV_MAX_LENGTH for a reference chunk read as the reference does
+    by default."""
     structs, kept = [], []
     for i, src in enumerate(sources):
-        s = read_pdb(src, **kw)
+        s = read_pdb(src, max_length=max_length, **kw)
         if s is not None:
             structs.append(s)
             kept.append(i)

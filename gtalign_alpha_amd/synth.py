@@ -262,3 +262,47 @@ def _chunks(qs, qr, rs, rr, src):
     qpos[qc.order] = np.arange(len(qs))
     s = np.asarray(src)[rc.order]
     return qc, rc, np.where(s >= 0, qpos[s], -1)
+
+
+# ---- nucleic acids ---------------------------------------------------------------
+# C3' traces: consecutive C3' atoms ~5.9 A apart; a base pair puts its two C3'
+# atoms ~13.75 A apart (the reference's C3' bounds 12.5-15 A,
+# PM2DVectorFields.h:178-206).
+NA_BASES = np.frombuffer(b"ACGU", dtype=np.uint8)
+_COMPLEMENT = {ord("A"): ord("U"), ord("U"): ord("A"), ord("G"): ord("C"), ord("C"): ord("G")}
+
+
+def na_compact(seed, L):
+    """A compact C3' trace of L residues (a generated chain scaled to 5.9 A steps)."""
+    return chain(seed, L) * np.float32(5.9 / 3.8)
+
+
+def na_hairpin(rng, nstem, nloop, noise=0.3):
+    """A stem-loop C3' trace: nstem residues up one strand, a loop of nloop
+    residues, nstem residues back down the other strand 13.75 A across; the
+    second strand's bases complement the first's.  Returns (coords, bases)."""
+    up = np.stack([np.zeros(nstem), 5.9 * np.arange(nstem), np.zeros(nstem)], 1)
+    down = up[::-1] + np.array([13.75, 0.0, 0.0])
+    ang = np.linspace(0, np.pi, nloop + 2)[1:-1]
+    loop = np.stack([6.875 - 6.875 * np.cos(ang), 5.9 * (nstem - 1) + 6.0 * np.sin(ang), np.zeros(nloop)], 1)
+    xyz = np.concatenate([up, loop, down]) + rng.normal(scale=noise, size=(2 * nstem + nloop, 3))
+    b1 = NA_BASES[rng.integers(0, 4, nstem)]
+    bl = NA_BASES[rng.integers(0, 4, nloop)]
+    b2 = np.array([_COMPLEMENT[int(b)] for b in b1[::-1]], dtype=np.uint8)
+    return xyz.astype(np.float32), np.concatenate([b1, bl, b2]).tobytes().decode()
+
+
+def na_bases(rng, L):
+    return NA_BASES[rng.integers(0, 4, L)].tobytes().decode()
+
+
+def mark_nucleic(c, nucleic):
+    """Sets the insertion/chain/order words of chunk c: positive (nucleic acid,
+    PM2DVectorFields.h:96-99) for the structures at the chunk positions where
+    nucleic[i] is true, negative (protein) for the others; chain 'A', order 1."""
+    word = (ord(" ") << 16) | (ord("A") << 8) | 1
+    for i in range(c.nstrs):
+        d, l = int(c.dst[i]), int(c.len[i])
+        c.ins_ch_ord[d:d + l] = word if nucleic[i] else -word
+    c._view = None
+    return c

@@ -51,7 +51,12 @@ enum gta_aln_field {
  * (reference: src/libgenp/gdats/PM2DVectorFields.h:220-258).
  * Structures are sorted by descending length (PMBatchStrData::Sort,
  * PMBatchStrData.cpp:110-160); dst[i] is the prefix offset of structure i.
- * ss may be NULL: the stage assigns secondary structure itself. */
+ * ss is not read: the stage assigns every position's secondary structure
+ * itself, as the reference does (MpSecStr::Run, MpSecStr.h:50-56).
+ * Coordinates must be finite with magnitude at most GTA_MAX_COORD (the pair
+ * score d0^2/(d0^2+d^2) is divided exactly only for d^2 far below the float
+ * exponent limit); gta_set_* rejects others with GTA_ERR_ARG. */
+#define GTA_MAX_COORD 1.0e15f
 typedef struct gta_chunk_view {
     int nstrs;
     int nposs;
@@ -64,15 +69,22 @@ typedef struct gta_chunk_view {
     const int* ins_ch_ord;   /* pmv2D_Ins_Ch_Ord: >0 marks a nucleic acid */
     const int* resnum;       /* pmv2DResNumber */
     const char* rsd;         /* pmv2Drsd: residue one-letter code */
-    const char* ss;          /* pmv2Dss (optional input) */
+    const char* ss;          /* pmv2Dss (not read; may be NULL) */
 } gta_chunk_view;
 
 /* ---- options: the CLOptions the stage reads (CuBatch.cu:353-361) ------ */
 enum gta_depth { GTA_DEPTH_DEEP = 0, GTA_DEPTH_HIGH, GTA_DEPTH_MEDIUM, GTA_DEPTH_SHALLOW };
-/* internal refinement codes (CLOptions.h:122-127; the CLI stores option+1).
- * GTA_REF_FULLA (final refinement by full search over all fragment lengths,
- * MpStageFin.h:363-376) is not implemented: gta_create rejects it with
- * GTA_ERR_ARG. */
+/* internal refinement codes (CLOptions.h:122-127; the CLI stores option+1),
+ * selecting the final stage's searches (MpStageFin.h:337-460):
+ *   LOG    production phase 2 as a log search of positions;
+ *   COARSE no final phase 2; production phase 2 at one fragment length;
+ *   ONE    final and production phase 2 at one fragment length (default);
+ *   FULL   production phase 2 over every fragment length;
+ *   FULLA  final and production phase 2 over every fragment length.
+ * The production full search (FULL, FULLA) runs only slots at alignment
+ * positions >= 0; the reference also runs slots at negative positions, on
+ * buffer memory past the pair's alignment, whose outcome depends on its
+ * memory history. */
 enum gta_refinement { GTA_REF_LOG = 0, GTA_REF_COARSE, GTA_REF_ONE, GTA_REF_FULL, GTA_REF_FULLA };
 enum gta_precision {
     GTA_PREC_FP32 = 0,       /* plain form: FP32 throughout, as the reference */

@@ -27,6 +27,7 @@
 // Compile with -ffp-contract=off: the DP recurrence must round like the
 // reference's scalar code for alignments to be bit-exact.
 
+#include <cfloat>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -420,13 +421,83 @@ char AssignSS(float d13, float d14, float d15, float d24, float d25, float d35) 
     if (d15 < 8.0f) return 't';
     return ' ';
 }
+// Nucleic-acid base pairing from C3' distances (MpSecStr::nass_helper,
+// MpSecStr.h:80-89; nass_initialize/calcdistances/calcsecstrs,
+// MpSecStr.cpp:113-360; GPU twin CalcDistances_NASS, sskna.cu:135-270;
+// pairing rule MpSecStr.h:171-178; deviation MpSecStr.h:191-200 with the C3'
+// bounds of PM2DVectorFields.h:178-206, the default --natom).  Residue y
+// pairs with the complementary residue x (|x - y| > 2) whose distance
+// deviates least from 13.75 A, if that deviation is at most 1.25 A (and below
+// the 1.5 A acceptance bound).  Mutual pairs y < x become '(' at y and ')' at
+// x; every other position of the structure is '.'.
+//
+// Which x is recorded for y:
+//  mode 0 (the stage's definition, as the GPU twin reduces it with a
+//    min-reduction over its lanes): the x of least deviation, the lowest x on
+//    exact ties;
+//  mode 1 (the CPU twin as g++ -O3 compiles it for x86-64): the reference
+//    keeps per lane xi = x mod 128 the first column block's minimum (strict
+//    <), then reduces the 128 lanes in a `#pragma omp simd reduction(min:devmin)`
+//    loop that also writes the non-reduction index `colmin`
+//    (MpSecStr.cpp:346-350).  The compiler vectorizes it over 4-wide vector
+//    lanes: each vector lane keeps a private running minimum starting at the
+//    reduction identity (FLT_MAX), and colmin ends as the index of the LAST
+//    lane entry (in lane order) that improved its own vector lane's minimum,
+//    often an empty lane (-1), while devmin is the true minimum.  Determined
+//    by matching this model against the reference build on random chunks; it
+//    is test infrastructure for pinning the shared parts against oracle/_ref.
+int g_na_ss_mode = 0;
+bool na_pairs(char a, char b) {
+    return ((a == 'T' || a == 'U') && b == 'A') || (a == 'A' && (b == 'T' || b == 'U')) ||
+           (a == 'G' && (b == 'C' || b == 'U')) || ((a == 'C' || a == 'U') && b == 'G');
+}
+void calc_secstr_na(const gta_chunk_view* c, int D, int L, char* ss) {
+    const float avg = (15.0f + 12.5f) * 0.5f, dlt = (15.0f - 12.5f) * 0.5f;
+    constexpr int NL = 128;
+    std::vector<int> pos(L, -1);
+    for (int y = 0; y < L; y++) {
+        float dst[NL];
+        int px[NL];
+        for (int k = 0; k < NL; k++) { dst[k] = 9999.9f; px[k] = -1; }
+        float best = 9999.9f;
+        int bx = -1;
+        for (int x = 0; x < L; x++) {
+            float dev = 9999.9f;
+            if (na_pairs(c->rsd[D + y], c->rsd[D + x]) && 2 < abs(x - y)) {
+                const float d = sqrtf(distance2(c->x[D + y], c->y[D + y], c->z[D + y], c->x[D + x],
+                                                c->y[D + x], c->z[D + x]));
+                dev = fabsf(d - avg);
+                if (dlt < dev) dev = 9999.9f;
+            }
+            if (dev < dst[x % NL]) { dst[x % NL] = dev; px[x % NL] = x; }
+            if (dev < best) { best = dev; bx = x; }
+        }
+        if (g_na_ss_mode == 1) {
+            float lane[4] = {FLT_MAX, FLT_MAX, FLT_MAX, FLT_MAX};
+            int cm = 0;
+            best = 9999.9f;
+            for (int k = 0; k < NL; k++) {
+                if (dst[k] < lane[k & 3]) { lane[k & 3] = dst[k]; cm = px[k]; }
+                best = std::min(best, dst[k]);
+            }
+            bx = cm;
+        }
+        if (best < 1.5f) pos[y] = bx;
+    }
+    for (int y = 0; y < L; y++) ss[y] = '.';
+    for (int y = 0; y < L; y++) {
+        const int p = pos[y];
+        if (0 <= p && p < L && y < p && pos[p] == y) { ss[y] = '('; ss[p] = ')'; }
+    }
+}
 void calc_secstr(const gta_chunk_view* c, std::vector<char>& ss) {
+    // the reference assigns every position itself (MpSecStr::Run,
+    // MpSecStr.h:50-56): a caller's SS field is not read
     ss.assign(c->nposs, ' ');
-    if (c->ss) std::memcpy(ss.data(), c->ss, c->nposs);
     for (int s = 0; s < c->nstrs; s++) {
         int L = c->len[s], D = (int)c->dst[s];
         bool na = c->ins_ch_ord && c->ins_ch_ord[D] > 0;
-        if (na) continue;  // nucleic-acid SS assignment is outside this stage
+        if (na) { calc_secstr_na(c, D, L, ss.data() + D); continue; }
         auto dist = [&](int i, int j) {
             return sqrtf(distance2(c->x[D + i], c->y[D + i], c->z[D + i], c->x[D + j], c->y[D + j],
                                    c->z[D + j]));
@@ -539,6 +610,7 @@ struct Ctx {
     ChunkGeom g;
     Str q, r;
     PairState* st;
+    size_t pidx = 0;         // pair index qi * nr + ri
     std::vector<float> dst;  // per-position distances of the current slot (tmpdpdiagbuffers)
     double cells = 0.0;
 };
@@ -1164,6 +1236,32 @@ void fin_phase2(Ctx& c) {
     if (have && s.grand < bscore) { s.grand = bscore; std::memcpy(s.tfm_grand, btfm, sizeof(btfm)); }
 }
 
+// FinalFragmentBasedDPAlignmentRefinementPhase2_fullsearchKernel
+// (MpStageFin.cpp:511-712; --refinement FULLA): every fragment length at
+// every position around phase 1's, slots in (position, length) order.
+void fin_phase2_full(Ctx& c) {
+    PairState& s = *c.st;
+    if (s.flags & CONV_LOWTMSC) return;
+    const int L = s.nalnposs;
+    const float d0 = GetD0(c.q.len, c.r.len), d02 = d0 * d0, d82 = GetD82(c.q.len, c.r.len);
+    float bscore = 0.f, btfm[12];
+    bool have = false;
+    for (int si = 0; si < c.g.maxnfragfcts * FRAGREF_NMAXSUBFRAGS; si++) {
+        const int sfragfct = si / FRAGREF_NMAXSUBFRAGS;
+        const int sfragndx = si - sfragfct * FRAGREF_NMAXSUBFRAGS;
+        const int sfragpos = s.sfragpos + (sfragfct - (c.g.maxnfragfcts >> 1)) * FRAGREF_SFRAGSTEP_mini;
+        const int fraglen = GetFragLength(L, L, 0, 0, sfragndx);
+        if (fraglen < 1) continue;
+        if (sfragpos < 0 || L + FRAGREF_SFRAGSTEP_mini <= sfragpos + fraglen) continue;
+        float tfmBest[12];
+        float best = refine_slot_dp(c, fraglen, sfragpos, d0, d02, d82, true, -1.0f, tfmBest, nullptr);
+        if (best > 0.f && bscore < best) { bscore = best; std::memcpy(btfm, tfmBest, sizeof(btfm)); have = true; }
+    }
+    s.best0 = have ? bscore : 0.f;
+    if (have) std::memcpy(s.tfm_ibest, btfm, sizeof(btfm));
+    if (have && s.grand < bscore) { s.grand = bscore; std::memcpy(s.tfm_grand, btfm, sizeof(btfm)); }
+}
+
 // production slot: refine with final d0 over all pairs, then score for the
 // longer structure (MpStageFin.cpp:590-680)
 float production_slot(Ctx& c, int fraglen, int fragpos, float d0, float d02, float d82, float g02,
@@ -1258,6 +1356,86 @@ void production_phase2(Ctx& c) {
     production_among_bests(c, bscore, bg, btfm, have, true);
 }
 
+// ProductionFragmentBasedDPAlignmentRefinementPhase2_fullsearchKernel
+// (MpStageFin.cpp:1096-1272; --refinement FULL and FULLA): every fragment
+// length at every position around phase 1's.  The reference applies no lower
+// bound to the position here (its ONE and LOG searches and the final stage's
+// searches skip positions below 0): a slot at a negative position starts from
+// the covariance of alignment buffer entries past the pair's alignment (older
+// alignments of the pair or the next reference's region), so its outcome
+// depends on the reference's memory history.  This restatement and the
+// library skip those slots, as the other searches do; the oracle records per
+// pair whether the reference would have run one (g_prod_negative).
+std::vector<unsigned char>* g_prod_negative = nullptr;
+std::vector<unsigned char> g_last_negative;
+void production_phase2_full(Ctx& c, size_t pidx) {
+    PairState& s = *c.st;
+    if (s.flags & CONV_LOWTMSC) return;
+    const int L = s.nalnposs;
+    const float d0 = GetD0fin(c.q.len, c.r.len, c.r.na), d02 = d0 * d0, d82 = GetD82(c.q.len, c.r.len);
+    const int glen = std::max(c.q.len, c.r.len);
+    const float g0 = GetD0fin(glen, glen, c.r.na), g02 = g0 * g0;
+    float bscore = 0.f, bg = 0.f, btfm[12];
+    bool have = false, negative = false;
+    for (int si = 0; si < c.g.maxnfragfcts * FRAGREF_NMAXSUBFRAGS; si++) {
+        const int sfragfct = si / FRAGREF_NMAXSUBFRAGS;
+        const int sfragndx = si - sfragfct * FRAGREF_NMAXSUBFRAGS;
+        const int sfragpos = s.sfragpos + (sfragfct - (c.g.maxnfragfcts >> 1)) * FRAGREF_SFRAGSTEP_mini;
+        const int fraglen = GetFragLength(L, L, 0, 0, sfragndx);
+        if (fraglen < 1) continue;
+        if (L + FRAGREF_SFRAGSTEP_mini <= sfragpos + fraglen) continue;
+        if (sfragpos < 0) { negative = true; continue; }
+        float tfmBest[12], gb;
+        float best = production_slot(c, fraglen, sfragpos, d0, d02, d82, g02, tfmBest, &gb, nullptr);
+        if (0.0f < best && bscore < best) { bscore = best; bg = gb; std::memcpy(btfm, tfmBest, sizeof(btfm)); have = true; }
+    }
+    if (g_prod_negative) (*g_prod_negative)[pidx] = negative;
+    production_among_bests(c, bscore, bg, btfm, have, true);
+}
+
+// ProductionFragmentBasedDPAlignmentRefinementPhase2_logsearchKernel
+// (MpStageFin.cpp:1528-1690; --refinement LOG): from phase 1's position,
+// steps of 32, 16, ..., 1 positions forward then backward, fragment lengths
+// 1..5 at each; the search moves to a position whenever it raises the best
+// score (which starts at the pair's grand best).  A result better than the
+// grand best goes directly to the output record and transform
+// (SaveBestQRScoresAndTM_Phase2_logsearch_Complete, MpStageFin.h:777-801); the
+// grand best itself is left unchanged.
+void production_phase2_log(Ctx& c) {
+    PairState& s = *c.st;
+    if (s.flags & CONV_LOWTMSC) return;
+    const int L = s.nalnposs;
+    const float d0 = GetD0fin(c.q.len, c.r.len, c.r.na), d02 = d0 * d0, d82 = GetD82(c.q.len, c.r.len);
+    const float bestorg = s.grand;
+    float best = bestorg, tfmBest[12];
+    int posorg = s.sfragpos;
+    for (int stepabs = 32; stepabs >= 1; stepabs >>= 1)
+        for (int sgn = 1; sgn >= -1; sgn -= 2) {
+            const int step = sgn * stepabs;
+            const int sfragpos = posorg + step;
+            for (int sfragndx = 1; sfragndx < FRAGREF_NMAXSUBFRAGS; sfragndx++) {
+                const int fraglen = GetFragLength(L, L, 0, 0, sfragndx);
+                if (fraglen < 1) continue;
+                if (sfragpos < 0 || L + std::max(0, step) <= sfragpos + fraglen) continue;
+                const float bestprev = best;
+                best = refine_slot_dp(c, fraglen, sfragpos, d0, d02, d82, false, best, tfmBest, nullptr);
+                if (bestprev < best) posorg = sfragpos;
+            }
+        }
+    if (best <= bestorg) return;
+    float gbest = best;
+    if (c.q.len != c.r.len) {
+        const int glen = std::max(c.q.len, c.r.len);
+        const float g0 = GetD0fin(glen, glen, c.r.na), g02 = g0 * g0;
+        gbest = scores_dp(c, READCNST_CALC2, g0, g02, d82, false, tfmBest, nullptr);
+    }
+    float sq = best, sr = gbest;
+    if (c.r.len < c.q.len) std::swap(sq, sr);
+    s.out[GTA_ALN_SCOREQ] = sq / (float)c.q.len;
+    s.out[GTA_ALN_SCORER] = sr / (float)c.r.len;
+    std::memcpy(s.tfm_grand, tfmBest, 12 * sizeof(float));
+}
+
 // ProductionMatchToAlignment128xKernel (MpDPHub.cpp:1919-2142)
 void production_alignment(Ctx& c) {
     PairState& s = *c.st;
@@ -1335,15 +1513,21 @@ void process_pair_rest(Ctx& c) {
     // final stage (MpStageFin.h:66-90)
     dp_align_grand(c, 0.0f, true, false, false, false);
     fin_phase1(c);
-    // (refinement by full search over all fragment lengths, MpStageFin.h:363, is
-    // not restated: the library rejects that option)
-    if (!(o.depth == GTA_DEPTH_SHALLOW || o.refinement == GTA_REF_COARSE)) fin_phase2(c);
+    // MpStageFin::Refine (MpStageFin.h:337-392)
+    if (!(o.depth == GTA_DEPTH_SHALLOW || o.refinement == GTA_REF_COARSE)) {
+        if (o.refinement == GTA_REF_FULLA) fin_phase2_full(c);
+        else fin_phase2(c);
+    }
     dp_align_grand(c, 0.0f, true, false, false, true);
     production_alignment(c);
+    // MpStageFin::ProduceOutputScores (MpStageFin.h:398-460)
     production_phase1(c);
-    if (o.refinement == GTA_REF_ONE || o.refinement == GTA_REF_COARSE) production_phase2(c);
-    if (o.referenced && !(c.st->flags & CONV_LOWTMSC)) {
+    if (o.refinement == GTA_REF_LOG) production_phase2_log(c);
+    else if (o.refinement == GTA_REF_ONE || o.refinement == GTA_REF_COARSE) production_phase2(c);
+    else production_phase2_full(c, c.pidx);
+    if (o.referenced) {
         // RevertTfmMatricesKernel (MpStageFin.cpp:1835-1880): inverse transform
+        // of every pair of the block, flagged low-scoring or not
         float* t = c.st->tfm_grand;
         std::swap(t[1], t[3]); std::swap(t[2], t[6]); std::swap(t[5], t[7]);
         float t0 = t[9], t1 = t[10], t2 = t[11];
@@ -1402,6 +1586,7 @@ double gta_oracle_process_block(const gta_chunk_view* q, const gta_chunk_view* r
         c.q = get_str(q, qss, qi);
         c.r = get_str(r, rss, ri);
         c.st = &states[(size_t)qi * nr + ri];
+        c.pidx = (size_t)qi * nr + ri;
         return c;
     };
     for (int qi = 0; qi < nq; qi++)
@@ -1433,6 +1618,8 @@ double gta_oracle_process_block(const gta_chunk_view* q, const gta_chunk_view* r
         }
     }
     if (pass) std::memcpy(pass, keep.data(), keep.size());
+    g_last_negative.assign((size_t)nq * nr, 0);
+    g_prod_negative = &g_last_negative;
     for (int qi = 0; qi < nq; qi++) {
         for (int ri = 0; ri < nr; ri++) {
             Ctx c = ctx(qi, ri);
@@ -1500,6 +1687,32 @@ int gta_oracle_dp_ssrr(const char* qss, const char* qrsd, int Lq, const char* rs
 
 // The residue score table (26 x 26, bits) the SS/residue fill uses.
 void gta_oracle_gonnet(float* out) { gta::gonnet_bits(out); }
+
+// Per pair of the last gta_oracle_process_block call (nq*nr bytes): 1 when
+// the production full search (--refinement FULL/FULLA) had slots at negative
+// alignment positions, which the reference runs on memory past the pair's
+// alignment and this restatement skips.
+int gta_oracle_last_prod_negative(unsigned char* out, size_t n) {
+    if (n != g_last_negative.size()) return -1;
+    std::memcpy(out, g_last_negative.data(), n);
+    return 0;
+}
+
+// Selects the recorded pairing partner of the nucleic-acid SS (see
+// calc_secstr_na): 0 the stage's definition, 1 the CPU twin as compiled.
+// Returns the previous mode.
+int gta_oracle_set_na_ss_mode(int mode) {
+    const int prev = g_na_ss_mode;
+    g_na_ss_mode = mode;
+    return prev;
+}
+
+// Secondary structure of every position of one chunk (nposs letters).
+void gta_oracle_secstr(const gta_chunk_view* c, char* out) {
+    std::vector<char> ss;
+    calc_secstr(c, ss);
+    std::memcpy(out, ss.data(), c->nposs);
+}
 
 // Score-only DP fill (ExecDPScore128xKernel<GAP0>, MpDPHub.cpp:389-647) under
 // transform tfm with the search d0 (GetD02): the bottom-right cell's value.

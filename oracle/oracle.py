@@ -48,6 +48,7 @@ def lib():
         _orc.gta_oracle_d0.argtypes = [C.c_int, C.c_int, C.c_void_p]
         _orc.gta_oracle_fraglen.argtypes = [C.c_int] * 5
         _orc.gta_oracle_fraglen.restype = C.c_int
+        _orc.gta_oracle_secstr.argtypes = [C.POINTER(ChunkView), C.c_void_p]
     return _orc
 
 
@@ -110,6 +111,51 @@ def process_block(q, r, opts=None):
                                            ssq.ctypes.data, ssr.ctypes.data, passed.ctypes.data)
     dt = time.perf_counter() - t0
     return BlockResult(q.nstrs, r.nstrs, aln, tfm, alns, dt, cells, ssq, ssr, passed)
+
+
+def last_prod_negative(nq, nr):
+    """Per pair of the last process_block call: True when the production full
+    search (--refinement FULL/FULLA) had slots at negative alignment
+    positions (run by the reference on memory past the alignment, skipped
+    here)."""
+    f = lib().gta_oracle_last_prod_negative
+    f.argtypes = [C.c_void_p, C.c_size_t]
+    f.restype = C.c_int
+    out = np.zeros(nq * nr, np.uint8)
+    if f(out.ctypes.data, out.size) != 0:
+        raise RuntimeError("no process_block record")
+    return out.reshape(nq, nr).astype(bool)
+
+
+def set_na_ss_mode(mode):
+    """Nucleic-acid pairing partner the oracle records: 0 the stage's
+    definition (least deviation, lowest index on ties), 1 the reference CPU
+    twin as compiled (its 4-lane vectorised minimum keeps the index of the
+    last lane improvement).  Returns the previous mode."""
+    f = lib().gta_oracle_set_na_ss_mode
+    f.argtypes = [C.c_int]
+    f.restype = C.c_int
+    return f(int(mode))
+
+
+def secstr(c):
+    """The oracle's secondary structure of every position of chunk c (bytes)."""
+    out = np.zeros(c.nposs, dtype=np.uint8)
+    lib().gta_oracle_secstr(C.byref(c.view()), out.ctypes.data)
+    return out
+
+
+def ref_secstr(c, with_pairs=False):
+    """The reference's own MpSecStr on chunk c (oracle/_ref); with_pairs also
+    returns the pairing partner it recorded per position (-1: none)."""
+    f = ref_lib().gta_ref_secstr
+    f.argtypes = [C.POINTER(ChunkView), C.c_void_p, C.c_void_p]
+    f.restype = C.c_int
+    out = np.zeros(c.nposs, dtype=np.uint8)
+    pos = np.zeros(c.nposs, dtype=np.int32)
+    if f(C.byref(c.view()), out.ctypes.data, pos.ctypes.data) != 0:
+        raise RuntimeError("reference SS assignment failed")
+    return (out, pos) if with_pairs else out
 
 
 def ref_process_block(q, r, opts=None, nthreads=1):

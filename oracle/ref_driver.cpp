@@ -384,6 +384,30 @@ int gta_ref_last_pass(unsigned char* out, size_t n) {
     return 0;
 }
 
+// Secondary structure of every position of one chunk on the reference's
+// MpSecStr (protein rule and nucleic-acid base pairing, MpSecStr.h:50-56),
+// the chunk serving as both query and reference chunk; out receives nposs
+// letters; pos (may be NULL) receives the pairing partner the reference
+// recorded for each position (-1: none).
+int gta_ref_secstr(const gta_chunk_view* c, char* out, int* pos) {
+    try {
+        ChunkFields qf, rf;
+        fill(c, qf);
+        fill(c, rf);
+        const size_t ns = c->nstrs, np = c->nposs;
+        const size_t L1 = c->len[0], Ln = c->len[ns - 1];
+        std::vector<float> tmp(3 * np + 64, 0.f);
+        MpSecStr(tmp.data(), qf.beg, qf.end, rf.beg, rf.end, ns, ns, np, np, L1, L1, Ln, Ln,
+                 ALIGN_UP(np, CUL2CLINESIZE) - np).Run();
+        std::memcpy(out, rf.beg[pmv2Dss], np);
+        if (pos)
+            for (size_t i = 0; i < np; i++) pos[i] = (int)tmp[i];
+        return 0;
+    } catch (...) {
+        return -1;
+    }
+}
+
 }  // extern "C"
 
 // One DP fill + BtckToMatched of one pair on the reference's code: the

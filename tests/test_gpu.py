@@ -203,12 +203,57 @@ def test_long_reference_row_buffer_path():
     assert g.alndata[0, 0, ALN_SCOREQ] > 0.5
 
 
-def test_options_depth_refinement_referenced():
+@pytest.mark.parametrize("kw", [dict(refinement=0), dict(refinement=1), dict(refinement=3), dict(refinement=4),
+                                dict(depth=0), dict(depth=1), dict(depth=3), dict(symmetric=1), dict(referenced=1),
+                                dict(nodeletions=1), dict(prescore=0.0, scorethld=0.0)],
+                         ids=["log", "coarse", "full", "fulla", "deep", "high", "shallow", "symmetric",
+                              "referenced", "nodeletions", "nothresholds"])
+def test_options_depth_refinement_referenced(kw):
+    """Option paths (each pinned to the reference's CPU path in the oracle's
+    tests) on the GPU: equal to the oracle."""
     qc, rc, _ = synth.config(0, scale=0.125)
-    for kw in (dict(refinement=0), dict(refinement=3), dict(depth=0), dict(depth=1),
-               dict(referenced=1), dict(nodeletions=1), dict(prescore=0.0, scorethld=0.0)):
-        o = oracle.process_block(qc, rc, default_options(**kw))
-        assert_same(o, run_gpu(qc, rc, **kw), qc, rc)
+    if kw.get("symmetric"):
+        # symmetric Kabsch swaps roles when Lq < Lr: queries shorter and longer
+        qc, rc, _ = synth.config(1, scale=0.001)
+        qc = qc.subset([0, 3])
+    o = oracle.process_block(qc, rc, default_options(**kw))
+    g = run_gpu(qc, rc, **kw)
+    assert_same(o, g, qc, rc)
+    al = o.alndata[..., ALN_RMSD] < 9999
+    assert np.abs(o.tfms[al] - g.tfms[al]).max() < 1e-3
+
+
+def test_nucleic_acid_stage_matches_oracle():
+    """Nucleic-acid chains: base-pairing SS (the least-deviation partner),
+    SS-weighted stage-2 DP over it, nucleic-acid d0 in the final stage; a
+    protein reference is never aligned to them."""
+    from tests.test_oracle import na_stage_chunks, na_test_chunk
+    q, r = na_stage_chunks()
+    o = oracle.process_block(q, r, default_options(qgroup=2))
+    g = run_gpu(q, r, qgroup=2)
+    assert_same(o, g, q, r)
+    assert (g.alndata[..., ALN_RMSD] < 9999).sum() >= 2
+    # the SS rows of the alignments carry base-pairing letters
+    assert {ord("("), ord(")")} <= set(np.unique(g.alns).tolist())
+    # long nucleic-acid chains as references (201 and 150 residues)
+    r2 = na_test_chunk()
+    o = oracle.process_block(q, r2, default_options(prescore=0.0, scorethld=0.0))
+    assert_same(o, run_gpu(q, r2, prescore=0.0, scorethld=0.0), q, r2)
+
+
+def test_coordinates_out_of_range_rejected():
+    """Coordinates must be finite and at most GTA_MAX_COORD (1e15) in
+    magnitude: the pair score's division is exact only far below the float
+    exponent limit."""
+    base = synth.chain(7, 30)
+    for bad in (np.nan, np.inf, 2e15):
+        x = base.copy()
+        x[4, 1] = bad
+        c = Chunk.from_structures([x])
+        with GtaBatch(0, default_options(), max_qrys=1, max_qry_poss=30, max_refs=1, max_ref_poss=30) as b:
+            with pytest.raises(GtaError) as e:
+                b.TransferCPMDataToDevice(c)
+        assert e.value.code == _code("GTA_ERR_ARG")
 
 
 def test_context_reuse_with_smaller_chunk():
@@ -674,6 +719,33 @@ def test_batched_form_with_large_pair_buckets(grid):
         assert np.array_equal(g.alns, out[0][0].alns)
         assert np.array_equal(g.alndata, out[0][0].alndata)
         assert np.array_equal(g.tfms, out[0][0].tfms)
+
+
+def test_batched_graph_follows_launch_setting_changes():
+    """Changing the DP launch forms or the split of the searches between two
+    calls of a batched context (whose first call captured the launch sequence
+    as a HIP graph) takes effect at the next call: results stay equal to the
+    plain form's."""
+    qc, rc, _ = synth.config(1, scale=0.002)
+    q = qc.subset([0, 5])
+    r = rc.subset([i for i in range(rc.nstrs) if rc.len[i] >= 256][:16] + [rc.nstrs - 1])
+    plain = run_gpu(q, r)
+    with GtaBatch(0, default_options(batched=1), max_qrys=q.nstrs, max_qry_poss=q.nposs,
+                  max_refs=r.nstrs, max_ref_poss=r.nposs) as b:
+        b.TransferQueryPMDataToDevice(q)
+        b.TransferCPMDataToDevice(r)
+        outs = [b.ProcessBlock()]
+        b.set_dp_buckets(0, 2)
+        b.set_dp_grid(2)
+        outs.append(b.ProcessBlock())
+        b.set_dp_grid(1)
+        outs.append(b.ProcessBlock())
+        b.set_split(300, 64, 8)
+        outs.append(b.ProcessBlock())
+    for k, g in enumerate(outs):
+        assert np.array_equal(g.alns, plain.alns), k
+        assert np.array_equal(g.alndata, plain.alndata), k
+        assert np.array_equal(g.tfms, plain.tfms), k
 
 
 # ---- the boundary's error behaviour ----------------------------------------------------

@@ -288,6 +288,50 @@ def test_stage_matches_reference_cpu_path(have_ref, cfg, scale, qsel):
     assert (r.passed & (o.alndata[..., 8] < 9999)).sum() >= 10
 
 
+OPTION_PATHS = [dict(refinement=0), dict(refinement=1), dict(refinement=3), dict(refinement=4), dict(depth=0),
+                dict(depth=1), dict(depth=3), dict(symmetric=1), dict(referenced=1), dict(nodeletions=1),
+                dict(convergence=5), dict(prescore=0.0, scorethld=0.0)]
+OPTION_IDS = ["log", "coarse", "full", "fulla", "deep", "high", "shallow", "symmetric", "referenced",
+              "nodeletions", "convergence5", "nothresholds"]
+
+
+@pytest.mark.parametrize("kw", OPTION_PATHS, ids=OPTION_IDS)
+def test_option_paths_match_reference_cpu_path(have_ref, kw):
+    """Every option path the stage takes (--refinement LOG/COARSE/FULL, --depth
+    DEEP/HIGH/SHALLOW, --symmetric, --referenced, --no-deletions,
+    --convergence, no score thresholds; CLOptions.cpp:47-48,68-70): the
+    oracle equals the reference's own CPU path bit for bit.  The reference
+    reads its options once per process, so each option set runs in a child
+    process of its own (oracle/ref_cli.py)."""
+    if not have_ref:
+        pytest.skip("reference CPU build (oracle/_ref) not present")
+    from oracle import ref_cli
+    qc, rc, _ = synth.config(1, scale=0.0015)
+    qc = qc.subset([0, 9])            # queries of 574 and 171 residues: both sides longer
+    opts = default_options(**kw)
+    rc = oracle.kept_first(qc, rc, opts)
+    o = oracle.process_block(qc, rc, opts)
+    d = ref_cli.run(qc, rc, 1, kw)
+    r = oracle.BlockResult(qc.nstrs, rc.nstrs, d["alndata"], d["tfms"], d["alns"], passed=d["passed"])
+    aligned = r.passed & (o.alndata[..., 8] < 9999)
+    if kw.get("refinement", 2) >= 3:
+        # the reference's production full search also runs slots at negative
+        # alignment positions, on memory past the pair's alignment (older
+        # alignments, the next reference's region): those pairs are left out
+        # of the bit-level comparison; at least half of the aligned pairs are
+        # compared
+        neg = oracle.last_prod_negative(qc.nstrs, rc.nstrs)
+        assert (aligned & ~neg).sum() >= 0.5 * aligned.sum()
+        assert np.array_equal(o.passed, r.passed)
+        assert np.array_equal(o.alns, r.alns)      # rows come before the production search
+        m = r.passed & ~neg
+        assert np.array_equal(o.alndata[m], r.alndata[m])
+        assert np.array_equal(o.tfms[m], r.tfms[m])
+        return
+    assert_kept_pairs_equal(o, r)
+    assert (r.passed & (o.alndata[..., 8] < 9999)).sum() >= 5
+
+
 def test_reference_filter_blocks_match_reference(have_ref):
     """The filter after stage 1 in blocks of 2 queries (the reference's
     scheduler): a reference stays for both queries of a block when either
@@ -354,3 +398,138 @@ def test_short_chains_match_reference(have_ref):
     assert_kept_pairs_equal(o, ref)
     # the 3-residue pair aligns completely; the exact copy scores 1
     assert o.alndata[0, 0, 4] == 30 and abs(o.alndata[0, 0, 9] - 1.0) < 1e-6
+
+
+# ---- secondary structure (MpSecStr.h:50-56) ------------------------------------------
+
+def na_test_chunk():
+    """Nucleic-acid C3' traces (two stem-loops, a compact random trace, a
+    201-residue construct with an exact tie) and one protein chain."""
+    from gtalign_alpha_amd.chunk import Chunk
+    rng = np.random.default_rng(11)
+    h1, b1 = synth.na_hairpin(rng, 12, 5)
+    h2, b2 = synth.na_hairpin(rng, 30, 7, noise=0.6)
+    cmp_ = synth.na_compact(41, 150)
+    # tie: residue 0 (G) has two complementary partners at exactly 13.75 A,
+    # residues 5 and 130 (C); the lower index, 5, is recorded
+    tie = np.stack([100.0 * np.arange(201), np.zeros(201), np.zeros(201)], 1).astype(np.float32)
+    tie[0] = (0.0, 0.0, 0.0)
+    tie[5] = (13.75, 0.0, 0.0)
+    tie[130] = (0.0, 13.75, 0.0)
+    tb = ["A"] * 201
+    tb[0], tb[5], tb[130] = "G", "C", "C"
+    prot = synth.chain(5, 90)
+    structs = [h1, h2, cmp_, tie, prot]
+    res = [b1, b2, synth.na_bases(rng, 150), "".join(tb), synth.residues(rng, 90)]
+    c = Chunk.from_structures(structs, res)
+    return synth.mark_nucleic(c, [c.order[i] != 4 for i in range(c.nstrs)])
+
+
+def na_pairs_brute_force(c, i):
+    """Base-pairing letters of nucleic-acid structure i by brute force over
+    all position pairs (numpy FP32): partner = the complementary residue more
+    than 2 positions away whose C3' distance deviates least from 13.75 A
+    (at most 1.25 A), lowest index on ties; mutual partners y < x give '('
+    at y and ')' at x."""
+    X = c.coords(i).astype(np.float32)
+    r = np.frombuffer(c.residues(i).encode(), np.uint8)
+    L = len(r)
+    d = X[:, None, :] - X[None, :, :]
+    dist = np.sqrt((d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1]) + d[..., 2] * d[..., 2])
+    A, C_, G, T, U = (ord(ch) for ch in "ACGTU")
+    a, b = r[:, None], r[None, :]
+    comp = (((a == T) | (a == U)) & (b == A)) | ((a == A) & ((b == T) | (b == U))) | \
+           ((a == G) & ((b == C_) | (b == U))) | (((a == C_) | (a == U)) & (b == G))
+    idx = np.arange(L)
+    ok = comp & (np.abs(idx[:, None] - idx[None, :]) > 2)
+    dev = np.abs(dist - np.float32(13.75))
+    dev = np.where(ok & (dev <= 1.25), dev, np.float32(np.inf))
+    part = np.where(np.isfinite(dev.min(1)), dev.argmin(1), -1)
+    out = np.full(L, ord("."), np.uint8)
+    for y in range(L):
+        p = part[y]
+        if p > y and part[p] == y:
+            out[y], out[p] = ord("("), ord(")")
+    return out
+
+
+def test_na_secstr_is_the_least_deviation_pairing():
+    """The oracle's base-pairing SS (the stage's definition) equals the brute
+    force above; protein chains keep the C-alpha rule."""
+    c = na_test_chunk()
+    o = oracle.secstr(c)
+    k = {int(c.order[i]): i for i in range(c.nstrs)}
+    for i in range(c.nstrs):
+        d, l = int(c.dst[i]), int(c.len[i])
+        if c.order[i] == 4:
+            prot = o[d:d + l].tobytes()
+            assert set(prot) <= set(b" het") and b"." not in prot
+            continue
+        assert np.array_equal(o[d:d + l], na_pairs_brute_force(c, i)), c.order[i]
+    tie = o[c.dst[k[3]]:c.dst[k[3]] + 201].tobytes()
+    assert tie[0:1] == b"(" and tie[5:6] == b")" and tie[130:131] == b"."
+    stem = o[c.dst[k[1]]:c.dst[k[1]] + c.len[k[1]]].tobytes()
+    assert stem.count(b"(") >= 25 and stem.count(b"(") == stem.count(b")")
+
+
+def test_na_secstr_cpu_twin_matches_reference(have_ref):
+    """The reference's CPU twin as compiled records for each residue the
+    index left by its vectorised minimum (see oracle.set_na_ss_mode); with
+    that one step switched, the oracle's SS equals the reference's MpSecStr
+    letter for letter and partner for partner, which pins the shared parts
+    (pairing rule, distances, deviation bounds, mutual check)."""
+    if not have_ref:
+        pytest.skip("reference CPU build (oracle/_ref) not present")
+    c = na_test_chunk()
+    prev = oracle.set_na_ss_mode(1)
+    try:
+        o = oracle.secstr(c)
+    finally:
+        oracle.set_na_ss_mode(prev)
+    r, rpos = oracle.ref_secstr(c, with_pairs=True)
+    assert np.array_equal(o, r), np.nonzero(o != r)[0][:10]
+    # the compiled minimum loses pairs the least-deviation rule finds (69 vs 48 here)
+    assert (oracle.secstr(c) == ord("(")).sum() > (r == ord("(")).sum()
+
+
+def test_na_stage_matches_reference(have_ref):
+    """The whole stage on nucleic-acid chains (SS-weighted stage-2 DP over
+    base-pairing SS, nucleic-acid d0 of the final stage): bit for bit the
+    reference's CPU path, with the oracle's SS step in the CPU twin's mode."""
+    if not have_ref:
+        pytest.skip("reference CPU build (oracle/_ref) not present")
+    q, r = na_stage_chunks()
+    opts = default_options(qgroup=2)
+    prev = oracle.set_na_ss_mode(1)
+    try:
+        o = oracle.process_block(q, r, opts)
+    finally:
+        oracle.set_na_ss_mode(prev)
+    prev = oracle.ref_set_move_slots(True)
+    try:
+        ref = oracle.ref_process_block(q, r, opts)
+    finally:
+        oracle.ref_set_move_slots(prev)
+    assert_kept_pairs_equal(o, ref)
+    # the copy aligns; the protein reference never does
+    k = {int(r.order[i]): i for i in range(r.nstrs)}
+    qk = {int(q.order[i]): i for i in range(q.nstrs)}
+    assert o.alndata[qk[0], k[0], 9] > 0.5
+    assert (o.alndata[:, k[3], 8] >= 9999).all()
+
+
+def na_stage_chunks():
+    from gtalign_alpha_amd.chunk import Chunk
+    rng = np.random.default_rng(12)
+    h, b = synth.na_hairpin(rng, 25, 6)
+    qs = [h, synth.na_compact(51, 80)]
+    qb = [b, synth.na_bases(rng, 80)]
+    R, t = random_rotation(rng), rng.normal(scale=20.0, size=3)
+    rs = [(h @ R.T + t + rng.normal(scale=0.5, size=h.shape)).astype(np.float32), synth.na_compact(52, 70),
+          synth.na_hairpin(rng, 18, 4)[0], synth.chain(53, 60)]
+    rb = [b, synth.na_bases(rng, 70), synth.na_bases(rng, 40), synth.residues(rng, 60)]
+    q = Chunk.from_structures(qs, qb)
+    r = Chunk.from_structures(rs, rb)
+    q = synth.mark_nucleic(q, [True] * q.nstrs)
+    r = synth.mark_nucleic(r, [r.order[i] != 3 for i in range(r.nstrs)])
+    return q, r

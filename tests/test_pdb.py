@@ -84,6 +84,21 @@ def test_short_empty_and_length_limit():
         pdb.read_pdb(bad[:30] + "     abc" + bad[38:] + "\n")
 
 
+def test_max_length_rule_of_the_reference_reader():
+    """PMBatchStrData::AddOneResidue (PMBatchStrData.h:368-386) abandons the
+    structure when a residue arrives while more than maxstrlen residues are
+    stored: with --dev-max-length 4000 a 4,001-residue chain is kept and a
+    4,002-residue one dropped; no limit for queries."""
+    def ca_chain(n):
+        return "\n".join(atom(k + 1, " CA ", "ALA", "A", k % 9999 + 1, k * 0.001, 0.0, 0.0) for k in range(n))
+    kept = pdb.read_pdb(ca_chain(4001), max_length=pdb.DEV_MAX_LENGTH)
+    assert kept is not None and len(kept) == 4001
+    assert pdb.read_pdb(ca_chain(4002), max_length=pdb.DEV_MAX_LENGTH) is None
+    assert len(pdb.read_pdb(ca_chain(4002))) == 4002
+    assert pdb.read_pdb(ca_chain(21), max_length=19) is None
+    assert len(pdb.read_pdb(ca_chain(20), max_length=19)) == 20
+
+
 def test_nucleic_acid_c3_prime():
     lines = []
     for k in range(24):
@@ -114,6 +129,8 @@ def test_chunk_from_files(tmp_path):
     (tmp_path / "short.pdb").write_text("\n".join(protein_chain("A", 5)))
     c, kept = pdb.read_chunk(paths)
     assert kept == [0, 1, 2, 3]
+    c40, kept40 = pdb.read_chunk(paths, max_length=40)   # keeps chains of <= 41 residues
+    assert kept40 == [0, 2] and c40.len.tolist() == [33, 21]
     assert c.len.tolist() == sorted(lens, reverse=True)
     for k in range(c.nstrs):
         assert np.array_equal(c.coords(k), ref[c.order[k]])
