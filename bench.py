@@ -38,6 +38,9 @@ from gtalign_alpha_amd.capi import default_options, PREC_FP32, PREC_MIXED  # noq
 QGROUP = 2
 
 HBM_PEAK = 8.0e12
+FP32_PEAK = 157.3e12      # MI355X FP32 vector (and dense matrix) peak
+# rocprofv3 SQ counter summary (VALU issue) of the same workload's kernels
+VALU_PROFILE = "profiles/refine_dp_16x50000_valu_issue.md"
 # rocprofv3 FETCH_SIZE/WRITE_SIZE summary of the 16 x 50,000 workload's kernels
 TRAFFIC_PROFILE = "profiles/refine_dp_16x50000_hbm_traffic.md"
 # kernel class -> the kernel (template instance) rocprofv3 lists for it in the plain FP32 form
@@ -64,6 +67,47 @@ def profiled_traffic(kernel):
             if len(parts) >= 3 and parts[0] == "traffic_bytes_per_launch" and " ".join(parts[1:-1]) == kernel:
                 return int(parts[-1])
     return None
+
+
+def profiled_valu(kernel):
+    """VALU-issue fraction of `kernel` on the full workload from the committed
+    SQ counter summary: SQ_ACTIVE_INST_VALU per dispatch against the SIMD
+    cycles of its duration, each scaled by what the counter reads for a
+    calibration kernel of known VALU instruction count; None when absent."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), VALU_PROFILE)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) >= 3 and parts[0] == "valu_issue_frac" and " ".join(parts[1:-1]) == kernel:
+                return float(parts[-1])
+    return None
+
+
+def valu_roofline(dom, kname, full):
+    """Roofline of the dominant kernel class.  Its passes re-read each pair's
+    coordinates from LDS and L2 many times per launch, so the per-pass bytes
+    of the algorithm (24-28 B per aligned pair and pass) exceed what HBM
+    delivers (the rocprofv3 FETCH/WRITE counters), and the kernel sits on the
+    vector ALU: the roofline is its algorithmic FP32 operations per second
+    against the MI355X FP32 vector peak, with the measured HBM rate and the
+    counters' VALU-issue fraction beside it."""
+    launches = dom["launches"]
+    t = dom["seconds"] / launches
+    flops = dom["alg_flops"] / launches
+    kern = kname.split(" (")[0]
+    traffic = profiled_traffic(kern) if full else None
+    return {"bound": "valu", "achieved": round(flops / t / 1e12, 3), "peak": FP32_PEAK / 1e12,
+            "unit": "TFLOP/s", "frac": round(flops / t / FP32_PEAK, 4), "traffic": traffic,
+            "hbm_GBps": round(traffic / t / 1e9, 1) if traffic else None,
+            "hbm_frac": round(traffic / t / HBM_PEAK, 5) if traffic else None,
+            "traffic_source": (TRAFFIC_PROFILE + ": rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 per dispatch"
+                               if traffic else None),
+            "valu_issue_frac": profiled_valu(kern) if full else None,
+            "valu_source": VALU_PROFILE if full else None,
+            "kernel": kname, "launches": launches, "launch_ms": round(1e3 * t, 3),
+            "alg_flops_per_launch": flops, "pass_bytes_per_launch": dom["alg_bytes"] / launches}
 
 
 def log(*a):
@@ -100,43 +144,55 @@ def time_form(qc, rc, prec, steps, warmup, profile, batched=0, tune=None, **opt)
             forms, secs = b.dp_forms()
             tune.update({"tuning_seconds": round(secs, 2), "pairs_per_form": forms,
                          "cache_lines": open(cache).read().splitlines() if os.path.exists(cache) else []})
+        launched = b.dp_cell_counts()[1] * steps
         stats = None
         if profile:
             b.set_profiling(True)
             s, _ = b.process_resident()
             stats = b.kernel_stats()
             log("  profiled step: %.3f s" % s)
-    return wall, dev, cells, stats
+    return wall, dev, (cells, launched), stats
 
 
-def pcie_inclusive(qc, rc):
-    """The boundary takes host buffers: the plain FP32 form timed with the
-    references' upload (host -> HBM) and the results' download (HBM -> host,
-    records, transforms and alignment rows) inside the clock, after one
-    untimed call on the same context."""
+def pcie_inclusive(qc, rc, steps, warmup):
+    """The boundary takes host buffers: the plain FP32 form timed over `steps`
+    calls with the references' upload (host -> HBM, every call) and the
+    results' download (HBM -> host: records, transforms and alignment rows)
+    inside the clock, after `warmup` untimed calls on the same context.
+    Returns the figures and the last call's results."""
     with GtaBatch(0, default_options(qgroup=QGROUP), max_qrys=qc.nstrs, max_qry_poss=qc.nposs,
                   max_refs=rc.nstrs, max_ref_poss=rc.nposs) as b:
         b.TransferQueryPMDataToDevice(qc)
         b.TransferCPMDataToDevice(rc)
-        b.ProcessBlock()
+        for _ in range(max(1, warmup)):
+            b.ProcessBlock()
+        up = 0.0
         t0 = time.perf_counter()
-        b.TransferCPMDataToDevice(rc)
-        t1 = time.perf_counter()
-        b.ProcessBlock()
-        t2 = time.perf_counter()
+        for _ in range(steps):
+            t1 = time.perf_counter()
+            b.TransferCPMDataToDevice(rc)
+            up += time.perf_counter() - t1
+            g = b.ProcessBlock()
+        wall = time.perf_counter() - t0
     n = qc.nstrs * rc.nstrs
-    return {"pairs_per_s": round(n / (t2 - t0), 1), "upload_s": round(t1 - t0, 4),
-            "process_and_download_s": round(t2 - t1, 4)}
+    return {"pairs_per_s": round(n * steps / wall, 1), "steps": steps,
+            "upload_s_per_step": round(up / steps, 4),
+            "process_and_download_s_per_step": round((wall - up) / steps, 4)}, g
 
 
-def _ref_run(q, r, nthreads, tmp):
+def _ref_run(q, r, nthreads, tmp, fast=False):
     """The reference's CPU path on (q, r) in a child process (its thread count
-    is fixed per process); returns (alndata, seconds, passed)."""
+    is fixed per process); fast selects the reference's own release build
+    (-O3 -ffast-math) instead of the deterministic parity build.  Returns
+    (alndata, seconds, passed)."""
     import subprocess
     from oracle.ref_cli import save_pair
-    src, dst = os.path.join(tmp, "in_%d.npz" % nthreads), os.path.join(tmp, "out_%d.npz" % nthreads)
+    tag = "%d%s" % (nthreads, "f" if fast else "")
+    src, dst = os.path.join(tmp, "in_%s.npz" % tag), os.path.join(tmp, "out_%s.npz" % tag)
     save_pair(src, q, r)
-    subprocess.run([sys.executable, "-m", "oracle.ref_cli", src, dst, str(nthreads)], cwd=ROOT, check=True)
+    env = dict(os.environ, GTA_REF_FAST="1" if fast else "0", OMP_NUM_THREADS=str(nthreads))
+    subprocess.run([sys.executable, "-m", "oracle.ref_cli", src, dst, str(nthreads)], cwd=ROOT, check=True,
+                   env=env)
     d = np.load(dst, allow_pickle=False)
     return (d["alndata"].reshape(q.nstrs, r.nstrs, -1), float(d["seconds"]),
             d["passed"].reshape(q.nstrs, r.nstrs).astype(bool))
@@ -193,11 +249,60 @@ def _tm_agreement(ref, g, passed=None):
             "tm_err_max": float(dtm.max()) if dtm.size else 0.0, "pairs_tm_err_gt_1e-4": int((dtm > 1e-4).sum())}
 
 
+def _attribute(g, ref_n, pass_n, nthreads):
+    """Each pair where the GPU and the reference at `nthreads` threads differ
+    (aligned or not, or TM-score by more than 1e-4), attributed with the
+    committed 1-thread runs of the same sample (tests/golden/scale_4x8000.npz,
+    written by tests/golden/make_scale_golden.py): `thread_count` when the
+    GPU equals the 1-thread reference and that run's pair is the same with
+    and without per-slot rows moved at packing; `packing` when the GPU
+    equals the 1-thread run with rows moved but not the one without (the
+    reference's stale per-slot rows); `unexplained` otherwise."""
+    from tests.golden.make_scale_golden import FIELDS, OUT
+    d = np.load(os.path.join(ROOT, OUT), allow_pickle=False)
+    nq, nr = ref_n.shape[:2]
+    n = nq * nr
+    col = {f: i for i, f in enumerate(FIELDS)}
+
+    def unpack(tag):
+        al = np.unpackbits(d[tag + "_aligned"])[:n].reshape(nq, nr).astype(bool)
+        tm = np.full((nq, nr, 2), -1.0)
+        tm[al] = d[tag + "_values"][:, [col[9], col[10]]]
+        return al, tm
+
+    def same(al_a, tm_a, al_b, tm_b):
+        return (al_a == al_b) & (~al_a | (np.abs(tm_a - tm_b).max(-1) <= 1e-4))
+
+    g_al = g.alndata[..., 8] < 9999
+    g_tm = np.where(g_al[..., None], g.alndata[..., 9:11], -1.0)
+    n_al = pass_n & (ref_n[..., 8] < 9999)
+    n_tm = np.where(n_al[..., None], ref_n[..., 9:11], -1.0)
+    m_al, m_tm = unpack("moved")
+    p_al, p_tm = unpack("plain")
+    diff = ~same(g_al, g_tm, n_al, n_tm)
+    g_eq_m = same(g_al, g_tm, m_al, m_tm)
+    m_eq_p = same(m_al, m_tm, p_al, p_tm)
+    out = {"pairs_differing": int(diff.sum()),
+           "thread_count": int((diff & g_eq_m & m_eq_p).sum()),
+           "packing": int((diff & g_eq_m & ~m_eq_p).sum()),
+           "unexplained": int((diff & ~g_eq_m).sum()),
+           "gpu_vs_1thread_moved": {"pairs_differing": int((~g_eq_m).sum()),
+                                    "same_aligned_set": bool(np.array_equal(g_al, m_al)),
+                                    "aligned_pairs": int(m_al.sum())},
+           "reference_%d_vs_1thread_moved_pairs_differing" % nthreads: int((~same(n_al, n_tm, m_al, m_tm)).sum())}
+    out["pairs"] = [{"q": int(a), "r": int(b), "cause": ("thread_count" if g_eq_m[a, b] and m_eq_p[a, b] else
+                                                          "packing" if g_eq_m[a, b] else "unexplained")}
+                    for a, b in np.argwhere(diff)[:64]]
+    return out
+
+
 def cpu_baseline(qc, rc, src):
     """The reference's own CPU path (oracle/_ref) on a bounded sample of the
-    workload: timed with the host's threads; the GPU's agreement is checked
-    against a single-thread run (the reference's result for a few pairs
-    depends on its thread count)."""
+    workload (queries 0, 5, 10, 15 x 8,000 references): timed in its own
+    release build (-O3 -ffast-math) on every core of this process's affinity
+    mask, and run in the deterministic parity build on as many threads for
+    the GPU's agreement, each disagreement attributed to the reference's
+    thread count or packing with the committed 1-thread runs."""
     import tempfile
     from oracle import oracle
     rng = np.random.default_rng(7)
@@ -217,10 +322,14 @@ def cpu_baseline(qc, rc, src):
                "sample": "%d queries x %d references of the same workload, %.1f s" % (q.nstrs, r.nstrs, dt)}
         out["gpu_vs_port"] = _tm_agreement(res.alndata, _gpu_block(q, r))
         return out
-    threads = max(1, min(16, os.cpu_count() or 1))
+    affinity = sorted(os.sched_getaffinity(0))
+    threads = len(affinity)
+    fast = os.path.exists(oracle.REF_FAST_LIB)
     with tempfile.TemporaryDirectory() as tmp:
-        ref_t, dt, pass_t = _ref_run(q, r, threads, tmp)
-        # single-thread parity sample: 2 queries x (their homologs + as many others)
+        _, dt, _ = _ref_run(q, r, threads, tmp, fast=fast)
+        ref_t, dt_parity, pass_t = _ref_run(q, r, threads, tmp)
+        # single-thread parity sample for the DP identity: 2 queries x (their
+        # homologs + as many others)
         qs = q.subset([0, 3])
         rsrc = src[rsel]
         hom = np.flatnonzero((rsrc == qsel[0]) | (rsrc == qsel[3]))[:200]
@@ -229,16 +338,19 @@ def cpu_baseline(qc, rc, src):
         rs = r.subset(np.union1d(hom, oth))
         ref_1, _, pass_1 = _ref_run(qs, rs, 1, tmp)
     out = {"value": npairs / dt, "unit": "pairs/s", "cores": threads, "kind": "reference",
-           "sample": "%d queries x %d references of the same workload (%d pairs), %.1f s with %d threads" %
-                     (q.nstrs, r.nstrs, npairs, dt, threads)}
+           "sample": "%d queries x %d references of the same workload (%d pairs), %.1f s with %d threads, "
+                     "reference release build (-O3 -ffast-math)%s" %
+                     (q.nstrs, r.nstrs, npairs, dt, threads, "" if fast else " absent: parity build timed"),
+           "host": {"nproc": os.cpu_count(), "affinity_cpus": threads,
+                    "affinity": "%d-%d" % (affinity[0], affinity[-1]) if affinity else ""},
+           "parity_build_pairs_per_s": round(npairs / dt_parity, 1)}
     g32, gmx = _gpu_block(qs, rs, PREC_FP32), _gpu_block(qs, rs, PREC_MIXED)
     out["gpu_vs_reference_1thread"] = {"sample": "%d x %d pairs" % (qs.nstrs, rs.nstrs),
                                        "plain": _tm_agreement(ref_1, g32, pass_1),
                                        "mixed": _tm_agreement(ref_1, gmx, pass_1)}
-    out["gpu_vs_reference_%dthreads" % threads] = _tm_agreement(ref_t, _gpu_block(q, r), pass_t)
-    # FP64 recomputation of every reported TM-score (under the reported
-    # transform) and RMSD (SVD superposition of the aligned pairs)
-    out["vs_fp64"] = {"plain": _fp64_summary(g32, qs, rs), "mixed": _fp64_summary(gmx, qs, rs)}
+    g = _gpu_block(q, r)
+    out["gpu_vs_reference_%dthreads" % threads] = _tm_agreement(ref_t, g, pass_t)
+    out["disagreements_attributed"] = _attribute(g, ref_t, pass_t, threads)
     frac, n = _dp_identity(qs, rs, g32.tfms)
     out["dp_bit_identical"] = {"fraction": frac, "fills": n}
     return out
@@ -323,9 +435,9 @@ def batched_vs_plain(steps, warmup):
                                        ("batched_mixed_autotuned", PREC_MIXED, 1, True)):
         log(" " + name)
         tune = {} if tuned else None
-        wall, dev, cells, _ = time_form(qc, rc, prec, steps, warmup, False, batched, tune)
+        wall, dev, (cells, launched), _ = time_form(qc, rc, prec, steps, warmup, False, batched, tune)
         out[name] = {"pairs_per_s": round(npairs * steps / wall, 1), "ms_per_step": round(1e3 * wall / steps, 2),
-                     "dp_gcups": round(cells / dev / 1e9, 2)}
+                     "dp_gcups": round(cells / dev / 1e9, 2), "dp_gcups_launched": round(launched / dev / 1e9, 2)}
         if tuned:
             out[name]["dp_autotune"] = tune
     plain = out["plain_fp32"]["pairs_per_s"]
@@ -370,9 +482,7 @@ def all_vs_all(steps, warmup):
             "form": "batched, mixed precision", "pairs": npairs,
             "pairs_per_s": round(npairs * steps / wall, 1), "ms_per_step": round(1e3 * wall / steps, 2),
             "dp_gcups": round(cells / dev / 1e9, 2),
-            "roofline": {"kernel": KERNEL_NAMES.get(dom["name"], dom["name"]),
-                         "achieved_GBps": round(dom["alg_bytes"] / dom["seconds"] / 1e9, 1),
-                         "frac": round(dom["alg_bytes"] / dom["seconds"] / HBM_PEAK, 4)},
+            "roofline": valu_roofline(dom, KERNEL_NAMES.get(dom["name"], dom["name"]), False),
             "aligned_pairs": int(len(dtm)),
             "vs_fp64": {"tm_err_max": float(dtm.max()) if len(dtm) else None,
                         "rmsd_err_max": float(drm.max()) if len(drm) else None}}
@@ -417,11 +527,15 @@ def main():
         (qc.nstrs, rc.nstrs, npairs, qc.nposs, rc.nposs, time.perf_counter() - t0))
 
     log("plain FP32 form")
-    wall, dev, cells, stats = time_form(qc, rc, PREC_FP32, args.steps, args.warmup, True)
+    wall, dev, (cells, launched), stats = time_form(qc, rc, PREC_FP32, args.steps, args.warmup, True)
     ms = 1e3 * wall / args.steps
     value = npairs * args.steps / wall
-    log("PCIe-inclusive call")
-    pcie = pcie_inclusive(qc, rc)
+    log("PCIe-inclusive calls (upload + stage + download)")
+    pcie, g_full = pcie_inclusive(qc, rc, args.steps, args.warmup)
+    log("FP64 recomputation of every aligned pair")
+    from oracle.fp64 import fp64_errors_all
+    dtm_all, drm_all = fp64_errors_all(g_full, qc, rc)
+    del g_full
     mixed = None
     if not args.no_mixed:
         log("mixed-precision form")
@@ -450,20 +564,12 @@ def main():
         if st["launches"] == 0:
             continue
         kern[st["name"]] = {"launches": st["launches"], "seconds": round(st["seconds"], 6),
-                            "alg_GB": round(st["alg_bytes"] / 1e9, 3)}
+                            "alg_GB": round(st["alg_bytes"] / 1e9, 3),
+                            "alg_TFLOP": round(st["alg_flops"] / 1e12, 4)}
         if dom is None or st["seconds"] > dom["seconds"]:
             dom = st
-    achieved = dom["alg_bytes"] / dom["seconds"]
     kname = KERNEL_NAMES.get(dom["name"], dom["name"])
-    traffic = profiled_traffic(kname.split(" (")[0]) if args.scale == 1.0 else None
-    roofline = {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic,
-                "traffic_source": (TRAFFIC_PROFILE + ": rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 per dispatch"
-                                   if traffic else None),
-                "kernel": kname,
-                "launches": dom["launches"],
-                "launch_ms": round(1e3 * dom["seconds"] / dom["launches"], 3),
-                "alg_bytes_per_launch": dom["alg_bytes"] / dom["launches"]}
+    roofline = valu_roofline(dom, kname, args.scale == 1.0)
 
     batched = None
     if not args.no_batched:
@@ -497,11 +603,21 @@ def main():
         "dtype": "fp32",
         "data": "synthetic",
         "config": {"workload": "16 queries x %d references, lognormal(250,0.5) in [30,1000], "
-                               "20%% homologs; default options" % rc.nstrs,
+                               "20%% homologs; default options with --no-detailed-search" % rc.nstrs,
                    "pairs": npairs, "form": "plain FP32, one chunk per launch sequence",
                    "query_blocks": QGROUP},
         "dp_gcups": round(cells / dev / 1e9, 2),
+        "dp_gcups_launched": round(launched / dev / 1e9, 2),
         "dp_kernel_gcups": round(cells / args.steps / kern["dp"]["seconds"] / 1e9, 2) if "dp" in kern else None,
+        "dp_kernel_gcups_launched": (round(launched / args.steps / kern["dp"]["seconds"] / 1e9, 2)
+                                     if "dp" in kern else None),
+        "vs_fp64_all_aligned": {"aligned_pairs": int(len(dtm_all)),
+                                "tm_err_max": float(dtm_all.max()) if len(dtm_all) else None,
+                                "tm_err_mean": float(dtm_all.mean()) if len(dtm_all) else None,
+                                "pairs_tm_err_gt_1e-4": int((dtm_all > 1e-4).sum()),
+                                "rmsd_err_max": float(drm_all.max()) if len(drm_all) else None,
+                                "rmsd_err_mean": float(drm_all.mean()) if len(drm_all) else None,
+                                "form": "plain FP32 (RMSD from the reference's FP32 E0 identity)"},
         "mixed_precision_pairs_per_s": round(mixed, 1) if mixed else None,
         "search_by_ss": ssrr,
         "pcie_inclusive": pcie,

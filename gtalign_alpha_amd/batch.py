@@ -22,7 +22,7 @@ _lib = None
 
 class _KStat(C.Structure):
     _fields_ = [("name", C.c_char_p), ("launches", C.c_int), ("seconds", C.c_double),
-                ("alg_bytes", C.c_double)]
+                ("alg_bytes", C.c_double), ("alg_flops", C.c_double)]
 
 
 class GtaError(RuntimeError):
@@ -59,6 +59,8 @@ def lib():
         L.gta_set_profiling.restype = C.c_int
         L.gta_kernel_stats.argtypes = [C.c_void_p, C.POINTER(_KStat), C.c_int]
         L.gta_kernel_stats.restype = C.c_int
+        L.gta_dp_cell_counts.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.gta_dp_cell_counts.restype = C.c_int
         L.gta_test_dp.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_float, C.c_int, C.c_int,
                                   C.c_void_p, C.c_void_p, C.c_void_p]
         L.gta_test_dp.restype = C.c_int
@@ -143,7 +145,14 @@ class GtaBatch:
         arr = (_KStat * 16)()
         n = lib().gta_kernel_stats(self._h, arr, 16)
         return [{"name": arr[i].name.decode(), "launches": arr[i].launches,
-                 "seconds": arr[i].seconds, "alg_bytes": arr[i].alg_bytes} for i in range(n)]
+                 "seconds": arr[i].seconds, "alg_bytes": arr[i].alg_bytes, "alg_flops": arr[i].alg_flops}
+                for i in range(n)]
+
+    def dp_cell_counts(self):
+        """DP cells of the last step: (useful Lq x Lr sums, launched lane-steps)."""
+        u, l = C.c_double(), C.c_double()
+        self._check(lib().gta_dp_cell_counts(self._h, C.byref(u), C.byref(l)))
+        return u.value, l.value
 
     def pair_score(self, d02, d2):
         """The device's pair score d02 / (d02 + d2) for arrays of operands

@@ -34,12 +34,17 @@ struct DevBuf {
     size_t bytes = 0;
     hipError_t ensure(size_t n) {
         if (n <= bytes && p) return hipSuccess;
+        const size_t need = std::max<size_t>(n, 256);
         if (p) { (void)hipFree(p); p = nullptr; bytes = 0; }
-        hipError_t e = hipMalloc(&p, std::max<size_t>(n, 256));
-        if (e == hipSuccess) bytes = std::max<size_t>(n, 256);
+        hipError_t e = hipMalloc(&p, need);
+        if (e == hipSuccess) bytes = need;
         return e;
     }
-    void release() { if (p) (void)hipFree(p); p = nullptr; bytes = 0; }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
     template <typename T> T* as() const { return (T*)p; }
 };
 
@@ -122,6 +127,8 @@ struct gta_ctx {
     int kc_launches[KC_COUNT] = {0};
     double kc_seconds[KC_COUNT] = {0};
     double kc_bytes[KC_COUNT] = {0};
+    double kc_flops[KC_COUNT] = {0};
+    double cells_useful = 0, cells_launched = 0;  // DP cells of the last run
     // host results
     std::vector<float> h_out, h_tfm;
     std::vector<char> h_alns;
@@ -213,21 +220,27 @@ int gta_create(int device, const gta_options* opts, int max_qrys, int max_qry_po
     return GTA_OK;
 }
 
+// every device buffer of a context
+static std::vector<DevBuf*> all_bufs(gta_ctx* c) {
+    return {&c->q.b_len, &c->q.b_dst, &c->q.b_na, &c->q.b_x, &c->q.b_y, &c->q.b_z, &c->q.b_rsd, &c->q.b_ss,
+            &c->q.b_napos, &c->r.b_len, &c->r.b_dst, &c->r.b_na, &c->r.b_x, &c->r.b_y, &c->r.b_z,
+            &c->r.b_rsd, &c->r.b_ss, &c->r.b_napos,
+            &c->s_flags, &c->s_grand, &c->s_best0, &c->s_prev, &c->s_naln, &c->s_qpos,
+            &c->s_rpos, &c->s_fndx, &c->s_fpos, &c->s_tibest, &c->s_tgrand, &c->s_out,
+            &c->s_aln, &c->s_btck, &c->s_btckoff, &c->s_rowbuf, &c->s_cells, &c->s_alns,
+            &c->s_algb, &c->s_dpsmall, &c->s_dpbig, &c->s_ring, &c->s_ringoff, &c->s_fail,
+            &c->s_gstrip0, &c->s_gsync,
+            &c->s_rsmall, &c->s_rbig, &c->s_parts, &c->s_bigmask, &c->s_lista, &c->s_listb,
+            &c->s_listc, &c->s_lcnt, &c->s_ltot, &c->s_heads, &c->s_sctfm, &c->s_scflag,
+            &c->s_scout};
+}
+
 void gta_destroy(gta_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    c->q.release(); c->r.release();
-    DevBuf* bufs[] = {&c->s_flags, &c->s_grand, &c->s_best0, &c->s_prev, &c->s_naln, &c->s_qpos,
-                      &c->s_rpos, &c->s_fndx, &c->s_fpos, &c->s_tibest, &c->s_tgrand, &c->s_out,
-                      &c->s_aln, &c->s_btck, &c->s_btckoff, &c->s_rowbuf, &c->s_cells, &c->s_alns,
-                      &c->s_algb, &c->s_dpsmall, &c->s_dpbig, &c->s_ring, &c->s_ringoff, &c->s_fail,
-                      &c->s_gstrip0, &c->s_gsync,
-                      &c->s_rsmall, &c->s_rbig, &c->s_parts, &c->s_bigmask, &c->s_lista, &c->s_listb,
-                      &c->s_listc, &c->s_lcnt, &c->s_ltot, &c->s_heads, &c->s_sctfm, &c->s_scflag,
-                      &c->s_scout};
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
-    for (DevBuf* b : bufs) b->release();
+    for (DevBuf* b : all_bufs(c)) b->release();
     for (hipEvent_t e : c->evpool) (void)hipEventDestroy(e);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -549,8 +562,8 @@ static int prepare_state(gta_ctx* c, DevState& S, size_t& alnsize) {
     HIPCHK(c, c->s_out.ensure(np * 15 * 4));
     const size_t alnplane = (size_t)nq * c->r.nposs;
     HIPCHK(c, c->s_aln.ensure(alnplane * 9 * 4));
-    HIPCHK(c, c->s_cells.ensure(8));
-    HIPCHK(c, c->s_algb.ensure(KC_COUNT * 8));
+    HIPCHK(c, c->s_cells.ensure(16));
+    HIPCHK(c, c->s_algb.ensure(KC_COUNT * 16));
     // backtrace: per pair ceil(Lq/64) strips x ceil((Lr+63)/16) blocks x 64 words
     std::vector<unsigned long long> off(np);
     unsigned long long tot = 0;
@@ -580,6 +593,7 @@ static int prepare_state(gta_ctx* c, DevState& S, size_t& alnsize) {
     S.rowbuf = c->s_rowbuf.as<float>();
     S.cells = c->s_cells.as<unsigned long long>();
     S.algbytes = c->s_algb.as<unsigned long long>();
+    S.algflops = S.algbytes + KC_COUNT;
     // DP launch forms (one wave, W waves or the whole device per pair)
     {
         int rc = build_dp_buckets(c, S);
@@ -749,8 +763,8 @@ static int run_stage(gta_ctx* c, double* seconds, double* cells) {
     const bool graph = batched && !c->profiling;
     auto enqueue = [&]() -> int {
         // cleared by kernels (no memset nodes in a captured graph)
-        HIPCHK(c, launch_fill_i32((int*)c->s_cells.p, 0, 2, st));
-        HIPCHK(c, launch_fill_i32((int*)c->s_algb.p, 0, KC_COUNT * 2, st));
+        HIPCHK(c, launch_fill_i32((int*)c->s_cells.p, 0, 4, st));
+        HIPCHK(c, launch_fill_i32((int*)c->s_algb.p, 0, KC_COUNT * 4, st));
         HIPCHK(c, launch_fill_i32((int*)c->s_alns.p, 0, (alnsize + 3) / 4, st));
         HIPCHK(c, launch_fill_i32(c->dpb.failflag, 0, 1, st));
         if (batched) HIPCHK(c, launch_fill_i32(heads, 0, GTA_MAX_QUEUES, st));
@@ -849,10 +863,12 @@ static int run_stage(gta_ctx* c, double* seconds, double* cells) {
     HIPCHK(c, hipEventSynchronize(c->ev1));
     float ms = 0.f;
     HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
-    unsigned long long ncells = 0;
-    HIPCHK(c, hipMemcpy(&ncells, c->s_cells.p, 8, hipMemcpyDeviceToHost));
+    unsigned long long ncells[2] = {0, 0};
+    HIPCHK(c, hipMemcpy(ncells, c->s_cells.p, 16, hipMemcpyDeviceToHost));
+    c->cells_useful = (double)ncells[0];
+    c->cells_launched = (double)ncells[1];
     if (seconds) *seconds = ms * 1e-3;
-    if (cells) *cells = (double)ncells;
+    if (cells) *cells = (double)ncells[0];
     for (int k = 0; k < KC_COUNT; k++) { c->kc_launches[k] = 0; c->kc_seconds[k] = 0; }
     if (c->profiling) {
         for (const auto& e : c->evrec) {
@@ -865,15 +881,25 @@ static int run_stage(gta_ctx* c, double* seconds, double* cells) {
     int dpfail = 0;
     HIPCHK(c, hipMemcpy(&dpfail, c->s_fail.p, 4, hipMemcpyDeviceToHost));
     if (dpfail) return fail(c, GTA_ERR_HIP, "multi-wave or grid DP: a row wait exceeded its spin limit");
-    unsigned long long ab[KC_COUNT];
+    unsigned long long ab[2 * KC_COUNT];
     HIPCHK(c, hipMemcpy(ab, c->s_algb.p, sizeof(ab), hipMemcpyDeviceToHost));
-    for (int k = 0; k < KC_COUNT; k++) c->kc_bytes[k] = (double)ab[k];
+    for (int k = 0; k < KC_COUNT; k++) {
+        c->kc_bytes[k] = (double)ab[k];
+        c->kc_flops[k] = (double)ab[KC_COUNT + k];
+    }
     return GTA_OK;
 }
 
 int gta_set_profiling(gta_ctx* c, int on) {
     if (!c) return fail(nullptr, GTA_ERR_ARG, "null context");
     c->profiling = on ? 1 : 0;
+    return GTA_OK;
+}
+
+int gta_dp_cell_counts(const gta_ctx* c, double* useful, double* launched) {
+    if (!c || !useful || !launched) return GTA_ERR_ARG;
+    *useful = c->cells_useful;
+    *launched = c->cells_launched;
     return GTA_OK;
 }
 
@@ -886,6 +912,7 @@ int gta_kernel_stats(const gta_ctx* c, gta_kernel_stat* out, int n) {
         out[k].launches = c->kc_launches[k];
         out[k].seconds = c->kc_seconds[k];
         out[k].alg_bytes = c->kc_bytes[k];
+        out[k].alg_flops = c->kc_flops[k];
     }
     return KC_COUNT;
 }

@@ -84,8 +84,9 @@ struct DevState {
     unsigned* btck;         // packed backtrace codes
     const unsigned long long* btckoff;  // per-pair offsets (in words)
     float* rowbuf;          // strip boundary rows for long references, laid out like aln planes
-    unsigned long long* cells;   // DP cells updated (device counter)
+    unsigned long long* cells;   // DP cells: [0] useful (Lq x Lr per fill), [1] launched lane-steps
     unsigned long long* algbytes;  // algorithmic bytes per kernel class (KC_*)
+    unsigned long long* algflops;  // algorithmic FP32 operations per kernel class
 };
 
 // kernel classes for the measurement counters
@@ -96,6 +97,12 @@ enum { KC_SECSTR = 0, KC_INIT, KC_FINDFRAG, KC_REFINE_GAPLESS, KC_DP, KC_REFINE_
 // (covariance pass 24 B, TM-score pass and thresholded covariance pass 28 B,
 // RMSD pass 24 B; 64 B covariance record + 48 B transform per iteration)
 constexpr unsigned long long AB_CC = 24, AB_TM = 28, AB_EXT = 28, AB_RMSD = 24, AB_SLOT = 112;
+// algorithmic FP32 operations per aligned pair and pass, per slot iteration and
+// per DP cell: covariance update 24 (9 products + 15 sums), thresholded
+// covariance 25 (one compare more), TM-score pass 33 (transform 21, distance
+// 9, score 3), Kabsch solve with its transform ~200, DP cell 12 (3
+// subtractions, 3 multiply-adds, an add, a divide, an add, 2 max, a sign)
+constexpr unsigned long long AF_CC = 24, AF_EXT = 25, AF_TM = 33, AF_SLOT = 200, AF_CELL = 12;
 
 __device__ __forceinline__ float sqrd(float x) { return x * x; }
 

@@ -465,7 +465,7 @@ __global__ __launch_bounds__(RTHREADS) void k_findfrag(DevChunk Q, DevChunk R, D
 
     SlotBest mine;
     mine.clear();
-    unsigned long long ab = 0;
+    unsigned long long ab = 0, af = 0;
     bool need = true, live = false;
     int si = 0, qpos = 0, rpos = 0, n = 0, round = 0;
     float tfm[12], thr = CP_LARGEDST, best = 0.f;
@@ -498,6 +498,7 @@ __global__ __launch_bounds__(RTHREADS) void k_findfrag(DevChunk Q, DevChunk R, D
             T cc[15];
             const int cnt = gcc_gapless<P, FG>(q, r, qpos, rpos, n, tfm, round > 0, thr, cc);
             ab += (round > 0 ? AB_EXT : AB_CC) * (unsigned long long)n;
+            af += (round > 0 ? AF_EXT : AF_CC) * (unsigned long long)n;
             bool done = false;
             if (round > 0 && (cnt < 1 || cnt == n)) {
                 done = true;
@@ -505,6 +506,7 @@ __global__ __launch_bounds__(RTHREADS) void k_findfrag(DevChunk Q, DevChunk R, D
                 kabsch<T>(cc, (T)cnt, dinv, tfm);
                 const float sc = gscore_seed<P, FG>(q, r, qpos, rpos, n, tfm, d0, d02, round > 0, round < 2, &thr);
                 ab += AB_TM * (unsigned long long)n + AB_SLOT;
+                af += AF_TM * (unsigned long long)n + AF_SLOT;
                 if (best < sc) best = sc;
                 if (++round == 3) done = true;
             }
@@ -517,7 +519,10 @@ __global__ __launch_bounds__(RTHREADS) void k_findfrag(DevChunk Q, DevChunk R, D
             }
         }
     }
-    if (glane<FG>() == 0 && ab) atomicAdd(S.algbytes + KC_FINDFRAG, ab);
+    if (glane<FG>() == 0 && ab) {
+        atomicAdd(S.algbytes + KC_FINDFRAG, ab);
+        atomicAdd(S.algflops + KC_FINDFRAG, af);
+    }
     SlotBest b;
     group_best<FG>(mine, lds, b);
     if (threadIdx.x == 0) {
@@ -579,7 +584,7 @@ __global__ __launch_bounds__(RTHREADS) void k_refine_gapless(DevChunk Q, DevChun
     SlotBest& mine = lds[grp];
     float* tfmBest = tbest[grp];
     if (glane() == 0) mine.clear();
-    unsigned long long ab = 0;
+    unsigned long long ab = 0, af = 0;
     bool need = anyslot, live = false;
     int si = 0, cit = 0, fqp = 0, frp = 0, fl = 0;
     float tfm[12], thr = CP_LARGEDST, best = 0.f;
@@ -621,6 +626,7 @@ __global__ __launch_bounds__(RTHREADS) void k_refine_gapless(DevChunk Q, DevChun
             if (cit == 0) {
                 ncc = (T)gcc_gapless<P>(q, r, fqp, frp, fl, tfm, false, 0.f, cc);
                 ab += AB_CC * (unsigned long long)fl;
+                af += AF_CC * (unsigned long long)fl;
             } else {
                 if (havenext) {
                     // computed by the previous iteration's fused score pass
@@ -632,6 +638,7 @@ __global__ __launch_bounds__(RTHREADS) void k_refine_gapless(DevChunk Q, DevChun
                     ncc = (T)gcc_gapless<P>(q, r, qpos, rpos, nall, tfm, true, (cit < 2) ? thr : d02s1, cc);
                 }
                 ab += AB_EXT * (unsigned long long)nall;
+                af += AF_EXT * (unsigned long long)nall;
                 if (cc_converged<T>(cc, ncc, last, last[15])) done = true;
             }
             if (!done && ncc < (T)1) done = true;
@@ -657,6 +664,7 @@ __global__ __launch_bounds__(RTHREADS) void k_refine_gapless(DevChunk Q, DevChun
                     sc = gscore_gapless<P>(q, r, qpos, rpos, nall, tfm, d0, d02, d82, cit < 1, &thr);
                 }
                 ab += AB_TM * (unsigned long long)nall + AB_SLOT;
+                af += AF_TM * (unsigned long long)nall + AF_SLOT;
                 if (best < sc) {
                     best = sc;
                     if (glane() == 0) copy12(tfmBest, tfm);
@@ -672,7 +680,10 @@ __global__ __launch_bounds__(RTHREADS) void k_refine_gapless(DevChunk Q, DevChun
             }
         }
     }
-    if (glane() == 0 && ab) atomicAdd(S.algbytes + KC_REFINE_GAPLESS, ab);
+    if (glane() == 0 && ab) {
+        atomicAdd(S.algbytes + KC_REFINE_GAPLESS, ab);
+        atomicAdd(S.algflops + KC_REFINE_GAPLESS, af);
+    }
     SlotBest b;
     group_best_lds(lds, b);
     if (threadIdx.x == 0) {
@@ -1064,6 +1075,10 @@ __device__ __forceinline__ void dp_count(const DevState& S, const DpPair& P) {
         const unsigned long long cells = (unsigned long long)P.Lq * (unsigned long long)P.Lr;
         atomicAdd(S.cells, cells);
         atomicAdd(S.algbytes + KC_DP, cells / 4 + 12ull * (unsigned long long)(P.Lq + P.Lr));
+        atomicAdd(S.algflops + KC_DP, AF_CELL * cells);
+        // the strip sweep's lane-steps: 64-row strips x 16-step blocks over Lr + 63 skewed steps
+        atomicAdd(S.cells + 1, (unsigned long long)((P.Lq + 63) >> 6) *
+                                   (unsigned long long)((P.Lr + 63 + 15) >> 4) * 1024ull);
     }
 }
 
@@ -1598,7 +1613,7 @@ __device__ void refine_pair(const DevChunk& Q, const DevChunk& R, const DevState
     SlotBest& mine = slds[grp];
     float* tfmBest = tbest[grp];
     if (glane() == 0) mine.clear();
-    unsigned long long ab = 0;
+    unsigned long long ab = 0, af = 0;
     // the log search's steps depend on each other: one slot group takes them in order
     bool need = slo < shi && (!logm || grp == 0), live = false;
     int si = 0, cit = 0, fl = 0, sfragndx = 0, sfragpos = 0;
@@ -1680,6 +1695,7 @@ __device__ void refine_pair(const DevChunk& Q, const DevChunk& R, const DevState
                 }
                 ncc = (T)fl;
                 ab += AB_CC * (unsigned long long)fl;
+                af += AF_CC * (unsigned long long)fl;
             } else {
                 if (havenext) {
                     // computed by the previous iteration's fused score pass
@@ -1691,6 +1707,7 @@ __device__ void refine_pair(const DevChunk& Q, const DevChunk& R, const DevState
                     ncc = (T)gcc_dp_ext<P>(a, L, tfm, (cit < 2) ? thr : d02s1, cc);
                 }
                 ab += AB_EXT * (unsigned long long)L;
+                af += AF_EXT * (unsigned long long)L;
                 if (cc_converged<T>(cc, ncc, last, last[15])) done = true;
             }
             if (!done && ncc < (T)1) done = true;
@@ -1716,6 +1733,7 @@ __device__ void refine_pair(const DevChunk& Q, const DevChunk& R, const DevState
                     sc = gscore_dp<P>(a, L, tfm, d0, d02, d82, chk, cit < 1, &thr);
                 }
                 ab += AB_TM * (unsigned long long)L + AB_SLOT;
+                af += AF_TM * (unsigned long long)L + AF_SLOT;
                 if (best < sc) {
                     best = sc;
                     if (glane() == 0) copy12(tfmBest, tfm);
@@ -1735,6 +1753,7 @@ __device__ void refine_pair(const DevChunk& Q, const DevChunk& R, const DevState
                     copy12(tb, tfmBest);
                     gbest = gscore_dp<P>(a, L, tb, g0, g02, d82, false, false, nullptr);
                     ab += AB_TM * (unsigned long long)L;
+                    af += AF_TM * (unsigned long long)L;
                 }
                 if (best > 0.f && mine.beats(best, si) && glane() == 0) {
                     mine.best = best; mine.slot = si; copy12(mine.tfm, tfmBest);
@@ -1755,6 +1774,7 @@ __device__ void refine_pair(const DevChunk& Q, const DevChunk& R, const DevState
         if (qlen != rlen) {
             gbest = gscore_dp<P>(a, L, tb, g0, g02, d82, false, false, nullptr);
             ab += AB_TM * (unsigned long long)L;
+            af += AF_TM * (unsigned long long)L;
         }
         if (grp == 0 && glane() == 0 && bestorg < lbest) {
             float sq = lbest, sr = gbest;
@@ -1771,7 +1791,10 @@ __device__ void refine_pair(const DevChunk& Q, const DevChunk& R, const DevState
     } else {
         slots(Aln{g, g + pl, g + 2 * pl, g + 3 * pl, g + 4 * pl, g + 5 * pl});
     }
-    if (glane() == 0 && ab) atomicAdd(S.algbytes + (MODE == RM_DPREFINE ? KC_REFINE_DP : KC_REFINE_FIN), ab);
+    if (glane() == 0 && ab) {
+        atomicAdd(S.algbytes + (MODE == RM_DPREFINE ? KC_REFINE_DP : KC_REFINE_FIN), ab);
+        atomicAdd(S.algflops + (MODE == RM_DPREFINE ? KC_REFINE_DP : KC_REFINE_FIN), af);
+    }
     if (logm) return;
     SlotBest b;
     group_best_lds(slds, b);

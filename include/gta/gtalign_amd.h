@@ -167,6 +167,7 @@ int gta_set_references(gta_ctx* ctx, const gta_chunk_view* refs);
  * result sections to host memory owned by ctx. */
 int gta_process(gta_ctx* ctx, gta_results* out);
 
+
 /* Same computation without the device-to-host copy; results stay in HBM
  * (for callers that consume them on the device, and for timing the stage
  * with inputs and outputs resident). */
@@ -179,17 +180,24 @@ int gta_process_resident(gta_ctx* ctx, double* seconds, double* dp_cells);
  * number of launches, the summed device time and the algorithmic bytes
  * moved (per aligned pair and pass: 24 B covariance, 28 B TM-score,
  * 28 B thresholded covariance; 112 B per slot iteration; DP: 2 bits per
- * cell + 12 B per residue). */
+ * cell + 12 B per residue) and the algorithmic FP32 operations (per aligned
+ * pair and pass: 24 covariance, 25 thresholded covariance, 33 TM-score;
+ * 200 per slot iteration; DP: 12 per cell). */
 typedef struct gta_kernel_stat {
     const char* name;
     int launches;
     double seconds;
     double alg_bytes;
+    double alg_flops;
 } gta_kernel_stat;
 
 int gta_set_profiling(gta_ctx* ctx, int on);
 /* fills up to n entries; returns the number of kernel classes */
 int gta_kernel_stats(const gta_ctx* ctx, gta_kernel_stat* out, int n);
+/* DP cells of the last gta_process* call: useful = sum of Lq x Lr over the
+ * fills that ran; launched = the lane-steps the strip sweep executed for them
+ * (64-row strips x (Lr + 63) skewed steps rounded up to 16-step blocks). */
+int gta_dp_cell_counts(const gta_ctx* ctx, double* useful, double* launched);
 
 /* ---- DP launch-form autotuning (no counterpart in the reference, whose DP
  * tile shape and launch geometry are fixed, dpw_btck.cu:78-615) ------------

@@ -99,3 +99,44 @@ def fp64_errors(res, q, r):
             rms = float(np.sqrt(((x @ R.T + t - y) ** 2).sum(1).mean()))
             drm.append(abs(rms - a[ALN_RMSD]))
     return np.array(dtm), np.array(drm)
+
+
+def matched_positions_np(alns, alndata, q, r, qi, ri):
+    """matched_positions(alignment_rows(...)) with numpy on the row bytes."""
+    nr = r.nstrs
+    qlen, qd = int(q.len[qi]), int(q.dst[qi])
+    off = (qi * r.nposs + nr * (qd + qi)) * 5
+    rowlen = r.nposs + nr * (qlen + 1)
+    beg = int(r.dst[ri]) + ri * (qlen + 1)
+    n = int(alndata[ALN_ALNLENGTH])
+    a = alns[off + beg: off + beg + n]
+    b = alns[off + beg + 2 * rowlen: off + beg + 2 * rowlen + n]
+    gap = ord('-')
+    qa, ra = a != gap, b != gap
+    bc = np.float32(alndata[ALN_BEGCOORDS]).view(np.uint32)
+    qp = np.cumsum(qa) - 1 + (int(bc & 0xffff) - 1)
+    rp = np.cumsum(ra) - 1 + (int(bc >> 16) - 1)
+    m = qa & ra
+    return np.stack([qp[m], rp[m]], 1).astype(np.int64)
+
+
+def fp64_errors_all(res, q, r):
+    """fp64_errors over every aligned pair of a large result, with the rows
+    parsed by numpy: per aligned pair |reported TM - FP64 TM| (both
+    normalisations, reported transform) and |reported RMSD - FP64 RMSD|."""
+    from gtalign_alpha_amd.capi import ALN_RMSD, ALN_SCOREQ, ALN_SCORER
+    alns = np.frombuffer(res.alns, np.uint8) if not isinstance(res.alns, np.ndarray) else res.alns.view(np.uint8)
+    dtm, drm = [], []
+    for qi, ri in np.argwhere(res.alndata[..., ALN_RMSD] < 9999):
+        a = res.alndata[qi, ri]
+        pairs = matched_positions_np(alns, a, q, r, qi, ri)
+        x, y = q.coords(qi)[pairs[:, 0]].astype(np.float64), r.coords(ri)[pairs[:, 1]].astype(np.float64)
+        d2 = ((tfm_apply(res.tfms[qi, ri], x) - y) ** 2).sum(1)
+        e = 0.0
+        for L, f in ((int(q.len[qi]), ALN_SCOREQ), (int(r.len[ri]), ALN_SCORER)):
+            d0 = d0_tm(L)
+            e = max(e, abs(float((d0 * d0 / (d0 * d0 + d2)).sum() / L) - a[f]))
+        dtm.append(e)
+        R, t = kabsch_svd(x, y)
+        drm.append(abs(float(np.sqrt(((x @ R.T + t - y) ** 2).sum(1).mean())) - a[ALN_RMSD]))
+    return np.array(dtm), np.array(drm)

@@ -17,6 +17,10 @@ from gtalign_alpha_amd.capi import (ChunkView, Options, GTA_ALN_NFIELDS, GTA_TFM
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_LIB = os.path.join(HERE, "libgta_oracle.so")
 REF_LIB = os.path.join(HERE, "_ref", "libgta_ref.so")
+# the reference's own release build (-O3 -ffast-math): CPU timing baseline only
+REF_FAST_LIB = os.path.join(HERE, "_ref", "libgta_ref_fast.so")
+if os.environ.get("GTA_REF_FAST") == "1":
+    REF_LIB = REF_FAST_LIB
 
 _orc = None
 _ref = None

@@ -9,7 +9,7 @@ import pytest
 from gtalign_alpha_amd.chunk import Chunk
 from oracle import oracle
 
-GOLDEN = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.npz")))
+GOLDEN = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "config*_*.npz")))
 
 
 def load(path):

@@ -136,6 +136,36 @@ def test_dp_gap0_is_the_optimum_over_paths():
             assert abs(best - got) < 1e-5, (Lq, Lr)
 
 
+def test_dp_score_is_the_optimum_over_paths():
+    """The score-only fill (GAP0, the only form the search uses:
+    ExecDPScore128xKernel<true>, MpStageFrg3.h:83-87) returns the bottom-right
+    cell: the maximum over every monotone path of the summed match scores
+    d0^2/(d0^2+d^2) with the search d0 of min(Lq, Lr), reached by brute force on
+    3-6 residue matrices; without a gap cost no cell is stored negated, so the
+    value is non-negative, and it grows when a residue pair is moved closer."""
+    rng = np.random.default_rng(17)
+    for Lq, Lr in [(3, 3), (3, 5), (4, 4), (5, 3), (4, 6), (6, 4)]:
+        for _ in range(3):
+            q = rng.normal(scale=4.0, size=(Lq, 3)).astype(np.float32)
+            r = rng.normal(scale=4.0, size=(Lr, 3)).astype(np.float32)
+            R = random_rotation(rng)
+            t = rng.normal(scale=1.0, size=3)
+            tfm = np.concatenate([R.reshape(-1), t]).astype(np.float32)
+            d0 = np.float32(oracle.d0(Lq, Lr)["d0"])
+            d02 = np.float64(d0) * np.float64(d0)
+            qt = q.astype(np.float64) @ R.T.astype(np.float32).astype(np.float64) + tfm[9:].astype(np.float64)
+            d2 = ((qt[:, None, :] - r[None, :, :].astype(np.float64)) ** 2).sum(-1)
+            s = d02 / (d02 + d2)
+            best = max(sum(s[i, j] for m, i, j in p if m == "D") for p in _paths(Lq, Lr))
+            got = oracle.dp_score(q, r, tfm)
+            assert got >= 0.0
+            assert abs(best - got) < 1e-4 * max(1.0, best), (Lq, Lr, best, got)
+    # a lower bound that any wrong cell would break: the diagonal path alone
+    c = synth.chain(31, 6)
+    I = np.eye(3).reshape(-1).tolist() + [0, 0, 0]
+    assert abs(oracle.dp_score(c, c, I) - 6.0) < 1e-5    # identical chains: every pair scores 1
+
+
 def test_dp_identical_chains_align_diagonally():
     c = synth.chain(11, 80)
     I = np.eye(3).reshape(-1).tolist() + [0, 0, 0]

@@ -137,3 +137,102 @@ def test_chunk_from_files(tmp_path):
         assert c.residues(k) == "L" * lens[c.order[k]]
     assert c.resnum[:80].tolist() == list(range(1, 81))
     assert (c.ins_ch_ord < 0).all()
+
+
+# ---- PDBx/mmCIF (FlexDataRead.cpp:553-908) ---------------------------------------
+
+CIF_HEADER = """data_TEST
+#
+loop_
+_atom_site.group_PDB
+_atom_site.id
+_atom_site.type_symbol
+_atom_site.label_atom_id
+_atom_site.label_alt_id
+_atom_site.label_comp_id
+_atom_site.label_asym_id
+_atom_site.label_entity_id
+_atom_site.label_seq_id
+_atom_site.pdbx_PDB_ins_code
+_atom_site.Cartn_x
+_atom_site.Cartn_y
+_atom_site.Cartn_z
+_atom_site.occupancy
+_atom_site.B_iso_or_equiv
+_atom_site.auth_seq_id
+_atom_site.auth_asym_id
+_atom_site.pdbx_PDB_model_num
+"""
+
+
+def cif_row(serial, atom, comp, chain, seq, x, y, z, alt=".", ins="?", rec="ATOM", auth_chain=None,
+            auth_seq=None, model=1):
+    return "%s %d C %s %s %s %s 1 %d %s %.3f %.3f %.3f 1.00 0.00 %d %s %d" % (
+        rec, serial, atom, alt, comp, chain, seq, ins, x, y, z,
+        seq if auth_seq is None else auth_seq, auth_chain or chain, model)
+
+
+def cif_protein_chain(chain, n, start=1, comp="ALA", first_serial=1, auth_chain=None, model=1):
+    rows = []
+    for k in range(n):
+        r = start + k
+        rows.append(cif_row(first_serial + 2 * k, "N", comp, chain, r, k, 0.0, 0.0, auth_chain=auth_chain,
+                            model=model))
+        rows.append(cif_row(first_serial + 2 * k + 1, "CA", comp, chain, r, k + 0.5, 1.25, -2.0,
+                            auth_chain=auth_chain, model=model))
+    return rows
+
+
+def test_cif_known_answer():
+    rows = cif_protein_chain("A", 20, auth_chain="B")
+    rows.append(cif_row(100, "CA", "GLY", "A", 21, 9.0, 8.0, 7.0, alt="A", ins="C", auth_chain="B",
+                        auth_seq=121))
+    rows.append(cif_row(101, "CA", "GLY", "A", 21, 1.0, 1.0, 1.0, alt="B", auth_chain="B"))
+    rows.append(cif_row(102, "CA", "MSE", "A", 22, 3.0, 3.0, 3.0, rec="HETATM", auth_chain="B"))
+    rows.append(cif_row(103, '"CA"', "TRP", "A", 23, -123.456, 45.5, 0.001, auth_chain="B"))
+    rows += cif_protein_chain("C", 30, first_serial=200)          # chain change ends the structure
+    text = CIF_HEADER + "\n".join(rows) + "\n#\n"
+    s = pdb.read_cif(text)
+    assert s is not None and len(s) == 22
+    assert s.chain == "B" and s.model == "1" and not s.nucleic    # auth_asym_id before label_asym_id
+    assert s.residues == "A" * 20 + "GW"
+    assert s.resnum.tolist() == list(range(1, 21)) + [121, 23]   # auth_seq_id
+    assert np.array_equal(s.coords[20], np.float32([9.0, 8.0, 7.0]))
+    assert np.array_equal(s.coords[21], np.float32([-123.456, 45.5, 0.001]))
+    assert s.ins_ch_ord[0] == -((ord(" ") << 16) | (ord("B") << 8) | 1)
+    assert s.ins_ch_ord[20] == -((ord("C") << 16) | (ord("B") << 8) | 1)
+    s2 = pdb.read_cif(text, hetatm=True)
+    assert s2.residues == "A" * 20 + "GMW"
+
+
+def test_cif_model_change_rna_and_short():
+    rows = cif_protein_chain("A", 25) + cif_protein_chain("A", 25, first_serial=100, model=2)
+    s = pdb.read_cif(CIF_HEADER + "\n".join(rows) + "\n")
+    assert len(s) == 25                                            # model change ends it
+    assert pdb.read_cif(CIF_HEADER + "\n".join(cif_protein_chain("A", 19)) + "\n") is None
+    # RNA: 1-letter names (nucleic), C3' atoms, quoted
+    rna = [cif_row(k + 1, '"C3\'"', "AUGC"[k % 4], "R", k + 1, 3.0 * k, 1.0, 0.5) for k in range(24)]
+    s = pdb.read_cif(CIF_HEADER + "\n".join(rna) + "\n")
+    assert s.nucleic and s.residues == "AUGC" * 6 and s.ins_ch_ord[0] > 0
+    # DNA names start with D: nucleic, code X
+    dna = [cif_row(k + 1, '"C3\'"', "DA", "D", k + 1, 3.0 * k, 1.0, 0.5) for k in range(24)]
+    s = pdb.read_cif(CIF_HEADER + "\n".join(dna) + "\n")
+    assert s.nucleic and s.residues == "X" * 24
+
+
+def test_cif_equals_pdb_reader(tmp_path):
+    """The same chains written as PDB and as mmCIF read to the same chunk."""
+    rng = np.random.default_rng(11)
+    names = ["ALA", "GLY", "TRP", "MSE", "LYS"]
+    for n in (20, 57):
+        xyz = np.round(rng.normal(scale=15.0, size=(n, 3)), 3)
+        comps = [names[i % 5] for i in range(n)]
+        plines = [atom(k + 1, " CA ", comps[k], "A", k + 5, *xyz[k]) for k in range(n)] + ["END"]
+        crows = [cif_row(k + 1, "CA", comps[k], "A", k + 5, *xyz[k]) for k in range(n)]
+        (tmp_path / ("s%d.pdb" % n)).write_text("\n".join(plines) + "\n")
+        (tmp_path / ("s%d.cif" % n)).write_text(CIF_HEADER + "\n".join(crows) + "\n#\n")
+    a, ka = pdb.read_chunk([str(tmp_path / "s20.pdb"), str(tmp_path / "s57.pdb")])
+    b, kb = pdb.read_chunk([str(tmp_path / "s20.cif"), str(tmp_path / "s57.cif")])
+    assert ka == kb == [0, 1]
+    for f in ("len", "x", "y", "z", "rsd", "resnum", "ins_ch_ord"):
+        assert np.array_equal(getattr(a, f), getattr(b, f)), f
