@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -29,19 +30,40 @@ namespace {
 
 thread_local std::string g_tls_error;
 
+// Bump allocator over the device scratch of gta_process_noalloc: while one is
+// set for the calling thread, device buffers are carved from it (256-byte
+// aligned) instead of being allocated, and never freed.
+struct Arena {
+    char* base = nullptr;
+    size_t cap = 0, off = 0;
+    bool over = false;
+};
+thread_local Arena* g_arena = nullptr;
+
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
     hipError_t ensure(size_t n) {
         if (n <= bytes && p) return hipSuccess;
         const size_t need = std::max<size_t>(n, 256);
+        if (g_arena) {
+            const size_t at = (g_arena->off + 255) & ~(size_t)255;
+            if (at > g_arena->cap || need > g_arena->cap - at) {
+                g_arena->over = true;
+                return hipErrorOutOfMemory;
+            }
+            p = g_arena->base + at;
+            bytes = need;
+            g_arena->off = at + need;
+            return hipSuccess;
+        }
         if (p) { (void)hipFree(p); p = nullptr; bytes = 0; }
         hipError_t e = hipMalloc(&p, need);
         if (e == hipSuccess) bytes = need;
         return e;
     }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p && !g_arena) (void)hipFree(p);
         p = nullptr;
         bytes = 0;
     }
@@ -79,6 +101,7 @@ int GetMaxNFragSteps(int maxalnlen, int step, int fraglen) {
 
 struct gta_ctx {
     int device = 0;
+    void* scratch = nullptr;    // this device's address of `scratch` (gta_process_noalloc)
     gta_options opts;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -172,6 +195,29 @@ const char* gta_last_error(const gta_ctx* ctx) {
     return ctx ? ctx->err.c_str() : g_tls_error.c_str();
 }
 
+// Device scratch of gta_process_noalloc: one array per device, held until the
+// process ends.  Host code never names it except in hipGetSymbolAddress; it
+// uses each device's address (scratch_address) everywhere else.
+static __device__ __align__(256) unsigned char scratch[GTA_NOALLOC_SCRATCH_BYTES];
+static std::mutex g_scratch_mu;
+static std::map<int, void*> g_scratch_ptr;
+
+// this device's address of `scratch`; the device must be current
+static hipError_t scratch_address(int device, void** out) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch_ptr.find(device);
+    if (it != g_scratch_ptr.end()) {
+        *out = it->second;
+        return hipSuccess;
+    }
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = p;
+    *out = p;
+    return hipSuccess;
+}
+
 static int check_options(const gta_options* o) {
     if (!(o->prescore >= 0.f && o->prescore < 1.f)) return 0;
     if (!(o->scorethld >= 0.f && o->scorethld < 1.f)) return 0;
@@ -211,6 +257,11 @@ int gta_create(int device, const gta_options* opts, int max_qrys, int max_qry_po
     int ncu = 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu > 0)
         c->ncu = ncu;
+    e = scratch_address(device, &c->scratch);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(nullptr, GTA_ERR_HIP, std::string("scratch address: ") + hipGetErrorString(e));
+    }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
         delete c;
@@ -1089,6 +1140,60 @@ int gta_process(gta_ctx* c, gta_results* out) {
     out->seconds = secs;
     out->dp_cells = cells;
     return GTA_OK;
+}
+
+int gta_process_noalloc(gta_ctx* c, const gta_chunk_view* queries, const gta_chunk_view* refs, float* alndata,
+                        float* tfms, char* alns, size_t alns_bytes) {
+    if (!c) return fail(nullptr, GTA_ERR_ARG, "null context");
+    if (!queries || !refs || !alndata || !tfms || !alns) return fail(c, GTA_ERR_ARG, "null argument");
+    if (!c->scratch) return fail(c, GTA_ERR_HIP, "no scratch address for this device");
+    const size_t need = gta_alignment_section_size(queries->nstrs, queries->nposs, refs->nstrs, refs->nposs);
+    if (alns_bytes < need) return fail(c, GTA_ERR_ARG, "alignment buffer smaller than the alignment section");
+    (void)hipSetDevice(c->device);
+    // the context's own buffers step aside; every buffer of the call is carved
+    // from this device's scratch
+    std::vector<DevBuf*> bufs = all_bufs(c);
+    std::vector<DevBuf> saved;
+    saved.reserve(bufs.size());
+    for (DevBuf* b : bufs) {
+        saved.push_back(*b);
+        b->p = nullptr;
+        b->bytes = 0;
+    }
+    const gta_options keep = c->opts;
+    const int keep_prof = c->profiling, keep_tune = c->autotune;
+    c->opts.batched = 0;
+    c->profiling = 0;
+    c->autotune = 0;
+    c->gvalid = false;
+    Arena arena;
+    arena.base = (char*)c->scratch;
+    arena.cap = GTA_NOALLOC_SCRATCH_BYTES;
+    g_arena = &arena;
+    int rc = gta_set_queries(c, queries);
+    if (rc == GTA_OK) rc = gta_set_references(c, refs);
+    double secs = 0, cells = 0;
+    if (rc == GTA_OK) rc = run_stage(c, &secs, &cells);
+    if (rc == GTA_OK) {
+        const size_t np = (size_t)queries->nstrs * refs->nstrs;
+        hipError_t e = hipMemcpy(alndata, c->s_out.p, np * 15 * 4, hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(tfms, c->s_tgrand.p, np * 12 * 4, hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(alns, c->s_alns.p, need, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = fail(c, GTA_ERR_HIP, std::string("result copy: ") + hipGetErrorString(e));
+    }
+    (void)hipStreamSynchronize(c->stream);
+    g_arena = nullptr;
+    if (arena.over)
+        rc = fail(c, GTA_ERR_SIZE, "the chunks need more device scratch than GTA_NOALLOC_SCRATCH_BYTES (" +
+                                       std::to_string((unsigned long long)GTA_NOALLOC_SCRATCH_BYTES) + " bytes)");
+    for (size_t i = 0; i < bufs.size(); i++) *bufs[i] = saved[i];
+    c->opts = keep;
+    c->profiling = keep_prof;
+    c->autotune = keep_tune;
+    // the resident chunks were replaced: set them again before gta_process
+    c->have_q = c->have_r = false;
+    c->gvalid = false;
+    return rc;
 }
 
 }  // extern "C"

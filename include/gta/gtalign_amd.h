@@ -167,6 +167,25 @@ int gta_set_references(gta_ctx* ctx, const gta_chunk_view* refs);
  * result sections to host memory owned by ctx. */
 int gta_process(gta_ctx* ctx, gta_results* out);
 
+/* ---- stage without device allocation --------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory during a call (serving loops, harnesses that forbid hipMalloc):
+ * the whole stage on the given query and reference chunks (host views, as
+ * for gta_set_queries / gta_set_references), with every device buffer of the
+ * call -- the uploaded chunks, the pair state, backtraces, work lists --
+ * carved from one scratch array per device of GTA_NOALLOC_SCRATCH_BYTES
+ * that the library holds until the process ends (its address is taken at
+ * gta_create).  Results go to the caller's host buffers: alndata nq*nr*15
+ * floats, tfms nq*nr*12 floats, alns at least gta_alignment_section_size()
+ * bytes; they equal gta_process's.  The plain form runs (no graph, no
+ * profiling, no autotuning).  Chunks that need more scratch are rejected
+ * with GTA_ERR_SIZE before any result is written.  Calls that overlap on one
+ * device race on its scratch (the array keeps the last call's contents; the
+ * stage clears what it needs).  The call replaces the context's resident
+ * chunks: set them again before the next gta_process. */
+#define GTA_NOALLOC_SCRATCH_BYTES (64u << 20)
+int gta_process_noalloc(gta_ctx* ctx, const gta_chunk_view* queries, const gta_chunk_view* refs,
+                        float* alndata, float* tfms, char* alns, size_t alns_bytes);
 
 /* Same computation without the device-to-host copy; results stay in HBM
  * (for callers that consume them on the device, and for timing the stage
