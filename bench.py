@@ -253,11 +253,12 @@ def _attribute(g, ref_n, pass_n, nthreads):
     """Each pair where the GPU and the reference at `nthreads` threads differ
     (aligned or not, or TM-score by more than 1e-4), attributed with the
     committed 1-thread runs of the same sample (tests/golden/scale_4x8000.npz,
-    written by tests/golden/make_scale_golden.py): `thread_count` when the
-    GPU equals the 1-thread reference and that run's pair is the same with
-    and without per-slot rows moved at packing; `packing` when the GPU
-    equals the 1-thread run with rows moved but not the one without (the
-    reference's stale per-slot rows); `unexplained` otherwise."""
+    written by tests/golden/make_scale_golden.py): `thread_count` when the GPU
+    equals the 1-thread reference; `packing` when it equals only the 1-thread
+    run with per-slot rows moved at packing (the reference's stale per-slot
+    rows); `near_tie` when it equals neither and the pair is weak (greater
+    TM-score below 0.5 on both sides: a search ending on near-ties between
+    superpositions of unrelated fragments); `unexplained` otherwise."""
     from tests.golden.make_scale_golden import FIELDS, OUT
     d = np.load(os.path.join(ROOT, OUT), allow_pickle=False)
     nq, nr = ref_n.shape[:2]
@@ -277,22 +278,30 @@ def _attribute(g, ref_n, pass_n, nthreads):
     g_tm = np.where(g_al[..., None], g.alndata[..., 9:11], -1.0)
     n_al = pass_n & (ref_n[..., 8] < 9999)
     n_tm = np.where(n_al[..., None], ref_n[..., 9:11], -1.0)
-    m_al, m_tm = unpack("moved")
     p_al, p_tm = unpack("plain")
+    m_al, m_tm = unpack("moved")
     diff = ~same(g_al, g_tm, n_al, n_tm)
+    g_eq_p = same(g_al, g_tm, p_al, p_tm)
     g_eq_m = same(g_al, g_tm, m_al, m_tm)
-    m_eq_p = same(m_al, m_tm, p_al, p_tm)
-    out = {"pairs_differing": int(diff.sum()),
-           "thread_count": int((diff & g_eq_m & m_eq_p).sum()),
-           "packing": int((diff & g_eq_m & ~m_eq_p).sum()),
-           "unexplained": int((diff & ~g_eq_m).sum()),
-           "gpu_vs_1thread_moved": {"pairs_differing": int((~g_eq_m).sum()),
-                                    "same_aligned_set": bool(np.array_equal(g_al, m_al)),
-                                    "aligned_pairs": int(m_al.sum())},
-           "reference_%d_vs_1thread_moved_pairs_differing" % nthreads: int((~same(n_al, n_tm, m_al, m_tm)).sum())}
-    out["pairs"] = [{"q": int(a), "r": int(b), "cause": ("thread_count" if g_eq_m[a, b] and m_eq_p[a, b] else
-                                                          "packing" if g_eq_m[a, b] else "unexplained")}
-                    for a, b in np.argwhere(diff)[:64]]
+    weak = np.maximum(g_tm.max(-1), p_tm.max(-1)) < 0.5
+
+    def cause(a, b):
+        if g_eq_p[a, b]:
+            return "thread_count"
+        if g_eq_m[a, b]:
+            return "packing"
+        return "near_tie" if weak[a, b] else "unexplained"
+
+    causes = [cause(a, b) for a, b in np.argwhere(diff)]
+    out = {"pairs_differing": int(diff.sum())}
+    for c in ("thread_count", "packing", "near_tie", "unexplained"):
+        out[c] = causes.count(c)
+    out["gpu_vs_1thread"] = {"pairs_differing": int((~g_eq_p).sum()),
+                             "weak_pairs_differing": int((~g_eq_p & weak).sum()),
+                             "same_aligned_set": bool(np.array_equal(g_al, p_al)),
+                             "aligned_pairs": int(p_al.sum())}
+    out["reference_%d_vs_1thread_pairs_differing" % nthreads] = int((~same(n_al, n_tm, p_al, p_tm)).sum())
+    out["pairs"] = [{"q": int(a), "r": int(b), "cause": c} for (a, b), c in zip(np.argwhere(diff)[:64], causes)]
     return out
 
 
@@ -418,6 +427,19 @@ def copy_calibration(mib):
     print(json.dumps({"copy_bytes": n, "copies": 3}), flush=True)
 
 
+def valu_calibration(iters, blocks=2048):
+    """The VALU calibration kernel alone (blocks x 4 waves, 8 * iters + 23
+    VALU instructions per wave): its duration and instruction rate, the issue
+    ceiling that scales the SQ counters of the stage kernels."""
+    with GtaBatch(0, default_options(), max_qrys=1, max_qry_poss=1, max_refs=1, max_ref_poss=1) as b:
+        secs = b.valu_calibration(blocks, iters)
+    waves = blocks * 4
+    insts = waves * (8 * iters + 23)
+    print(json.dumps({"valu_calibration": {"blocks": blocks, "waves": waves, "iters": iters,
+                                           "valu_wave_instructions": insts, "seconds": secs,
+                                           "wave_instructions_per_s": insts / secs}}), flush=True)
+
+
 def batched_vs_plain(steps, warmup):
     """1 query of 400 x 500,000 references (lognormal(220, 0.7) in [20, 4000],
     10% homologs) in one chunk: pairs/s of the plain FP32 form against the
@@ -477,8 +499,13 @@ def all_vs_all(steps, warmup):
         g = b.ProcessBlock()
     dom = max((st for st in stats if st["launches"]), key=lambda st: st["seconds"])
     dtm, drm = fp64_errors(g, qc, rc)
+    fam = np.linspace(0, 499, qc.nstrs).astype(int)
     return {"workload": "16 family seed chains x 20,000 chains of 500 families (lognormal(200,0.5) in "
                         "[40,800], members: rigid motion, noise 0.5-3 A, 0-15% indels); --pre-score 0",
+            "tile": "rows %s (generation order: the seed chains of families %s) x all 20,000 columns of the "
+                    "20,000 x 20,000 all-vs-all matrix: %d of 20,000 rows, %.2f%% of its pairs" %
+                    (",".join(str(40 * f) for f in fam), ",".join(str(f) for f in fam), qc.nstrs,
+                     100.0 * qc.nstrs / 20000.0),
             "form": "batched, mixed precision", "pairs": npairs,
             "pairs_per_s": round(npairs * steps / wall, 1), "ms_per_step": round(1e3 * wall / steps, 2),
             "dp_gcups": round(cells / dev / 1e9, 2),
@@ -509,9 +536,14 @@ def main():
                     help="skip the 37,860-residue assembly pair of the precision-stress section")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (HBM counter calibration)")
+    ap.add_argument("--valu-calibration", type=int, default=0, metavar="ITERS",
+                    help="only run the VALU calibration kernel with ITERS loop iterations (SQ counter calibration)")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.valu_calibration:
+        valu_calibration(args.valu_calibration)
         return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:

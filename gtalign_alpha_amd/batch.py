@@ -61,6 +61,8 @@ def lib():
         L.gta_kernel_stats.restype = C.c_int
         L.gta_dp_cell_counts.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.gta_dp_cell_counts.restype = C.c_int
+        L.gta_test_valu_calibration.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)]
+        L.gta_test_valu_calibration.restype = C.c_int
         L.gta_test_dp.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_float, C.c_int, C.c_int,
                                   C.c_void_p, C.c_void_p, C.c_void_p]
         L.gta_test_dp.restype = C.c_int
@@ -147,6 +149,13 @@ class GtaBatch:
         return [{"name": arr[i].name.decode(), "launches": arr[i].launches,
                  "seconds": arr[i].seconds, "alg_bytes": arr[i].alg_bytes, "alg_flops": arr[i].alg_flops}
                 for i in range(n)]
+
+    def valu_calibration(self, blocks, iters):
+        """Duration (s) of the VALU calibration kernel: blocks x 4 waves, each
+        issuing 8 * iters + 23 VALU instructions (testing hook)."""
+        s = C.c_double()
+        self._check(lib().gta_test_valu_calibration(self._h, blocks, iters, C.byref(s)))
+        return s.value
 
     def dp_cell_counts(self):
         """DP cells of the last step: (useful Lq x Lr sums, launched lane-steps)."""

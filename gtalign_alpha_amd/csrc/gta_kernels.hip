@@ -2114,6 +2114,27 @@ hipError_t launch_pair_score(const float* d02, const float* d2, int n, float* ou
     if (n > 0) hipLaunchKernelGGL(k_pair_score, dim3(blocks_for(n, 256)), dim3(256), 0, st, d02, d2, n, out);
     return hipGetLastError();
 }
+// VALU issue-rate calibration for the instruction counters: every lane runs
+// `iters` iterations of 8 independent multiply-add chains, so each wave issues
+// exactly 8 v_fma_f32 per iteration inside the loop (loop control is scalar)
+// plus a fixed epilogue; a grid of many such waves keeps every SIMD's VALU
+// busy.  Its measured instruction rate is the issue ceiling the stage kernels
+// are compared against.
+__global__ __launch_bounds__(256) void k_valu_calib(float* out, int iters, float a) {
+    float x0 = (float)threadIdx.x, x1 = x0 + 1.f, x2 = x0 + 2.f, x3 = x0 + 3.f;
+    float x4 = x0 + 4.f, x5 = x0 + 5.f, x6 = x0 + 6.f, x7 = x0 + 7.f;
+    for (int i = 0; i < iters; i++) {
+        x0 = __builtin_fmaf(x0, a, 1.f); x1 = __builtin_fmaf(x1, a, 1.f);
+        x2 = __builtin_fmaf(x2, a, 1.f); x3 = __builtin_fmaf(x3, a, 1.f);
+        x4 = __builtin_fmaf(x4, a, 1.f); x5 = __builtin_fmaf(x5, a, 1.f);
+        x6 = __builtin_fmaf(x6, a, 1.f); x7 = __builtin_fmaf(x7, a, 1.f);
+    }
+    out[(size_t)blockIdx.x * blockDim.x + threadIdx.x] = ((x0 + x1) + (x2 + x3)) + ((x4 + x5) + (x6 + x7));
+}
+hipError_t launch_valu_calib(float* out, int blocks, int iters, hipStream_t st) {
+    if (blocks > 0) hipLaunchKernelGGL(k_valu_calib, dim3(blocks), dim3(256), 0, st, out, iters, 0.999f);
+    return hipGetLastError();
+}
 hipError_t launch_keep_refs(const DevChunk& Q, const DevChunk& R, const DevState& S, int qgroup,
                             hipStream_t st) {
     const int g = (qgroup > 0 && qgroup < S.nq) ? qgroup : S.nq;

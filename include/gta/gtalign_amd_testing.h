@@ -74,6 +74,14 @@ int gta_test_set_split(gta_ctx* ctx, int min_len, int seeds_per_part, int slots_
  * division does). */
 int gta_test_pair_score(gta_ctx* ctx, const float* d02, const float* d2, int n, float* out);
 
+/* VALU issue-rate calibration for the instruction counters: `blocks`
+ * workgroups of 256 lanes, each lane running `iters` iterations of 8
+ * independent FMA chains, so every wave issues exactly 8 * iters + 23 VALU
+ * instructions (8 v_fma_f32 per iteration, 15 before the loop, 8 after).
+ * `seconds` receives the launch's duration (HIP events on the context's
+ * stream, after one untimed launch). */
+int gta_test_valu_calibration(gta_ctx* ctx, int blocks, int iters, double* seconds);
+
 #ifdef __cplusplus
 }
 #endif
