@@ -39,6 +39,9 @@ QGROUP = 2
 
 HBM_PEAK = 8.0e12
 FP32_PEAK = 157.3e12      # MI355X FP32 vector (and dense matrix) peak
+# the VALU issue ceiling that peak implies: one 64-lane FP32 multiply-add
+# (2 x 64 operations) per SIMD every 2 cycles, 1,024 SIMDs at 2.4 GHz
+VALU_ISSUE_PEAK = FP32_PEAK / (2 * 64)
 # rocprofv3 SQ counter summary (VALU issue) of the same workload's kernels
 VALU_PROFILE = "profiles/refine_dp_16x50000_valu_issue.md"
 # rocprofv3 FETCH_SIZE/WRITE_SIZE summary of the 16 x 50,000 workload's kernels
@@ -71,9 +74,10 @@ def profiled_traffic(kernel):
 
 def profiled_valu(kernel):
     """VALU-issue fraction of `kernel` on the full workload from the committed
-    SQ counter summary: SQ_ACTIVE_INST_VALU per dispatch against the SIMD
-    cycles of its duration, each scaled by what the counter reads for a
-    calibration kernel of known VALU instruction count; None when absent."""
+    SQ counter summary: SQ_INSTS_VALU per dispatch (wave-instructions, exactly
+    what the counter reads for the calibration kernel of known instruction
+    count) per second of the dispatch, against VALU_ISSUE_PEAK; None when
+    absent."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), VALU_PROFILE)
     if not os.path.exists(path):
         return None
@@ -427,17 +431,32 @@ def copy_calibration(mib):
     print(json.dumps({"copy_bytes": n, "copies": 3}), flush=True)
 
 
+VALU_KINDS = ["v_fma_f32 (compiler-formed, scalar factor)", "v_fma_f32", "v_mul_f32", "v_add_u32", "v_max_f32",
+              "v_add_f32 with DPP row shift", "v_exp_f32", "v_mov_b32"]
+
+
 def valu_calibration(iters, blocks=2048):
-    """The VALU calibration kernel alone (blocks x 4 waves, 8 * iters + 23
-    VALU instructions per wave): its duration and instruction rate, the issue
-    ceiling that scales the SQ counters of the stage kernels."""
+    """The VALU calibration kernel alone (blocks x 4 waves; every wave issues
+    8 * iters instructions of one kind plus 25 or 26 around its loop): the
+    first record is kind 0 on full waves, the instruction count the SQ
+    counters are checked against; then the issue rate of every kind on full
+    waves and of three kinds with 8 of 64 lanes active, against the FP32
+    issue ceiling VALU_ISSUE_PEAK."""
+    runs = [(k, 64) for k in range(len(VALU_KINDS))] + [(1, 8), (2, 8), (4, 8)]
+    out = []
     with GtaBatch(0, default_options(), max_qrys=1, max_qry_poss=1, max_refs=1, max_ref_poss=1) as b:
-        secs = b.valu_calibration(blocks, iters)
-    waves = blocks * 4
-    insts = waves * (8 * iters + 23)
-    print(json.dumps({"valu_calibration": {"blocks": blocks, "waves": waves, "iters": iters,
-                                           "valu_wave_instructions": insts, "seconds": secs,
-                                           "wave_instructions_per_s": insts / secs}}), flush=True)
+        for kind, lanes in runs:
+            secs = b.valu_calibration(blocks, iters, kind, lanes)
+            waves = blocks * 4
+            insts = waves * (8 * iters + (26 if 1 <= kind <= 4 else 25))
+            out.append({"kind": kind, "instruction": VALU_KINDS[kind], "active_lanes": lanes, "blocks": blocks,
+                        "waves": waves, "iters": iters, "valu_wave_instructions": insts, "seconds": secs,
+                        "wave_instructions_per_s": insts / secs,
+                        "frac_of_issue_peak": round(insts / secs / VALU_ISSUE_PEAK, 4)})
+            log("  %-42s %2d lanes: %8.3f ms  %7.1f G wave-instructions/s  %.3f of %.1f G/s" %
+                (VALU_KINDS[kind], lanes, secs * 1e3, insts / secs * 1e-9, insts / secs / VALU_ISSUE_PEAK,
+                 VALU_ISSUE_PEAK * 1e-9))
+    print(json.dumps({"valu_calibration": out[0], "issue_rates": out[1:]}), flush=True)
 
 
 def batched_vs_plain(steps, warmup):

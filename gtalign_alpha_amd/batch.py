@@ -61,7 +61,8 @@ def lib():
         L.gta_kernel_stats.restype = C.c_int
         L.gta_dp_cell_counts.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.gta_dp_cell_counts.restype = C.c_int
-        L.gta_test_valu_calibration.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)]
+        L.gta_test_valu_calibration.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int,
+                                                C.POINTER(C.c_double)]
         L.gta_test_valu_calibration.restype = C.c_int
         L.gta_test_dp.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_float, C.c_int, C.c_int,
                                   C.c_void_p, C.c_void_p, C.c_void_p]
@@ -150,11 +151,13 @@ class GtaBatch:
                  "seconds": arr[i].seconds, "alg_bytes": arr[i].alg_bytes, "alg_flops": arr[i].alg_flops}
                 for i in range(n)]
 
-    def valu_calibration(self, blocks, iters):
+    def valu_calibration(self, blocks, iters, kind=0, lanes=64):
         """Duration (s) of the VALU calibration kernel: blocks x 4 waves, each
-        issuing 8 * iters + 23 VALU instructions (testing hook)."""
+        issuing 8 * iters VALU instructions of kind `kind` on its first `lanes`
+        lanes plus 25 or 26 around the loop (testing hook,
+        include/gta/gtalign_amd_testing.h)."""
         s = C.c_double()
-        self._check(lib().gta_test_valu_calibration(self._h, blocks, iters, C.byref(s)))
+        self._check(lib().gta_test_valu_calibration(self._h, blocks, iters, kind, lanes, C.byref(s)))
         return s.value
 
     def dp_cell_counts(self):

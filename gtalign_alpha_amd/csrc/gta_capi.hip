@@ -1103,17 +1103,18 @@ int gta_test_pair_score(gta_ctx* c, const float* d02, const float* d2, int n, fl
     return rc;
 }
 
-int gta_test_valu_calibration(gta_ctx* c, int blocks, int iters, double* seconds) {
-    if (!c || blocks < 1 || iters < 1 || !seconds) return fail(c, GTA_ERR_ARG, "invalid argument");
+int gta_test_valu_calibration(gta_ctx* c, int blocks, int iters, int kind, int lanes, double* seconds) {
+    if (!c || blocks < 1 || iters < 1 || kind < 0 || kind > 7 || lanes < 1 || lanes > 64 || !seconds)
+        return fail(c, GTA_ERR_ARG, "invalid argument");
     HIPCHK(c, hipSetDevice(c->device));
     DevBuf b;
     hipError_t e = b.ensure((size_t)blocks * 256 * sizeof(float));
     if (e != hipSuccess) return fail(c, GTA_ERR_HIP, hipGetErrorString(e));
     float ms = 0.f;
     int rc = GTA_OK;
-    if ((e = launch_valu_calib(b.as<float>(), blocks, 1, c->stream)) != hipSuccess ||
+    if ((e = launch_valu_calib(b.as<float>(), blocks, 1, kind, lanes, c->stream)) != hipSuccess ||
         (e = hipEventRecord(c->ev0, c->stream)) != hipSuccess ||
-        (e = launch_valu_calib(b.as<float>(), blocks, iters, c->stream)) != hipSuccess ||
+        (e = launch_valu_calib(b.as<float>(), blocks, iters, kind, lanes, c->stream)) != hipSuccess ||
         (e = hipEventRecord(c->ev1, c->stream)) != hipSuccess ||
         (e = hipEventSynchronize(c->ev1)) != hipSuccess ||
         (e = hipEventElapsedTime(&ms, c->ev0, c->ev1)) != hipSuccess)

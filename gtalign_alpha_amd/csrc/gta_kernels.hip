@@ -79,6 +79,11 @@ __device__ __forceinline__ float GetD0s(float d0) { return fmaxf(fminf(d0, 8.f),
 
 constexpr int RWAVES = 4;                 // waves per workgroup in the refinement kernels
 constexpr int RTHREADS = RWAVES * GTA_WAVE;
+// The FP32 refinements run 4 workgroups (16 waves) per CU: the kernel is held
+// to 128 VGPRs and alignments are staged in LDS (24 B per position) up to
+// RLDS_CAP32 positions, so that 4 x (7.8 KB of fixed LDS + 24 x RLDS_CAP32)
+// fits the CU's 160 KB. Longer alignments are read from global memory.
+constexpr int RLDS_CAP32 = 1344;
 
 __device__ __forceinline__ int pair_q(const DevState& S, int p) { return p / S.nr; }
 __device__ __forceinline__ int pair_r(const DevState& S, int p) { return p % S.nr; }
@@ -1808,7 +1813,7 @@ __device__ void refine_pair(const DevChunk& Q, const DevChunk& R, const DevState
 // gridDim.y workgroups when parts is given, or a persistent grid draining a
 // work queue of pairs.
 template <int P, int MODE>
-__global__ __launch_bounds__(RTHREADS) void k_refine_dp(DevChunk Q, DevChunk R, DevState S, Geom G,
+__global__ __launch_bounds__(RTHREADS) __attribute__((amdgpu_waves_per_eu(P == 0 ? 4 : 1))) void k_refine_dp(DevChunk Q, DevChunk R, DevState S, Geom G,
                                                          RefineParams prm, int ldscap, const int* plist,
                                                          SlotBest* parts, WorkQ wq) {
     extern __shared__ float lds_rf[];
@@ -2114,25 +2119,70 @@ hipError_t launch_pair_score(const float* d02, const float* d2, int n, float* ou
     if (n > 0) hipLaunchKernelGGL(k_pair_score, dim3(blocks_for(n, 256)), dim3(256), 0, st, d02, d2, n, out);
     return hipGetLastError();
 }
-// VALU issue-rate calibration for the instruction counters: every lane runs
-// `iters` iterations of 8 independent multiply-add chains, so each wave issues
-// exactly 8 v_fma_f32 per iteration inside the loop (loop control is scalar)
-// plus a fixed epilogue; a grid of many such waves keeps every SIMD's VALU
-// busy.  Its measured instruction rate is the issue ceiling the stage kernels
-// are compared against.
-__global__ __launch_bounds__(256) void k_valu_calib(float* out, int iters, float a) {
+// VALU issue-rate calibration for the instruction counters: the first `lanes`
+// lanes of every wave run `iters` iterations of 8 independent chains of one
+// instruction kind K, so each wave issues exactly 8 VALU instructions of that
+// kind per iteration (loop control is scalar) plus a fixed prologue and
+// epilogue.  K = 0: v_fma_f32 as the compiler forms it (the factor in a scalar
+// register); 1-7 one instruction each, written in inline assembly with
+// register operands only: 1 v_fma_f32, 2 v_mul_f32, 3 v_add_u32, 4 v_max_f32,
+// 5 v_add_f32 with a DPP row shift, 6 v_exp_f32, 7 v_mov_b32.  A grid of many
+// such waves keeps every SIMD's VALU busy; the measured rates are the issue
+// ceilings per instruction kind that the stage kernels are compared against.
+#define GTA_CAL8(S) S(x0) S(x1) S(x2) S(x3) S(x4) S(x5) S(x6) S(x7)
+#define GTA_CAL_FMA(x) asm volatile("v_fma_f32 %0, %0, %1, 1.0" : "+v"(x) : "v"(a));
+#define GTA_CAL_MUL(x) asm volatile("v_mul_f32 %0, %0, %1" : "+v"(x) : "v"(a));
+#define GTA_CAL_ADDU(x) asm volatile("v_add_u32 %0, %0, %1" : "+v"(x) : "v"(a));
+#define GTA_CAL_MAX(x) asm volatile("v_max_f32 %0, %0, %1" : "+v"(x) : "v"(a));
+#define GTA_CAL_DPP(x) asm volatile("v_add_f32_dpp %0, %0, %0 row_shr:1 bound_ctrl:0" : "+v"(x));
+#define GTA_CAL_EXP(x) asm volatile("v_exp_f32 %0, %0" : "+v"(x));
+#define GTA_CAL_MOV(x) asm volatile("v_mov_b32 %0, %1" : "=v"(x) : "v"(x));
+template <int K>
+__global__ __launch_bounds__(256) void k_valu_calib(float* out, int iters, float a, int lanes) {
     float x0 = (float)threadIdx.x, x1 = x0 + 1.f, x2 = x0 + 2.f, x3 = x0 + 3.f;
     float x4 = x0 + 4.f, x5 = x0 + 5.f, x6 = x0 + 6.f, x7 = x0 + 7.f;
-    for (int i = 0; i < iters; i++) {
-        x0 = __builtin_fmaf(x0, a, 1.f); x1 = __builtin_fmaf(x1, a, 1.f);
-        x2 = __builtin_fmaf(x2, a, 1.f); x3 = __builtin_fmaf(x3, a, 1.f);
-        x4 = __builtin_fmaf(x4, a, 1.f); x5 = __builtin_fmaf(x5, a, 1.f);
-        x6 = __builtin_fmaf(x6, a, 1.f); x7 = __builtin_fmaf(x7, a, 1.f);
+    if ((int)(threadIdx.x & (GTA_WAVE - 1)) < lanes) {
+        for (int i = 0; i < iters; i++) {
+            if (K == 0) {
+                x0 = __builtin_fmaf(x0, a, 1.f); x1 = __builtin_fmaf(x1, a, 1.f);
+                x2 = __builtin_fmaf(x2, a, 1.f); x3 = __builtin_fmaf(x3, a, 1.f);
+                x4 = __builtin_fmaf(x4, a, 1.f); x5 = __builtin_fmaf(x5, a, 1.f);
+                x6 = __builtin_fmaf(x6, a, 1.f); x7 = __builtin_fmaf(x7, a, 1.f);
+            }
+            if (K == 1) { GTA_CAL8(GTA_CAL_FMA) }
+            if (K == 2) { GTA_CAL8(GTA_CAL_MUL) }
+            if (K == 3) { GTA_CAL8(GTA_CAL_ADDU) }
+            if (K == 4) { GTA_CAL8(GTA_CAL_MAX) }
+            if (K == 5) { GTA_CAL8(GTA_CAL_DPP) }
+            if (K == 6) { GTA_CAL8(GTA_CAL_EXP) }
+            if (K == 7) { GTA_CAL8(GTA_CAL_MOV) }
+        }
     }
     out[(size_t)blockIdx.x * blockDim.x + threadIdx.x] = ((x0 + x1) + (x2 + x3)) + ((x4 + x5) + (x6 + x7));
 }
-hipError_t launch_valu_calib(float* out, int blocks, int iters, hipStream_t st) {
-    if (blocks > 0) hipLaunchKernelGGL(k_valu_calib, dim3(blocks), dim3(256), 0, st, out, iters, 0.999f);
+#undef GTA_CAL8
+#undef GTA_CAL_FMA
+#undef GTA_CAL_MUL
+#undef GTA_CAL_ADDU
+#undef GTA_CAL_MAX
+#undef GTA_CAL_DPP
+#undef GTA_CAL_EXP
+#undef GTA_CAL_MOV
+hipError_t launch_valu_calib(float* out, int blocks, int iters, int kind, int lanes, hipStream_t st) {
+    if (blocks <= 0) return hipSuccess;
+#define GTA_KC(K) hipLaunchKernelGGL(k_valu_calib<K>, dim3(blocks), dim3(256), 0, st, out, iters, 0.999f, lanes)
+    switch (kind) {
+        case 0: GTA_KC(0); break;
+        case 1: GTA_KC(1); break;
+        case 2: GTA_KC(2); break;
+        case 3: GTA_KC(3); break;
+        case 4: GTA_KC(4); break;
+        case 5: GTA_KC(5); break;
+        case 6: GTA_KC(6); break;
+        case 7: GTA_KC(7); break;
+        default: return hipErrorInvalidValue;
+    }
+#undef GTA_KC
     return hipGetLastError();
 }
 hipError_t launch_keep_refs(const DevChunk& Q, const DevChunk& R, const DevState& S, int qgroup,
@@ -2315,7 +2365,8 @@ hipError_t launch_refine_dp(int prec, int mode, const DevChunk& Q, const DevChun
                             const Geom& G, const RefineLaunch& L, const RefSplit& Sp, const WorkQ& wq,
                             int pgrid, hipStream_t st) {
     RefineParams prm{L.dinv, L.epi, L.dpi, L.maxndpiters, L.prescore};
-    const size_t lds = (size_t)L.ldscap * 6 * sizeof(float);
+    const int ldscap = (prec == 0 || mode == RM_DPREFINE) ? std::min(L.ldscap, RLDS_CAP32) : L.ldscap;
+    const size_t lds = (size_t)ldscap * 6 * sizeof(float);
     const int nsmall = wq.items ? pgrid : (Sp.small ? Sp.nsmall : S.nq * S.nr);
     const bool phase2 = (mode == RM_FIN2 || mode == RM_PROD2);
     const bool full2 = (mode == RM_FIN2F || mode == RM_PROD2F);
@@ -2328,10 +2379,10 @@ hipError_t launch_refine_dp(int prec, int mode, const DevChunk& Q, const DevChun
     do {                                                                                                      \
         if (nsmall > 0)                                                                                       \
             hipLaunchKernelGGL((k_refine_dp<PP, MM>), dim3(nsmall), dim3(RTHREADS), lds, st, Q, R, S, G, prm, \
-                               L.ldscap, Sp.small, (SlotBest*)nullptr, wq);                                   \
+                               ldscap, Sp.small, (SlotBest*)nullptr, wq);                                   \
         if (Sp.nbig > 0) {                                                                                    \
             hipLaunchKernelGGL((k_refine_dp<PP, MM>), dim3(Sp.nbig, np), dim3(RTHREADS), lds, st, Q, R, S, G, \
-                               prm, L.ldscap, Sp.big, parts, WorkQ{});                                        \
+                               prm, ldscap, Sp.big, parts, WorkQ{});                                        \
             hipLaunchKernelGGL((k_merge_parts<2, MM>), dim3((Sp.nbig + 63) / 64), dim3(64), 0, st, Q, R, S,  \
                                prm, Sp.big, Sp.nbig, np, parts);                                              \
         }                                                                                                     \
@@ -2341,10 +2392,10 @@ hipError_t launch_refine_dp(int prec, int mode, const DevChunk& Q, const DevChun
     do {                                                                                                       \
         if (nsmall > 0)                                                                                        \
             hipLaunchKernelGGL((k_refine_dp<PP, RM_PROD2L>), dim3(nsmall), dim3(RTHREADS), lds, st, Q, R, S, G, \
-                               prm, L.ldscap, Sp.small, (SlotBest*)nullptr, wq);                               \
+                               prm, ldscap, Sp.small, (SlotBest*)nullptr, wq);                               \
         if (Sp.nbig > 0)                                                                                       \
             hipLaunchKernelGGL((k_refine_dp<PP, RM_PROD2L>), dim3(Sp.nbig), dim3(RTHREADS), lds, st, Q, R, S, G,\
-                               prm, L.ldscap, Sp.big, (SlotBest*)nullptr, WorkQ{});                            \
+                               prm, ldscap, Sp.big, (SlotBest*)nullptr, WorkQ{});                            \
     } while (0)
 #define GTA_RLM(PP)                                      \
     switch (mode) {                                      \

@@ -70,7 +70,9 @@ hipError_t launch_secstr(const DevChunk& C, hipStream_t st);
 hipError_t launch_init(const DevChunk& Q, const DevChunk& R, const DevState& S, hipStream_t st);
 // the pair score d02 / (d02 + d2) over device arrays of n operands
 hipError_t launch_pair_score(const float* d02, const float* d2, int n, float* out, hipStream_t st);
-hipError_t launch_valu_calib(float* out, int blocks, int iters, hipStream_t st);
+// the VALU calibration kernel of instruction kind `kind` (0-7) on the first
+// `lanes` lanes of every wave
+hipError_t launch_valu_calib(float* out, int blocks, int iters, int kind, int lanes, hipStream_t st);
 // the reference filter after the stage-1 prescreen, per block of qgroup queries (0: all)
 hipError_t launch_keep_refs(const DevChunk& Q, const DevChunk& R, const DevState& S, int qgroup,
                             hipStream_t st);

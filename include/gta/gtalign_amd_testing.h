@@ -75,12 +75,16 @@ int gta_test_set_split(gta_ctx* ctx, int min_len, int seeds_per_part, int slots_
 int gta_test_pair_score(gta_ctx* ctx, const float* d02, const float* d2, int n, float* out);
 
 /* VALU issue-rate calibration for the instruction counters: `blocks`
- * workgroups of 256 lanes, each lane running `iters` iterations of 8
- * independent FMA chains, so every wave issues exactly 8 * iters + 23 VALU
- * instructions (8 v_fma_f32 per iteration, 15 before the loop, 8 after).
- * `seconds` receives the launch's duration (HIP events on the context's
- * stream, after one untimed launch). */
-int gta_test_valu_calibration(gta_ctx* ctx, int blocks, int iters, double* seconds);
+ * workgroups of 256 lanes; the first `lanes` (1-64) lanes of every wave run
+ * `iters` iterations of 8 independent chains of one instruction kind:
+ * 0 v_fma_f32 (compiler-formed, factor in a scalar register), 1 v_fma_f32,
+ * 2 v_mul_f32, 3 v_add_u32, 4 v_max_f32, 5 v_add_f32 with a DPP row shift,
+ * 6 v_exp_f32, 7 v_mov_b32 (1-7 with vector register operands).  Every wave
+ * issues exactly 8 * iters VALU instructions of the kind in the loop plus 25
+ * (kinds 0, 5, 6, 7) or 26 (kinds 1-4) before and after it.  `seconds`
+ * receives the launch's duration (HIP events on the context's stream, after
+ * one untimed launch). */
+int gta_test_valu_calibration(gta_ctx* ctx, int blocks, int iters, int kind, int lanes, double* seconds);
 
 #ifdef __cplusplus
 }
