@@ -773,7 +773,7 @@ static int run_stage(gta_ctx* c, double* seconds, double* cells) {
     int* listc = c->dpb.any_big() ? c->s_listc.as<int>() : nullptr;
     int* ltot = c->s_ltot.as<int>();
     int* heads = batched ? c->s_heads.as<int>() : nullptr;
-    const int pgrid_dp = c->ncu * 32, pgrid_rf = c->ncu * 4;
+    const int pgrid_dp = c->ncu * 32, pgrid_rf = c->ncu * REFINE_WGS_PER_CU;
     DpBuckets dpb = c->dpb;         // plain form: one-wave list narrowed to live pairs
     RefSplit rsp = c->rsp;
     auto queue = [&](const int* items, const int* count) -> WorkQ {

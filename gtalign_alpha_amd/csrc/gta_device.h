@@ -34,6 +34,13 @@ enum { BT_STOP = 0, BT_LEFT = 1, BT_UP = 2, BT_DIAG = 3 };  // gproc.h:180-185
 // covariance record: 9 cross terms, query sum, reference sum, count
 enum { CCM00 = 0, CVq0 = 9, CVr0 = 12, NALN = 15, NCC = 16 };
 
+// waves per workgroup of the superposition searches and refinements (one pair,
+// or one part of a pair, per workgroup): two waves keep the share of a
+// workgroup's time spent waiting for its last slots small
+constexpr int REFINE_WAVES = 2;
+// workgroups of the refinements resident per CU (4 waves per SIMD)
+constexpr int REFINE_WGS_PER_CU = 16 / REFINE_WAVES;
+
 // ---- chunk and pair-state views passed to every kernel -----------------------
 struct DevChunk {
     int nstrs, nposs;

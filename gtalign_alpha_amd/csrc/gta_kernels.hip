@@ -77,13 +77,14 @@ __device__ __forceinline__ float GetD0fin(int q, int r, int na) {
 }
 __device__ __forceinline__ float GetD0s(float d0) { return fmaxf(fminf(d0, 8.f), 4.5f); }
 
-constexpr int RWAVES = 4;                 // waves per workgroup in the refinement kernels
+constexpr int RWAVES = REFINE_WAVES;      // waves per workgroup in the refinement kernels
 constexpr int RTHREADS = RWAVES * GTA_WAVE;
-// The FP32 refinements run 4 workgroups (16 waves) per CU: the kernel is held
-// to 128 VGPRs and alignments are staged in LDS (24 B per position) up to
-// RLDS_CAP32 positions, so that 4 x (7.8 KB of fixed LDS + 24 x RLDS_CAP32)
-// fits the CU's 160 KB. Longer alignments are read from global memory.
-constexpr int RLDS_CAP32 = 1344;
+// The FP32 refinements run REFINE_WGS_PER_CU workgroups (16 waves) per CU:
+// the kernel is held to 128 VGPRs and alignments are staged in LDS (24 B per
+// position) up to RLDS_CAP32 positions, so that the workgroups' fixed LDS
+// (3.9 KB each) and staged alignments fit the CU's 160 KB. Longer alignments
+// are read from global memory.
+constexpr int RLDS_CAP32 = (160 * 1024 / REFINE_WGS_PER_CU - 4096) / 24;
 
 __device__ __forceinline__ int pair_q(const DevState& S, int p) { return p / S.nr; }
 __device__ __forceinline__ int pair_r(const DevState& S, int p) { return p % S.nr; }
