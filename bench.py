@@ -440,9 +440,10 @@ def valu_calibration(iters, blocks=2048):
     8 * iters instructions of one kind plus 25 or 26 around its loop): the
     first record is kind 0 on full waves, the instruction count the SQ
     counters are checked against; then the issue rate of every kind on full
-    waves and of three kinds with 8 of 64 lanes active, against the FP32
-    issue ceiling VALU_ISSUE_PEAK."""
-    runs = [(k, 64) for k in range(len(VALU_KINDS))] + [(1, 8), (2, 8), (4, 8)]
+    waves and of three kinds with 1 to 48 of 64 lanes active, against the
+    FP32 issue ceiling VALU_ISSUE_PEAK."""
+    runs = [(k, 64) for k in range(len(VALU_KINDS))]
+    runs += [(k, n) for k in (1, 2, 4) for n in (1, 4, 8, 9, 12, 16, 24, 32, 48)]
     out = []
     with GtaBatch(0, default_options(), max_qrys=1, max_qry_poss=1, max_refs=1, max_ref_poss=1) as b:
         for kind, lanes in runs:

@@ -135,7 +135,9 @@ struct gta_ctx {
     DevBuf s_rsmall, s_rbig, s_parts;
     RefSplit rsp{};
     int split_min_len = 2048;
-    int split_seeds = 256, split_slots = 32;
+    // per part: 4 seed diagonals per 4-lane group of the seed search and one
+    // fragment slot per 8-lane group of the refinements
+    int split_seeds = 64 * REFINE_WAVES, split_slots = 8 * REFINE_WAVES;
     // batched form: live-pair work lists, queue counters, the captured graph
     DevBuf s_bigmask, s_lista, s_listb, s_listc, s_lcnt, s_ltot, s_heads;
     DevBuf s_sctfm, s_scflag, s_scout;      // score-only DP test hook
