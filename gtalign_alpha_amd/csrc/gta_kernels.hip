@@ -840,11 +840,14 @@ constexpr int DPRING = 128;
 // strip's last column (row 64s+l).  SC_SSRR ring entries hold {residue code
 // (int bits), SS letter, 0, row-above value} and read the weighted residue
 // scores from gtab.  Returns false when a row wait gave up (Sync::wait).
-template <int SC, bool GAP0, bool BTCK = true, class Sync>
-__device__ __forceinline__ bool dp_strip(const DevChunk& Q, const DevChunk& R, const DevState& S,
-                                         const DpParams& prm, const DpPair& P, int s, const float* rowin,
-                                         float* rowout, unsigned* bt, float4* ring, float* ssring, Sync& sync,
-                                         float* vend = nullptr, const float* gtab = nullptr) {
+// PUT: the strip stores its last row (rowout given), fixed at compile time.
+// The steps where every lane's cell lies inside the matrix (a strip of 64 rows
+// of the query, steps 63 to Lr-1) run a form without the bounds selects.
+template <int SC, bool GAP0, bool BTCK, bool PUT, class Sync>
+__device__ __forceinline__ bool dp_strip_impl(const DevChunk& Q, const DevChunk& R, const DevState& S,
+                                              const DpParams& prm, const DpPair& P, int s, const float* rowin,
+                                              float* rowout, unsigned* bt, float4* ring, float* ssring,
+                                              Sync& sync, float* vend, const float* gtab) {
     constexpr bool USESS = SC == SC_TMSS;
     constexpr bool SSRR = SC == SC_SSRR;
     // value of the cells outside the matrix
@@ -876,7 +879,7 @@ __device__ __forceinline__ bool dp_strip(const DevChunk& Q, const DevChunk& R, c
     }
     const bool qok = rowok && qx < DEFCOORD_QRY_cmp;
     const bool haveup = rowin != nullptr;
-    const bool putrow = rowout != nullptr;
+    constexpr bool putrow = PUT;
     if (haveup && !sync.wait(s - 1, min(128, Lr))) return false;
     // block 0 at entries 0..63, columns -64..-1 (entries 64..127) outside
     {
@@ -913,8 +916,11 @@ __device__ __forceinline__ bool dp_strip(const DevChunk& Q, const DevChunk& R, c
     float val = OUT, upprev = OUT, oblk = 0.f;
     unsigned word = 0;
     int j = -lane;
-    // one step (anti-diagonal); false when a row wait gave up
-    auto step = [&](const int t) -> bool {
+    // one step (anti-diagonal); false when a row wait gave up.  In an inner
+    // step every lane's cell is in the matrix, so the bounds test (and the
+    // column counter it reads) drops out.
+    auto step = [&](const int t, auto inner) -> bool {
+        constexpr bool INNER = decltype(inner)::value;
         const float4 rc = cur;
         const float rss = curs;
         if ((t & 63) == 63) {
@@ -942,7 +948,7 @@ __device__ __forceinline__ bool dp_strip(const DevChunk& Q, const DevChunk& R, c
         const float up = wave_shr1(val, rc.w);
         const float diag = upprev;
         upprev = up;
-        const bool valid = rowok && (unsigned)j < (unsigned)Lr;
+        const bool valid = INNER || (rowok && (unsigned)j < (unsigned)Lr);
         float v, nv;
         unsigned bk = BT_DIAG;
         if (SSRR) {
@@ -976,7 +982,7 @@ __device__ __forceinline__ bool dp_strip(const DevChunk& Q, const DevChunk& R, c
         // cells left of the strip's columns keep 0, cells past them the last value
         val = valid ? nv : val;
         const unsigned code = valid ? bk : (unsigned)BT_STOP;
-        j++;
+        if (!INNER) j++;
         if (putrow) {
             // lane l now holds column t-126+l of row 64s+63
             oblk = wave_shl1(oblk, val);
@@ -997,14 +1003,41 @@ __device__ __forceinline__ bool dp_strip(const DevChunk& Q, const DevChunk& R, c
     };
     // two steps per iteration, so that the prefetched ring entry and the
     // carried values alternate registers instead of being copied every step
+    const std::integral_constant<bool, false> edge;
+    const std::integral_constant<bool, true> inner;
+    // inner steps: a full strip, steps 64 .. (Lr & ~1) - 1 (even bounds keep the pairs)
+    const bool full = (s + 1) << 6 <= Lq;
+    const int tin = full ? 64 : nsteps, tout = full ? max(64, Lr & ~1) : nsteps;
     int t = 0;
-    for (; t + 1 < nsteps; t += 2) {
-        if (!step(t)) return false;
-        if (!step(t + 1)) return false;
+    for (; t + 1 < min(tin, nsteps); t += 2) {
+        if (!step(t, edge)) return false;
+        if (!step(t + 1, edge)) return false;
     }
-    if (t < nsteps && !step(t)) return false;
+    if (t < tout) {
+        for (; t + 1 < tout; t += 2) {
+            if (!step(t, inner)) return false;
+            if (!step(t + 1, inner)) return false;
+        }
+        j = t - lane;   // the column counter as the edge steps would have left it
+    }
+    for (; t + 1 < nsteps; t += 2) {
+        if (!step(t, edge)) return false;
+        if (!step(t + 1, edge)) return false;
+    }
+    if (t < nsteps && !step(t, edge)) return false;
     if (vend) *vend = val;
     return true;
+}
+template <int SC, bool GAP0, bool BTCK = true, class Sync>
+__device__ __forceinline__ bool dp_strip(const DevChunk& Q, const DevChunk& R, const DevState& S,
+                                         const DpParams& prm, const DpPair& P, int s, const float* rowin,
+                                         float* rowout, unsigned* bt, float4* ring, float* ssring, Sync& sync,
+                                         float* vend = nullptr, const float* gtab = nullptr) {
+    if (rowout)
+        return dp_strip_impl<SC, GAP0, BTCK, true>(Q, R, S, prm, P, s, rowin, rowout, bt, ring, ssring, sync,
+                                                   vend, gtab);
+    return dp_strip_impl<SC, GAP0, BTCK, false>(Q, R, S, prm, P, s, rowin, rowout, bt, ring, ssring, sync, vend,
+                                                gtab);
 }
 
 // Backtrace walk from the bottom-right cell (BtckToMatched, btck2match.cuh:46-52;
