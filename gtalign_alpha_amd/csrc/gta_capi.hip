@@ -137,7 +137,7 @@ struct gta_ctx {
     int split_min_len = 2048;
     // per part: 4 seed diagonals per 4-lane group of the seed search and one
     // fragment slot per 8-lane group of the refinements
-    int split_seeds = 64 * REFINE_WAVES, split_slots = 8 * REFINE_WAVES;
+    int split_seeds = 64 * SEARCH_WAVES, split_slots = 8 * REFINE_WAVES;
     // batched form: live-pair work lists, queue counters, the captured graph
     DevBuf s_bigmask, s_lista, s_listb, s_listc, s_lcnt, s_ltot, s_heads;
     DevBuf s_sctfm, s_scflag, s_scout;      // score-only DP test hook

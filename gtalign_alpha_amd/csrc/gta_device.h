@@ -34,12 +34,16 @@ enum { BT_STOP = 0, BT_LEFT = 1, BT_UP = 2, BT_DIAG = 3 };  // gproc.h:180-185
 // covariance record: 9 cross terms, query sum, reference sum, count
 enum { CCM00 = 0, CVq0 = 9, CVr0 = 12, NALN = 15, NCC = 16 };
 
-// waves per workgroup of the superposition searches and refinements (one pair,
-// or one part of a pair, per workgroup): two waves keep the share of a
-// workgroup's time spent waiting for its last slots small
+// waves per workgroup of the refinements (one pair, or one part of a pair,
+// per workgroup): two waves keep the share of a workgroup's time spent
+// waiting for its last slots small and leave LDS to stage alignments of up to
+// 682 positions
 constexpr int REFINE_WAVES = 2;
 // workgroups of the refinements resident per CU (4 waves per SIMD)
 constexpr int REFINE_WGS_PER_CU = 16 / REFINE_WAVES;
+// waves per workgroup of the seed search (every pair's slots are many and
+// even, so a larger workgroup costs no idle lanes)
+constexpr int SEARCH_WAVES = 4;
 
 // ---- chunk and pair-state views passed to every kernel -----------------------
 struct DevChunk {

@@ -79,12 +79,13 @@ __device__ __forceinline__ float GetD0s(float d0) { return fmaxf(fminf(d0, 8.f),
 
 constexpr int RWAVES = REFINE_WAVES;      // waves per workgroup in the refinement kernels
 constexpr int RTHREADS = RWAVES * GTA_WAVE;
+constexpr int FTHREADS = SEARCH_WAVES * GTA_WAVE;   // threads per workgroup of the seed search
 // The FP32 refinements run REFINE_WGS_PER_CU workgroups (16 waves) per CU:
 // the kernel is held to 128 VGPRs and alignments are staged in LDS (24 B per
 // position) up to RLDS_CAP32 positions, so that the workgroups' fixed LDS
-// (3.9 KB each) and staged alignments fit the CU's 160 KB. Longer alignments
-// are read from global memory.
-constexpr int RLDS_CAP32 = (160 * 1024 / REFINE_WGS_PER_CU - 4096) / 24;
+// (just under 2 KB per wave) and staged alignments fit the CU's 160 KB.
+// Longer alignments are read from global memory.
+constexpr int RLDS_CAP32 = (160 * 1024 / REFINE_WGS_PER_CU - 2048 * REFINE_WAVES) / 24;
 
 __device__ __forceinline__ int pair_q(const DevState& S, int p) { return p / S.nr; }
 __device__ __forceinline__ int pair_r(const DevState& S, int p) { return p % S.nr; }
@@ -287,13 +288,13 @@ static_assert(GTA_FINDFRAG_LANES == 1 || GTA_FINDFRAG_LANES == 2 || GTA_FINDFRAG
 constexpr int FF_GL = GTA_FINDFRAG_LANES;
 
 // the workgroup's best over its groups of G lanes (every thread returns it)
-template <int G = GL>
+template <int G = GL, int NT = RTHREADS>
 __device__ void group_best(const SlotBest& mine, SlotBest* lds, SlotBest& out) {
     const int g = threadIdx.x / G;
     if (glane<G>() == 0) lds[g] = mine;
     __syncthreads();
     out = lds[0];
-    for (int i = 1; i < RTHREADS / G; i++)
+    for (int i = 1; i < NT / G; i++)
         if (out.beats(lds[i].best, lds[i].slot)) out = lds[i];
 }
 // the same when each group already keeps its record in lds[group]
@@ -441,10 +442,10 @@ __device__ __forceinline__ void part_range(int n0, int n1, int part, int nparts,
 }
 
 template <int P>
-__global__ __launch_bounds__(RTHREADS) void k_findfrag(DevChunk Q, DevChunk R, DevState S, Geom G,
+__global__ __launch_bounds__(FTHREADS) void k_findfrag(DevChunk Q, DevChunk R, DevState S, Geom G,
                                                         int dinv_on, const int* plist, SlotBest* parts) {
     constexpr int FG = FF_GL;
-    __shared__ SlotBest lds[RTHREADS / FG];
+    __shared__ SlotBest lds[FTHREADS / FG];
     __shared__ int ctr;
     const int p = plist ? plist[blockIdx.x] : (int)blockIdx.x;
     const int qi = pair_q(S, p), ri = pair_r(S, p);
@@ -530,7 +531,7 @@ __global__ __launch_bounds__(RTHREADS) void k_findfrag(DevChunk Q, DevChunk R, D
         atomicAdd(S.algflops + KC_FINDFRAG, af);
     }
     SlotBest b;
-    group_best<FG>(mine, lds, b);
+    group_best<FG, FTHREADS>(mine, lds, b);
     if (threadIdx.x == 0) {
         if (parts) parts[(size_t)blockIdx.x * gridDim.y + blockIdx.y] = b;
         else findfrag_epilogue(S, p, b);
@@ -2263,12 +2264,12 @@ hipError_t launch_findfrag(const DevChunk& Q, const DevChunk& R, const DevState&
                            int dinv, const RefSplit& Sp, hipStream_t st) {
     const int nsmall = Sp.small ? Sp.nsmall : S.nq * S.nr;
     if (nsmall > 0) {
-        hipLaunchKernelGGL(k_findfrag<0>, dim3(nsmall), dim3(RTHREADS), 0, st, Q, R, S, G, dinv, Sp.small, nullptr);
+        hipLaunchKernelGGL(k_findfrag<0>, dim3(nsmall), dim3(FTHREADS), 0, st, Q, R, S, G, dinv, Sp.small, nullptr);
     }
     if (Sp.nbig > 0) {
         const int np = nparts_for(G.nseeds, Sp.seeds_per_part);
         SlotBest* parts = (SlotBest*)Sp.partbuf;
-        hipLaunchKernelGGL(k_findfrag<0>, dim3(Sp.nbig, np), dim3(RTHREADS), 0, st, Q, R, S, G, dinv, Sp.big, parts);
+        hipLaunchKernelGGL(k_findfrag<0>, dim3(Sp.nbig, np), dim3(FTHREADS), 0, st, Q, R, S, G, dinv, Sp.big, parts);
         RefineParams prm{};
         hipLaunchKernelGGL((k_merge_parts<0, 0>), dim3((Sp.nbig + 63) / 64), dim3(64), 0, st, Q, R, S, prm, Sp.big,
                            Sp.nbig, np, parts);
