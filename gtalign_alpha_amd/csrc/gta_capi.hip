@@ -713,6 +713,22 @@ static int prepare_state(gta_ctx* c, DevState& S, size_t& alnsize) {
     return GTA_OK;
 }
 
+// Waves per workgroup of the DPRefine launches.  One-wave workgroups leave
+// fewer of a CU's lanes idle while a pair's last slots run, but stage only
+// alignments of up to rlds_cap32(1) positions in LDS; longer ones are read from
+// L2 in every pass.  An alignment is at most min(Lq, Lr) long, so one wave is
+// used when at most 15% of the chunks' pairs can have longer alignments (16 x
+// 50,000 workload: 7%, DPRefine 0.85 -> 0.79 s with one wave; 1 x 500,000: 27%,
+// 0.40 -> 0.50 s).
+static int dprefine_waves(const gta_ctx* c) {
+    const int cap = rlds_cap32(1);
+    long long ql = 0, rl = 0;
+    for (int v : c->q.len) ql += v > cap;
+    for (int v : c->r.len) rl += v > cap;
+    const double np = (double)c->q.len.size() * (double)c->r.len.size();
+    return (np > 0 && (double)ql * (double)rl <= 0.15 * np) ? 1 : 2;
+}
+
 static int run_stage(gta_ctx* c, double* seconds, double* cells) {
     if (!c) return fail(nullptr, GTA_ERR_ARG, "null context");
     if (!c->have_q || !c->have_r) return fail(c, GTA_ERR_STATE, "queries and references must be set first");
@@ -760,7 +776,7 @@ static int run_stage(gta_ctx* c, double* seconds, double* cells) {
         HIPCHK(c, ev_end());                 \
     } while (0)
     int maxaln = std::min(c->q.len.front(), c->r.len.front());
-    RefineLaunch RL{dinv, 0, 0, 0, 0.f, std::min(maxaln, 1536)};
+    RefineLaunch RL{dinv, 0, 0, 0, 0.f, std::min(maxaln, 1536), dprefine_waves(c)};
     // After each filter (the stage-1 prescreen, the -s threshold) the live
     // pairs are listed on the device.  The batched form's launches that follow
     // are persistent grids draining those lists (no host round trip: the
@@ -800,7 +816,9 @@ static int run_stage(gta_ctx* c, double* seconds, double* cells) {
     auto refine = [&](int mode, int epi, int dpi, int maxdpi, float prescore) -> hipError_t {
         RefineLaunch L = RL;
         L.epi = epi; L.dpi = dpi; L.maxndpiters = maxdpi; L.prescore = prescore;
-        return launch_refine_dp(prec, mode, Q, R, S, G, L, rsp, queue(listb, ltot + 1), pgrid_rf, st);
+        // a persistent grid keeps 16 waves per CU whatever the workgroup size
+        const int pg = (mode == RM_DPREFINE_L && L.waves == 1) ? c->ncu * 16 : pgrid_rf;
+        return launch_refine_dp(prec, mode, Q, R, S, G, L, rsp, queue(listb, ltot + 1), pg, st);
     };
     // DPRefine<GAP0,PRESCREEN> (MpStage1.cpp:455-510)
     auto dprefine = [&](int gap0, float prescore, int maxdpi) -> int {

@@ -41,6 +41,11 @@ enum { CCM00 = 0, CVq0 = 9, CVr0 = 12, NALN = 15, NCC = 16 };
 constexpr int REFINE_WAVES = 2;
 // workgroups of the refinements resident per CU (4 waves per SIMD)
 constexpr int REFINE_WGS_PER_CU = 16 / REFINE_WAVES;
+// Alignments staged in LDS by the FP32 refinements (24 B per position) in
+// workgroups of W waves, 16 waves per CU: the workgroups' fixed LDS (just
+// under 2 KB per wave) and staged alignments fit the CU's 160 KB (W = 2: 682
+// positions; W = 1: 341).  Longer alignments are read from global memory.
+constexpr int rlds_cap32(int W) { return (160 * 1024 * W / 16 - 2048 * W) / 24; }
 // waves per workgroup of the seed search (every pair's slots are many and
 // even, so a larger workgroup costs no idle lanes)
 constexpr int SEARCH_WAVES = 4;

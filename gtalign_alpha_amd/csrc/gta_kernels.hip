@@ -80,12 +80,8 @@ __device__ __forceinline__ float GetD0s(float d0) { return fmaxf(fminf(d0, 8.f),
 constexpr int RWAVES = REFINE_WAVES;      // waves per workgroup in the refinement kernels
 constexpr int RTHREADS = RWAVES * GTA_WAVE;
 constexpr int FTHREADS = SEARCH_WAVES * GTA_WAVE;   // threads per workgroup of the seed search
-// The FP32 refinements run REFINE_WGS_PER_CU workgroups (16 waves) per CU:
-// the kernel is held to 128 VGPRs and alignments are staged in LDS (24 B per
-// position) up to RLDS_CAP32 positions, so that the workgroups' fixed LDS
-// (just under 2 KB per wave) and staged alignments fit the CU's 160 KB.
-// Longer alignments are read from global memory.
-constexpr int RLDS_CAP32 = (160 * 1024 / REFINE_WGS_PER_CU - 2048 * REFINE_WAVES) / 24;
+// The FP32 refinements run 16 waves per CU in workgroups of W waves, held to
+// 128 VGPRs, with alignments staged in LDS up to rlds_cap32(W) positions.
 
 __device__ __forceinline__ int pair_q(const DevState& S, int p) { return p / S.nr; }
 __device__ __forceinline__ int pair_r(const DevState& S, int p) { return p % S.nr; }
@@ -297,11 +293,12 @@ __device__ void group_best(const SlotBest& mine, SlotBest* lds, SlotBest& out) {
     for (int i = 1; i < NT / G; i++)
         if (out.beats(lds[i].best, lds[i].slot)) out = lds[i];
 }
-// the same when each group already keeps its record in lds[group]
+// the same when each of the workgroup's NG groups already keeps its record in lds[group]
+template <int NG = NGROUPS>
 __device__ void group_best_lds(const SlotBest* lds, SlotBest& out) {
     __syncthreads();
     out = lds[0];
-    for (int i = 1; i < NGROUPS; i++)
+    for (int i = 1; i < NG; i++)
         if (out.beats(lds[i].best, lds[i].slot)) out = lds[i];
 }
 
@@ -1570,20 +1567,20 @@ __device__ void refine_dp_epilogue(const DevChunk& Q, const DevChunk& R, const D
     }
 }
 
-// LDS of a refinement workgroup
-template <int P>
+// LDS of a refinement workgroup of W waves (W * NGW slot groups)
+template <int P, int W>
 struct RefineLds {
-    SlotBest slds[NGROUPS];
-    float tbest[NGROUPS][12];
-    typename GPrec<P>::T tlast[NGROUPS][16];
-    typename GPrec<P>::T tnext[NGROUPS][16];   // next iteration's covariance from a fused pass
+    SlotBest slds[W * NGW];
+    float tbest[W * NGW][12];
+    typename GPrec<P>::T tlast[W * NGW][16];
+    typename GPrec<P>::T tnext[W * NGW][16];   // next iteration's covariance from a fused pass
     int ctr;
 };
 
-template <int P, int MODE>
+template <int P, int MODE, int W>
 __device__ void refine_pair(const DevChunk& Q, const DevChunk& R, const DevState& S, const Geom& G,
                             const RefineParams& prm, int ldscap, int p, SlotBest* parts, float* lds_rf,
-                            RefineLds<P>& rl) {
+                            RefineLds<P, W>& rl) {
     SlotBest* slds = rl.slds;
     float (*tbest)[12] = rl.tbest;
     typename GPrec<P>::T (*tlast)[16] = rl.tlast;
@@ -1606,7 +1603,7 @@ __device__ void refine_pair(const DevChunk& Q, const DevChunk& R, const DevState
     const size_t pl = S.alnplane;
     const bool staged = !skip && L <= ldscap;
     if (staged) {
-        for (int k = threadIdx.x; k < L; k += RTHREADS) {
+        for (int k = threadIdx.x; k < L; k += W * GTA_WAVE) {
             float* d = lds_rf + 6 * k;
             d[0] = g[k]; d[1] = g[pl + k]; d[2] = g[2 * pl + k];
             d[3] = g[3 * pl + k]; d[4] = g[4 * pl + k]; d[5] = g[5 * pl + k];
@@ -1837,32 +1834,32 @@ __device__ void refine_pair(const DevChunk& Q, const DevChunk& R, const DevState
     }
     if (logm) return;
     SlotBest b;
-    group_best_lds(slds, b);
+    group_best_lds<W * NGW>(slds, b);
     if (threadIdx.x == 0) {
         if (parts) parts[(size_t)blockIdx.x * gridDim.y + blockIdx.y] = b;
         else refine_dp_epilogue<MODE>(Q, R, S, prm, p, b);
     }
 }
 
-// One workgroup per pair (plist[b], or b), each pair's slots split over
-// gridDim.y workgroups when parts is given, or a persistent grid draining a
-// work queue of pairs.
-template <int P, int MODE>
-__global__ __launch_bounds__(RTHREADS) __attribute__((amdgpu_waves_per_eu(P == 0 ? 4 : 1))) void k_refine_dp(DevChunk Q, DevChunk R, DevState S, Geom G,
+// One workgroup of W waves per pair (plist[b], or b), each pair's slots split
+// over gridDim.y workgroups when parts is given, or a persistent grid draining
+// a work queue of pairs.
+template <int P, int MODE, int W = RWAVES>
+__global__ __launch_bounds__(W * GTA_WAVE) __attribute__((amdgpu_waves_per_eu(P == 0 ? 4 : 1))) void k_refine_dp(DevChunk Q, DevChunk R, DevState S, Geom G,
                                                          RefineParams prm, int ldscap, const int* plist,
                                                          SlotBest* parts, WorkQ wq) {
     extern __shared__ float lds_rf[];
-    __shared__ RefineLds<P> rl;
+    __shared__ RefineLds<P, W> rl;
     __shared__ int item;
     if (wq.items) {
         const int n = *wq.count;
         for (int it = wq_next_block(wq, &item); it < n; it = wq_next_block(wq, &item)) {
-            refine_pair<P, MODE>(Q, R, S, G, prm, ldscap, wq.items[it], nullptr, lds_rf, rl);
+            refine_pair<P, MODE, W>(Q, R, S, G, prm, ldscap, wq.items[it], nullptr, lds_rf, rl);
             __syncthreads();
         }
     } else {
-        refine_pair<P, MODE>(Q, R, S, G, prm, ldscap, plist ? plist[blockIdx.x] : (int)blockIdx.x, parts,
-                             lds_rf, rl);
+        refine_pair<P, MODE, W>(Q, R, S, G, prm, ldscap, plist ? plist[blockIdx.x] : (int)blockIdx.x, parts,
+                                lds_rf, rl);
     }
 }
 
@@ -2400,7 +2397,10 @@ hipError_t launch_refine_dp(int prec, int mode, const DevChunk& Q, const DevChun
                             const Geom& G, const RefineLaunch& L, const RefSplit& Sp, const WorkQ& wq,
                             int pgrid, hipStream_t st) {
     RefineParams prm{L.dinv, L.epi, L.dpi, L.maxndpiters, L.prescore};
-    const int ldscap = (prec == 0 || mode == RM_DPREFINE) ? std::min(L.ldscap, RLDS_CAP32) : L.ldscap;
+    // DPRefine runs in workgroups of L.waves (1 or 2), the other modes in 2-wave ones
+    const bool w1 = mode == RM_DPREFINE && L.waves == 1;
+    const int ldscap = (prec == 0 || mode == RM_DPREFINE) ? std::min(L.ldscap, rlds_cap32(w1 ? 1 : RWAVES))
+                                                          : L.ldscap;
     const size_t lds = (size_t)ldscap * 6 * sizeof(float);
     const int nsmall = wq.items ? pgrid : (Sp.small ? Sp.nsmall : S.nq * S.nr);
     const bool phase2 = (mode == RM_FIN2 || mode == RM_PROD2);
@@ -2422,6 +2422,19 @@ hipError_t launch_refine_dp(int prec, int mode, const DevChunk& Q, const DevChun
                                prm, Sp.big, Sp.nbig, np, parts);                                              \
         }                                                                                                     \
     } while (0)
+    // the same with 1-wave workgroups
+#define GTA_RL1(PP, MM)                                                                                        \
+    do {                                                                                                       \
+        if (nsmall > 0)                                                                                        \
+            hipLaunchKernelGGL((k_refine_dp<PP, MM, 1>), dim3(nsmall), dim3(GTA_WAVE), lds, st, Q, R, S, G, prm, \
+                               ldscap, Sp.small, (SlotBest*)nullptr, wq);                                    \
+        if (Sp.nbig > 0) {                                                                                     \
+            hipLaunchKernelGGL((k_refine_dp<PP, MM, 1>), dim3(Sp.nbig, np), dim3(GTA_WAVE), lds, st, Q, R, S, G, \
+                               prm, ldscap, Sp.big, parts, WorkQ{});                                         \
+            hipLaunchKernelGGL((k_merge_parts<2, MM>), dim3((Sp.nbig + 63) / 64), dim3(64), 0, st, Q, R, S,   \
+                               prm, Sp.big, Sp.nbig, np, parts);                                               \
+        }                                                                                                      \
+    } while (0)
     // the log search's steps run in order within one workgroup: no split
 #define GTA_RL_LOG(PP)                                                                                         \
     do {                                                                                                       \
@@ -2434,7 +2447,10 @@ hipError_t launch_refine_dp(int prec, int mode, const DevChunk& Q, const DevChun
     } while (0)
 #define GTA_RLM(PP)                                      \
     switch (mode) {                                      \
-        case RM_DPREFINE: GTA_RL(0, RM_DPREFINE); break;  \
+        case RM_DPREFINE:                                \
+            if (w1) GTA_RL1(0, RM_DPREFINE);             \
+            else GTA_RL(0, RM_DPREFINE);                 \
+            break;                                       \
         case RM_FIN1: GTA_RL(PP, RM_FIN1); break;         \
         case RM_FIN2: GTA_RL(PP, RM_FIN2); break;         \
         case RM_FIN2F: GTA_RL(PP, RM_FIN2F); break;       \
@@ -2446,6 +2462,7 @@ hipError_t launch_refine_dp(int prec, int mode, const DevChunk& Q, const DevChun
     if (prec) { GTA_RLM(1) } else { GTA_RLM(0) }
 #undef GTA_RLM
 #undef GTA_RL_LOG
+#undef GTA_RL1
 #undef GTA_RL
     return hipGetLastError();
 }

@@ -64,6 +64,7 @@ struct RefineLaunch {
     int epi, dpi, maxndpiters;
     float prescore;
     int ldscap;        // alignments up to this length are staged in LDS
+    int waves;         // waves per workgroup of DPRefine (1 or 2; the other modes use 2)
 };
 
 hipError_t launch_secstr(const DevChunk& C, hipStream_t st);
