@@ -51,7 +51,7 @@ KERNEL_NAMES = {
     "findfrag": "gta::k_findfrag<0>",
     "refine_gapless": "gta::k_refine_gapless<0>",
     "dp": "gta::k_dp<false> (29 of 30 launches; 1 is k_dp<true>)",
-    "refine_dp": "gta::k_refine_dp<0, 0> (RM_DPREFINE)",
+    "refine_dp": "gta::k_refine_dp<0, 0, 1> (RM_DPREFINE, 1-wave workgroups on the bench's chunks)",
     "refine_final": "gta::k_refine_dp<0, 1..4> (final and production phases)",
 }
 
