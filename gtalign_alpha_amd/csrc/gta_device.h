@@ -188,16 +188,20 @@ __device__ __forceinline__ int wave_sum_i(int v) {
 }
 
 // three smallest values of a multiset, kept sorted per lane
-// (a <= b <= c: the lowest three of {a, b, c, d} by a min/max network, the
-// same triple as sorted insertion, without branches)
+// (a <= b <= c: the lowest three of {a, b, c, d} are min(a, d), the median of
+// {a, b, d} and the median of {b, c, d}, the same triple as sorted insertion;
+// three v_med3_f32 without branches, and, unlike fminf/fmaxf, without the
+// operand canonicalisations the compiler adds for IEEE mode.  Distances are
+// never NaN, so the medians equal the min/max network's values)
 struct Min3 {
     float a, b, c;
     __device__ void init() { a = b = c = CP_LARGEDST; }
     __device__ __forceinline__ void put(float d) {
-        const float ha = fmaxf(a, d), hb = fmaxf(b, d);
-        c = fminf(c, hb);
-        b = fminf(b, ha);
-        a = fminf(a, d);
+        const float na = __builtin_amdgcn_fmed3f(a, d, -__builtin_inff());
+        const float nb = __builtin_amdgcn_fmed3f(a, b, d);
+        c = __builtin_amdgcn_fmed3f(b, c, d);
+        b = nb;
+        a = na;
     }
 };
 // three smallest distinct values (a value already held changes nothing: it
@@ -207,10 +211,11 @@ struct Min3d {
     __device__ void init() { a = b = c = CP_LARGEDST; }
     __device__ __forceinline__ void put(float d) {
         if (d == a || d == b || d == c) d = c;
-        const float ha = fmaxf(a, d), hb = fmaxf(b, d);
-        c = fminf(c, hb);
-        b = fminf(b, ha);
-        a = fminf(a, d);
+        const float na = __builtin_amdgcn_fmed3f(a, d, -__builtin_inff());
+        const float nb = __builtin_amdgcn_fmed3f(a, b, d);
+        c = __builtin_amdgcn_fmed3f(b, c, d);
+        b = nb;
+        a = na;
     }
 };
 
