@@ -917,8 +917,11 @@ __device__ __forceinline__ bool dp_strip_impl(const DevChunk& Q, const DevChunk&
     // one step (anti-diagonal); false when a row wait gave up.  In an inner
     // step every lane's cell is in the matrix, so the bounds test (and the
     // column counter it reads) drops out.
-    auto step = [&](const int t, auto inner) -> bool {
-        constexpr bool INNER = decltype(inner)::value;
+    auto step = [&](const int t, auto kind) -> bool {
+        // kind 0: edge step; 1: inner step; 2: inner step of a strip whose
+        // query rows and reference columns all hold defined coordinates
+        constexpr bool INNER = decltype(kind)::value >= 1;
+        constexpr bool DEFINED = decltype(kind)::value == 2;
         const float4 rc = cur;
         const float rss = curs;
         if ((t & 63) == 63) {
@@ -966,7 +969,7 @@ __device__ __forceinline__ bool dp_strip_impl(const DevChunk& Q, const DevChunk&
             nv = (bk != BT_DIAG) ? v : __fadd_rn(v, DPSS_INIT);
         } else {
             float sc = pair_score(d02, dist2(qx, qy, qz, rc.x, rc.y, rc.z));
-            sc = (qok && DEFCOORD_RFN_cmp < rc.x) ? sc : 0.0f;
+            if (!DEFINED) sc = (qok && DEFCOORD_RFN_cmp < rc.x) ? sc : 0.0f;
             if (USESS) sc = __fadd_rn(sc, (qss == rss) ? ssw : 0.0f);
             v = __fadd_rn(sc, GAP0 ? diag : fabsf(diag));
             float w = val;
@@ -1001,20 +1004,37 @@ __device__ __forceinline__ bool dp_strip_impl(const DevChunk& Q, const DevChunk&
     };
     // two steps per iteration, so that the prefetched ring entry and the
     // carried values alternate registers instead of being copied every step
-    const std::integral_constant<bool, false> edge;
-    const std::integral_constant<bool, true> inner;
+    const std::integral_constant<int, 0> edge;
+    const std::integral_constant<int, 1> inner;
+    const std::integral_constant<int, 2> inner_defined;
     // inner steps: a full strip, steps 64 .. (Lr & ~1) - 1 (even bounds keep the pairs)
     const bool full = (s + 1) << 6 <= Lq;
     const int tin = full ? 64 : nsteps, tout = full ? max(64, Lr & ~1) : nsteps;
+    // a full strip whose 64 query rows and Lr reference columns are all
+    // defined scores every inner cell without the undefined-coordinate select
+    // (the select would keep the score in every such cell)
+    bool defined = false;
+    if (!SSRR && full) {
+        bool d = qok;
+        for (int c = lane; c < Lr; c += GTA_WAVE) d = d && DEFCOORD_RFN_cmp < Rx[c];
+        defined = __all(d);
+    }
     int t = 0;
     for (; t + 1 < min(tin, nsteps); t += 2) {
         if (!step(t, edge)) return false;
         if (!step(t + 1, edge)) return false;
     }
     if (t < tout) {
-        for (; t + 1 < tout; t += 2) {
-            if (!step(t, inner)) return false;
-            if (!step(t + 1, inner)) return false;
+        if (defined) {
+            for (; t + 1 < tout; t += 2) {
+                if (!step(t, inner_defined)) return false;
+                if (!step(t + 1, inner_defined)) return false;
+            }
+        } else {
+            for (; t + 1 < tout; t += 2) {
+                if (!step(t, inner)) return false;
+                if (!step(t + 1, inner)) return false;
+            }
         }
         j = t - lane;   // the column counter as the edge steps would have left it
     }
@@ -1146,7 +1166,8 @@ __device__ void dp_pair(const DevChunk& Q, const DevChunk& R, const DevState& S,
 // a work queue, the waves of a persistent grid take queued pairs until the
 // queue is drained.
 template <int SC, bool GAP0>
-__global__ __launch_bounds__(GTA_WAVE) void k_dp(DevChunk Q, DevChunk R, DevState S, DpParams prm,
+__global__ __launch_bounds__(GTA_WAVE) __attribute__((amdgpu_waves_per_eu(SC == SC_TMSS ? 6 : 7))) void k_dp(
+    DevChunk Q, DevChunk R, DevState S, DpParams prm,
                                                   const int* plist, WorkQ wq) {
     __shared__ float4 ring[DPRING];
     __shared__ float ssring[SC == SC_TMSS ? DPRING : 1];
