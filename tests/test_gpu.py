@@ -516,6 +516,34 @@ def test_dp_kernel_bit_exact(gap0, gapopen, usess, dpscan):
             assert np.array_equal(got[i, j], want), (i, j, Lq, Lr)
 
 
+@pytest.mark.parametrize("gap0,gapopen", [(True, 0.0), (False, -0.6)], ids=["gap0", "gap-0.6"])
+def test_dp_kernel_undefined_coordinates(gap0, gapopen):
+    """Cells of an undefined query row (transformed x >= 99,990) or reference
+    column (x <= -99,990) score 0 (mpdpbase.h:26-29).  Full 64-row strips
+    with such a row or column take the inner steps with the select, the
+    others the inner steps without it; both must walk to the oracle's matched
+    positions."""
+    q, r, tf = _dp_workload(7, 3, 10, 3, 400)
+    for i in range(q.nstrs):
+        if q.len[i] >= 64:
+            q.x[int(q.dst[i]) + 5] = 1.0e5
+    for j in range(r.nstrs):
+        if j % 2 == 1 and r.len[j] >= 100:
+            r.x[int(r.dst[j]) + 70] = -1.0e5
+    assert (q.len >= 128).any() and (r.len >= 128).any()
+    with GtaBatch(0, default_options(), max_qrys=q.nstrs, max_qry_poss=q.nposs, max_refs=r.nstrs,
+                  max_ref_poss=r.nposs) as b:
+        b.TransferQueryPMDataToDevice(q)
+        b.TransferCPMDataToDevice(r)
+        got = b.dp_under(tf, gap0, gapopen, False, False)
+    for i in range(q.nstrs):
+        for j in range(r.nstrs):
+            Lq, Lr = int(q.len[i]), int(r.len[j])
+            d02 = float(oracle.d0(Lq, Lr)["d0"]) ** 2
+            _, want = oracle.dp(q.coords(i), r.coords(j), tf[i, j], d02, gapopen, gap0)
+            assert np.array_equal(got[i, j], want), (i, j, Lq, Lr)
+
+
 GRID_SSRR_SKIP = ("the device-wide (grid) form of the SS/residue fill stalled on the GPU with no cause "
                   "found; the library routes those pairs to 4-wave workgroups and this case stays off")
 
