@@ -1012,11 +1012,15 @@ __device__ __forceinline__ bool dp_strip_impl(const DevChunk& Q, const DevChunk&
     const int tin = full ? 64 : nsteps, tout = full ? max(64, Lr & ~1) : nsteps;
     // a full strip whose 64 query rows and Lr reference columns are all
     // defined scores every inner cell without the undefined-coordinate select
-    // (the select would keep the score in every such cell)
+    // (the select would keep the score in every such cell).  The check reads
+    // the reference's x once per strip (independent loads, no early exit);
+    // references longer than 4,096 residues, whose strips the device-wide
+    // pipeline runs back to back, keep the select instead.
     bool defined = false;
-    if (!SSRR && full) {
+    if (!SSRR && full && Lr <= 4096) {
         bool d = qok;
-        for (int c = lane; c < Lr; c += GTA_WAVE) d = d && DEFCOORD_RFN_cmp < Rx[c];
+#pragma unroll 4
+        for (int c = lane; c < Lr; c += GTA_WAVE) d &= DEFCOORD_RFN_cmp < Rx[c];
         defined = __all(d);
     }
     int t = 0;
